@@ -1,0 +1,201 @@
+"""ctypes binding of libadsgpu.so (the C ABI in include/adsgpu.h).
+
+The Python side is a thin driver for tests and the benchmark; the product is the C ABI
+and the C++ host surface in iga-ads_amd/host/ads_gpu/simulation.hpp. Importing this
+module loads the HIP library and raises if it has not been built: there is no CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libadsgpu.so")
+
+OK, EINVAL, EHIP, ENOMEM, ESTATE, ENONFINITE = 0, -1, -2, -3, -4, -5
+FP64, FP32 = 0, 1
+U, U_PREV, BUFFER, KFIELD, SOURCE = 0, 1, 2, 3, 4
+STEP_DIRICHLET = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class DimConfig(C.Structure):
+    _fields_ = [("p", C.c_int), ("elements", C.c_int), ("a", C.c_double), ("b", C.c_double),
+                ("quad_order", C.c_int), ("repeated_nodes", C.c_int)]
+
+
+class Config(C.Structure):
+    _fields_ = [("dim", C.c_int), ("x", DimConfig), ("y", DimConfig), ("z", DimConfig),
+                ("step_count", C.c_int), ("dt", C.c_double), ("derivatives", C.c_int),
+                ("precision", C.c_int)]
+
+
+class RhsForm(C.Structure):
+    _fields_ = [("mass", C.c_double), ("dt", C.c_double), ("grad", C.c_double * 3),
+                ("reaction", C.c_double), ("use_kfield", C.c_int), ("use_source", C.c_int)]
+
+
+def rhs_form(dt, grad=(1.0, 1.0, 1.0), mass=1.0, reaction=0.0, kfield=False, source=False):
+    f = RhsForm()
+    f.mass = mass
+    f.dt = dt
+    for i in range(3):
+        f.grad[i] = grad[i]
+    f.reaction = reaction
+    f.use_kfield = int(bool(kfield))
+    f.use_source = int(bool(source))
+    return f
+
+
+class AdsgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+EXPORTS = [
+    "adsg_create", "adsg_destroy", "adsg_last_error", "adsg_dofs", "adsg_size", "adsg_precision",
+    "adsg_fix_dof", "adsg_set_matrix", "adsg_set_band_matrix", "adsg_prepare_matrices",
+    "adsg_set_vector", "adsg_get_vector", "adsg_device_ptr", "adsg_swap", "adsg_compute_rhs",
+    "adsg_zero_boundary", "adsg_ads_solve", "adsg_step", "adsg_synchronize",
+    "adsg_band_factorize",
+]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AdsgError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        L.adsg_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+        L.adsg_destroy.argtypes = [C.c_void_p]
+        L.adsg_destroy.restype = None
+        L.adsg_last_error.argtypes = [C.c_void_p]
+        L.adsg_last_error.restype = C.c_char_p
+        L.adsg_dofs.argtypes = [C.c_void_p, C.c_int]
+        L.adsg_size.argtypes = [C.c_void_p]
+        L.adsg_size.restype = C.c_long
+        L.adsg_precision.argtypes = [C.c_void_p]
+        L.adsg_fix_dof.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.adsg_set_matrix.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.adsg_set_band_matrix.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.adsg_prepare_matrices.argtypes = [C.c_void_p]
+        L.adsg_set_vector.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.adsg_get_vector.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.adsg_device_ptr.argtypes = [C.c_void_p, C.c_int]
+        L.adsg_device_ptr.restype = C.c_void_p
+        L.adsg_swap.argtypes = [C.c_void_p]
+        L.adsg_compute_rhs.argtypes = [C.c_void_p, C.POINTER(RhsForm)]
+        L.adsg_zero_boundary.argtypes = [C.c_void_p]
+        L.adsg_ads_solve.argtypes = [C.c_void_p, _ip]
+        L.adsg_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), C.c_int]
+        L.adsg_synchronize.argtypes = [C.c_void_p]
+        L.adsg_band_factorize.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip]
+        _lib = L
+    return _lib
+
+
+class Simulation:
+    """One adsg_sim handle: the simulation_2d / simulation_3d surface on the GPU."""
+
+    def __init__(self, dim, p, elements, dt=0.0, steps=0, precision=FP64, a=None, b=None):
+        L = lib()
+        cfg = Config()
+        cfg.dim = dim
+        ps = list(p) if hasattr(p, "__len__") else [p] * dim
+        es = list(elements) if hasattr(elements, "__len__") else [elements] * dim
+        for i, dc in enumerate((cfg.x, cfg.y, cfg.z)):
+            if i < dim:
+                dc.p = ps[i]
+                dc.elements = es[i]
+                dc.a = a[i] if a else 0.0
+                dc.b = b[i] if b else 1.0
+                dc.quad_order = 0
+                dc.repeated_nodes = 0
+        cfg.step_count = steps
+        cfg.dt = dt
+        cfg.derivatives = 1
+        cfg.precision = precision
+        h = C.c_void_p()
+        rc = L.adsg_create(C.byref(cfg), C.byref(h))
+        if rc != OK:
+            raise AdsgError(f"adsg_create: {rc}: {L.adsg_last_error(None).decode()}")
+        self.h = h
+        self.dim = dim
+        self.precision = precision
+        self.shape = tuple(L.adsg_dofs(h, d) for d in range(dim))
+        self.size = int(L.adsg_size(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().adsg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != OK:
+            msg = lib().adsg_last_error(self.h).decode()
+            raise AdsgError(f"{what}: {rc}: {msg}")
+        return rc
+
+    def fix_dof(self, d, k):
+        self._check(lib().adsg_fix_dof(self.h, d, k), "fix_dof")
+
+    def fix_all_boundaries(self):
+        for d in range(self.dim):
+            self.fix_dof(d, 0)
+            self.fix_dof(d, self.shape[d] - 1)
+
+    def set_matrix(self, d, slot, mass, stiff):
+        self._check(lib().adsg_set_matrix(self.h, d, slot, mass, stiff), "set_matrix")
+
+    def set_band_matrix(self, d, slot, kl, ku, ab):
+        ab = np.ascontiguousarray(ab, dtype=np.float64)
+        self._check(lib().adsg_set_band_matrix(self.h, d, slot, kl, ku, ab.ctypes.data_as(_dp)),
+                    "set_band_matrix")
+
+    def prepare_matrices(self):
+        return self._check(lib().adsg_prepare_matrices(self.h), "prepare_matrices")
+
+    def set(self, which, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        assert v.size == self.size
+        self._check(lib().adsg_set_vector(self.h, which, v.ctypes.data_as(_dp)), "set_vector")
+
+    def get(self, which=U):
+        out = np.zeros(self.size)
+        self._check(lib().adsg_get_vector(self.h, which, out.ctypes.data_as(_dp)), "get_vector")
+        return out
+
+    def swap(self):
+        self._check(lib().adsg_swap(self.h), "swap")
+
+    def compute_rhs(self, form):
+        self._check(lib().adsg_compute_rhs(self.h, C.byref(form)), "compute_rhs")
+
+    def zero_boundary(self):
+        self._check(lib().adsg_zero_boundary(self.h), "zero_boundary")
+
+    def ads_solve(self, slots=None):
+        if slots is None:
+            self._check(lib().adsg_ads_solve(self.h, None), "ads_solve")
+        else:
+            s = (C.c_int * 3)(*(list(slots) + [0] * (3 - len(slots))))
+            self._check(lib().adsg_ads_solve(self.h, s), "ads_solve")
+
+    def step(self, n, form, flags=0):
+        self._check(lib().adsg_step(self.h, n, C.byref(form), flags), "step")
+
+    def synchronize(self):
+        self._check(lib().adsg_synchronize(self.h), "synchronize")
+
+    def device_ptr(self, which):
+        return lib().adsg_device_ptr(self.h, which)

@@ -1,0 +1,539 @@
+// C ABI of the MI355X ADS step (declared in include/adsgpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/adsgpu.h"
+#include "adsg_kernels.hpp"
+#include "adsg_setup.hpp"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct dev_slot {
+    bool set = false;
+    bool factored = false;
+    int kl = 0, ku = 0;
+    std::vector<double> ab;  // unfactored band, column size 2kl+ku+1
+    adsg::line_factors lf;
+    // device factors in the precision of the sweeps
+    void* lower = nullptr;
+    void* upper = nullptr;
+    void* rdiag = nullptr;
+    int* poff = nullptr;
+};
+
+struct dev_dir {
+    adsg::direction tab;
+    std::vector<dev_slot> slots;
+    void* val = nullptr;  // [elements][nq][p+1] in the step precision
+    void* der = nullptr;
+    void* wj = nullptr;
+};
+
+}  // namespace
+
+struct adsg_sim {
+    int dim = 0;
+    int precision = ADSG_FP64;
+    int step_count = 0;
+    double dt = 0;
+    dev_dir dirs[3];
+    long N = 0;
+    hipStream_t stream = nullptr;
+    void* vec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool prepared = false;
+    std::string err;
+    // captured step graphs keyed by (form, flags, u pointer, steps in graph)
+    std::map<std::tuple<std::string, int, void*, int>, hipGraphExec_t> graphs;
+};
+
+namespace {
+
+size_t elt(const adsg_sim* h) { return h->precision == ADSG_FP32 ? sizeof(float) : sizeof(double); }
+
+int fail(adsg_sim* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+int hip_fail(adsg_sim* h, hipError_t e, const char* what) {
+    return fail(h, ADSG_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(h, expr)                                      \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) return hip_fail((h), e_, #expr); \
+    } while (0)
+
+template <typename T>
+int upload(adsg_sim* h, void** dst, const std::vector<double>& src) {
+    if (*dst) {
+        (void)hipFree(*dst);
+        *dst = nullptr;
+    }
+    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    if (hipMalloc(dst, bytes) != hipSuccess) return fail(h, ADSG_ENOMEM, "hipMalloc failed for tables");
+    std::vector<T> tmp(std::max<size_t>(src.size(), 1), T(0));
+    for (size_t i = 0; i < src.size(); ++i) tmp[i] = static_cast<T>(src[i]);
+    HIP_TRY(h, hipMemcpy(*dst, tmp.data(), bytes, hipMemcpyHostToDevice));
+    return ADSG_OK;
+}
+
+int upload_tables(adsg_sim* h, dev_dir& d) {
+    int rc;
+    if (h->precision == ADSG_FP32) {
+        if ((rc = upload<float>(h, &d.val, d.tab.val))) return rc;
+        if ((rc = upload<float>(h, &d.der, d.tab.der))) return rc;
+        if ((rc = upload<float>(h, &d.wj, d.tab.wj))) return rc;
+    } else {
+        if ((rc = upload<double>(h, &d.val, d.tab.val))) return rc;
+        if ((rc = upload<double>(h, &d.der, d.tab.der))) return rc;
+        if ((rc = upload<double>(h, &d.wj, d.tab.wj))) return rc;
+    }
+    return ADSG_OK;
+}
+
+void free_slot(dev_slot& s) {
+    for (void** p : {&s.lower, &s.upper, &s.rdiag}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (s.poff) (void)hipFree(s.poff);
+    s.poff = nullptr;
+}
+
+void drop_graphs(adsg_sim* h) {
+    for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    h->graphs.clear();
+}
+
+bool valid_dir(const adsg_sim* h, int dir) { return h && dir >= 0 && dir < h->dim; }
+
+adsg::line_geometry geometry(const adsg_sim* h, int dir) {
+    const long nx = h->dirs[0].tab.dofs, ny = h->dirs[1].tab.dofs;
+    const long nz = h->dim == 3 ? h->dirs[2].tab.dofs : 1;
+    adsg::line_geometry g;
+    if (dir == 0) {
+        g.n_lines = ny * nz;
+        g.S = 1;
+    } else if (dir == 1) {
+        g.n_lines = nx * nz;
+        g.A = nx;
+        g.B = nx * ny;
+        g.S = nx;
+    } else {
+        g.n_lines = nx * ny;
+        g.A = nx * ny;
+        g.B = 0;
+        g.S = nx * ny;
+    }
+    return g;
+}
+
+template <typename T>
+int sweep_all(adsg_sim* h, T* u, const int* slots) {
+    for (int d = 0; d < h->dim; ++d) {
+        const int sl = slots ? slots[d] : 0;
+        if (sl < 0 || sl >= static_cast<int>(h->dirs[d].slots.size()) || !h->dirs[d].slots[sl].factored)
+            return fail(h, ADSG_ESTATE, "ads_solve: matrix slot not set or not factored");
+        const dev_slot& s = h->dirs[d].slots[sl];
+        adsg::dev_factors<T> f;
+        f.lower = static_cast<const T*>(s.lower);
+        f.upper = static_cast<const T*>(s.upper);
+        f.rdiag = static_cast<const T*>(s.rdiag);
+        f.poff = s.poff;
+        f.n = s.lf.n;
+        f.kl = s.lf.kl;
+        f.ku = s.lf.ku_eff;
+        f.pivoted = s.lf.pivoted;
+        hipError_t e = adsg::launch_line_sweep<T, T>(u, geometry(h, d), f, h->stream);
+        if (e == hipErrorInvalidValue)
+            return fail(h, ADSG_EINVAL, "ads_solve: unsupported band widths (kl, ku)");
+        if (e != hipSuccess) return hip_fail(h, e, "line sweep");
+    }
+    return ADSG_OK;
+}
+
+template <typename T>
+int rhs_impl(adsg_sim* h, const adsg_rhs_form* form) {
+    adsg::dev_tables<T> t[3];
+    for (int d = 0; d < h->dim; ++d) {
+        t[d].val = static_cast<const T*>(h->dirs[d].val);
+        t[d].der = static_cast<const T*>(h->dirs[d].der);
+        t[d].wj = static_cast<const T*>(h->dirs[d].wj);
+        t[d].elements = h->dirs[d].tab.elements;
+        t[d].dofs = h->dirs[d].tab.dofs;
+    }
+    adsg::rhs_params<T> prm;
+    prm.mass = static_cast<T>(form->mass);
+    prm.dt = static_cast<T>(form->dt);
+    prm.gx = static_cast<T>(form->grad[0]);
+    prm.gy = static_cast<T>(form->grad[1]);
+    prm.gz = static_cast<T>(form->grad[2]);
+    prm.reaction = static_cast<T>(form->reaction);
+    if (form->use_kfield) {
+        if (!h->vec[ADSG_KFIELD]) return fail(h, ADSG_ESTATE, "compute_rhs: k field not set");
+        prm.kfield = static_cast<const T*>(h->vec[ADSG_KFIELD]);
+    }
+    if (form->use_source) {
+        if (!h->vec[ADSG_SOURCE]) return fail(h, ADSG_ESTATE, "compute_rhs: source not set");
+        prm.source = static_cast<const T*>(h->vec[ADSG_SOURCE]);
+    }
+    T* rhs = static_cast<T*>(h->vec[ADSG_U]);
+    const T* up = static_cast<const T*>(h->vec[ADSG_U_PREV]);
+    hipError_t e = adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
+    if (e != hipSuccess) return hip_fail(h, e, "zero rhs");
+    e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
+    return ADSG_OK;
+}
+
+int zero_boundary_impl(adsg_sim* h) {
+    const int nx = h->dirs[0].tab.dofs, ny = h->dirs[1].tab.dofs;
+    const int nz = h->dim == 3 ? h->dirs[2].tab.dofs : 1;
+    hipError_t e = h->precision == ADSG_FP32
+                       ? adsg::launch_zero_boundary<float>(h->dim, static_cast<float*>(h->vec[ADSG_U]), nx, ny, nz, h->stream)
+                       : adsg::launch_zero_boundary<double>(h->dim, static_cast<double*>(h->vec[ADSG_U]), nx, ny, nz, h->stream);
+    return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "zero boundary");
+}
+
+int solve_impl(adsg_sim* h, const int* slots) {
+    if (h->precision == ADSG_FP32) return sweep_all<float>(h, static_cast<float*>(h->vec[ADSG_U]), slots);
+    return sweep_all<double>(h, static_cast<double*>(h->vec[ADSG_U]), slots);
+}
+
+int rhs_dispatch(adsg_sim* h, const adsg_rhs_form* form) {
+    return h->precision == ADSG_FP32 ? rhs_impl<float>(h, form) : rhs_impl<double>(h, form);
+}
+
+// one explicit step as validation.hpp:73-81 (before_step swap, compute_rhs, solve)
+int one_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
+    std::swap(h->vec[ADSG_U], h->vec[ADSG_U_PREV]);
+    int rc = rhs_dispatch(h, form);
+    if (rc) return rc;
+    if (flags & ADSG_STEP_DIRICHLET) {
+        rc = zero_boundary_impl(h);
+        if (rc) return rc;
+    }
+    return solve_impl(h, nullptr);
+}
+
+std::string form_key(const adsg_rhs_form* f) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%a,%a,%a,%a,%a,%a,%d,%d", f->mass, f->dt, f->grad[0], f->grad[1],
+                  f->grad[2], f->reaction, f->use_kfield, f->use_source);
+    return buf;
+}
+
+int get_graph(adsg_sim* h, const adsg_rhs_form* form, int flags, int steps, hipGraphExec_t* out) {
+    auto key = std::make_tuple(form_key(form), flags, h->vec[ADSG_U], steps);
+    auto it = h->graphs.find(key);
+    if (it != h->graphs.end()) {
+        *out = it->second;
+        return ADSG_OK;
+    }
+    void* u0 = h->vec[ADSG_U];
+    void* p0 = h->vec[ADSG_U_PREV];
+    HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ADSG_OK;
+    for (int s = 0; s < steps && rc == ADSG_OK; ++s) rc = one_step(h, form, flags);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->vec[ADSG_U] = u0;  // capture only recorded the swaps; restore the logical state
+    h->vec[ADSG_U_PREV] = p0;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
+    h->graphs[key] = ex;
+    *out = ex;
+    return ADSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* adsg_last_error(const adsg_sim* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int adsg_create(const adsg_config* cfg, adsg_sim** out) {
+    if (!cfg || !out) {
+        g_create_error = "adsg_create: null argument";
+        return ADSG_EINVAL;
+    }
+    *out = nullptr;
+    if (cfg->dim != 2 && cfg->dim != 3) {
+        g_create_error = "adsg_create: dim must be 2 or 3";
+        return ADSG_EINVAL;
+    }
+    if (cfg->precision != ADSG_FP64 && cfg->precision != ADSG_FP32) {
+        g_create_error = "adsg_create: precision must be ADSG_FP64 or ADSG_FP32";
+        return ADSG_EINVAL;
+    }
+    auto* h = new adsg_sim;
+    h->dim = cfg->dim;
+    h->precision = cfg->precision;
+    h->step_count = cfg->step_count;
+    h->dt = cfg->dt;
+    const adsg_dim_config* dc[3] = {&cfg->x, &cfg->y, &cfg->z};
+    h->N = 1;
+    for (int d = 0; d < h->dim; ++d) {
+        std::string err;
+        if (!adsg::build_direction(h->dirs[d].tab, dc[d]->p, dc[d]->elements, dc[d]->a, dc[d]->b,
+                                   dc[d]->quad_order, dc[d]->repeated_nodes, err)) {
+            g_create_error = "adsg_create: direction " + std::to_string(d) + ": " + err;
+            delete h;
+            return ADSG_EINVAL;
+        }
+        if (dc[d]->p != dc[0]->p) {
+            g_create_error = "adsg_create: the element kernels need the same degree in every direction";
+            delete h;
+            return ADSG_EINVAL;
+        }
+        h->N *= h->dirs[d].tab.dofs;
+        dev_slot s0;
+        s0.set = true;
+        s0.kl = s0.ku = dc[d]->p;
+        s0.ab = adsg::band_form(h->dirs[d].tab, 1.0, 0.0);  // gram_matrix_1d (dimension.cpp:17)
+        h->dirs[d].slots.push_back(std::move(s0));
+    }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "adsg_create: hipStreamCreate failed (no HIP device?)";
+        delete h;
+        return ADSG_EHIP;
+    }
+    for (int d = 0; d < h->dim; ++d) {
+        if (upload_tables(h, h->dirs[d])) {
+            g_create_error = "adsg_create: " + h->err;
+            adsg_destroy(h);
+            return ADSG_EHIP;
+        }
+    }
+    for (int v = ADSG_U; v <= ADSG_BUFFER; ++v) {
+        if (hipMalloc(&h->vec[v], h->N * elt(h)) != hipSuccess) {
+            g_create_error = "adsg_create: device allocation of the state vectors failed";
+            adsg_destroy(h);
+            return ADSG_ENOMEM;
+        }
+        if (hipMemsetAsync(h->vec[v], 0, h->N * elt(h), h->stream) != hipSuccess) {
+            g_create_error = "adsg_create: clearing the state vectors failed";
+            adsg_destroy(h);
+            return ADSG_EHIP;
+        }
+    }
+    (void)hipStreamSynchronize(h->stream);
+    *out = h;
+    return ADSG_OK;
+}
+
+void adsg_destroy(adsg_sim* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graphs(h);
+    for (auto& d : h->dirs) {
+        for (auto& s : d.slots) free_slot(s);
+        for (void** p : {&d.val, &d.der, &d.wj})
+            if (*p) (void)hipFree(*p);
+    }
+    for (auto& v : h->vec)
+        if (v) (void)hipFree(v);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int adsg_dofs(const adsg_sim* h, int dir) { return valid_dir(h, dir) ? h->dirs[dir].tab.dofs : ADSG_EINVAL; }
+long adsg_size(const adsg_sim* h) { return h ? h->N : ADSG_EINVAL; }
+int adsg_precision(const adsg_sim* h) { return h ? h->precision : ADSG_EINVAL; }
+
+int adsg_fix_dof(adsg_sim* h, int dir, int k) {
+    if (!valid_dir(h, dir)) return fail(h, ADSG_EINVAL, "fix_dof: bad direction");
+    dev_dir& d = h->dirs[dir];
+    if (k < 0 || k >= d.tab.dofs) return fail(h, ADSG_EINVAL, "fix_dof: dof out of range");
+    dev_slot& s = d.slots[0];
+    if (s.kl != d.tab.p || s.ku != d.tab.p)
+        return fail(h, ADSG_EINVAL, "fix_dof: slot 0 is a user band matrix");
+    adsg::band_fix_dof(s.ab, d.tab.dofs, d.tab.p, k);
+    s.factored = false;
+    h->prepared = false;
+    drop_graphs(h);
+    return ADSG_OK;
+}
+
+int adsg_set_matrix(adsg_sim* h, int dir, int slot, double mass, double stiff) {
+    if (!valid_dir(h, dir)) return fail(h, ADSG_EINVAL, "set_matrix: bad direction");
+    if (slot < 1 || slot > 64) return fail(h, ADSG_EINVAL, "set_matrix: slot must be in 1..64");
+    dev_dir& d = h->dirs[dir];
+    if (static_cast<int>(d.slots.size()) <= slot) d.slots.resize(slot + 1);
+    dev_slot& s = d.slots[slot];
+    free_slot(s);
+    s = dev_slot{};
+    s.set = true;
+    s.kl = s.ku = d.tab.p;
+    s.ab = adsg::band_form(d.tab, mass, stiff);
+    h->prepared = false;
+    drop_graphs(h);
+    return ADSG_OK;
+}
+
+int adsg_set_band_matrix(adsg_sim* h, int dir, int slot, int kl, int ku, const double* ab) {
+    if (!valid_dir(h, dir) || !ab) return fail(h, ADSG_EINVAL, "set_band_matrix: bad argument");
+    if (slot < 0 || slot > 64) return fail(h, ADSG_EINVAL, "set_band_matrix: slot must be in 0..64");
+    if (kl < 1 || kl > 3 || ku < 1 || ku > 3)
+        return fail(h, ADSG_EINVAL, "set_band_matrix: kl and ku must be in 1..3");
+    dev_dir& d = h->dirs[dir];
+    if (static_cast<int>(d.slots.size()) <= slot) d.slots.resize(slot + 1);
+    dev_slot& s = d.slots[slot];
+    free_slot(s);
+    s = dev_slot{};
+    s.set = true;
+    s.kl = kl;
+    s.ku = ku;
+    const size_t n = static_cast<size_t>(2 * kl + ku + 1) * d.tab.dofs;
+    s.ab.assign(ab, ab + n);
+    h->prepared = false;
+    drop_graphs(h);
+    return ADSG_OK;
+}
+
+int adsg_prepare_matrices(adsg_sim* h) {
+    if (!h) return ADSG_EINVAL;
+    int first_info = 0;
+    for (int dd = 0; dd < h->dim; ++dd) {
+        dev_dir& d = h->dirs[dd];
+        for (auto& s : d.slots) {
+            if (!s.set || s.factored) continue;
+            int info = adsg::factor_for_lines(s.ab, d.tab.dofs, s.kl, s.ku, s.lf);
+            if (info != 0) {
+                if (!first_info) first_info = info;
+                continue;
+            }
+            free_slot(s);
+            int rc;
+            if (h->precision == ADSG_FP32) {
+                if ((rc = upload<float>(h, &s.lower, s.lf.lower))) return rc;
+                if ((rc = upload<float>(h, &s.upper, s.lf.upper))) return rc;
+                if ((rc = upload<float>(h, &s.rdiag, s.lf.rdiag))) return rc;
+            } else {
+                if ((rc = upload<double>(h, &s.lower, s.lf.lower))) return rc;
+                if ((rc = upload<double>(h, &s.upper, s.lf.upper))) return rc;
+                if ((rc = upload<double>(h, &s.rdiag, s.lf.rdiag))) return rc;
+            }
+            if (hipMalloc(reinterpret_cast<void**>(&s.poff), sizeof(int) * s.lf.n) != hipSuccess)
+                return fail(h, ADSG_ENOMEM, "hipMalloc failed for pivots");
+            HIP_TRY(h, hipMemcpy(s.poff, s.lf.poff.data(), sizeof(int) * s.lf.n, hipMemcpyHostToDevice));
+            s.factored = true;
+        }
+    }
+    h->prepared = first_info == 0;
+    drop_graphs(h);
+    if (first_info) return fail(h, first_info, "prepare_matrices: zero pivot (dgbtrf info > 0)");
+    return ADSG_OK;
+}
+
+int adsg_set_vector(adsg_sim* h, int which, const double* host) {
+    if (!h || !host || which < ADSG_U || which > ADSG_SOURCE) return fail(h, ADSG_EINVAL, "set_vector: bad argument");
+    if (!h->vec[which]) {
+        if (hipMalloc(&h->vec[which], h->N * elt(h)) != hipSuccess)
+            return fail(h, ADSG_ENOMEM, "set_vector: device allocation failed");
+        drop_graphs(h);
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->precision == ADSG_FP32) {
+        std::vector<float> tmp(h->N);
+        for (long i = 0; i < h->N; ++i) tmp[i] = static_cast<float>(host[i]);
+        HIP_TRY(h, hipMemcpy(h->vec[which], tmp.data(), h->N * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(h, hipMemcpy(h->vec[which], host, h->N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return ADSG_OK;
+}
+
+int adsg_get_vector(adsg_sim* h, int which, double* host) {
+    if (!h || !host || which < ADSG_U || which > ADSG_SOURCE || !h->vec[which])
+        return fail(h, ADSG_EINVAL, "get_vector: bad argument");
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->precision == ADSG_FP32) {
+        std::vector<float> tmp(h->N);
+        HIP_TRY(h, hipMemcpy(tmp.data(), h->vec[which], h->N * sizeof(float), hipMemcpyDeviceToHost));
+        for (long i = 0; i < h->N; ++i) host[i] = tmp[i];
+    } else {
+        HIP_TRY(h, hipMemcpy(host, h->vec[which], h->N * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return ADSG_OK;
+}
+
+void* adsg_device_ptr(adsg_sim* h, int which) {
+    if (!h || which < ADSG_U || which > ADSG_SOURCE) return nullptr;
+    return h->vec[which];
+}
+
+int adsg_swap(adsg_sim* h) {
+    if (!h) return ADSG_EINVAL;
+    std::swap(h->vec[ADSG_U], h->vec[ADSG_U_PREV]);
+    return ADSG_OK;
+}
+
+int adsg_compute_rhs(adsg_sim* h, const adsg_rhs_form* form) {
+    if (!h || !form) return fail(h, ADSG_EINVAL, "compute_rhs: null argument");
+    return rhs_dispatch(h, form);
+}
+
+int adsg_zero_boundary(adsg_sim* h) {
+    if (!h) return ADSG_EINVAL;
+    return zero_boundary_impl(h);
+}
+
+int adsg_ads_solve(adsg_sim* h, const int* slots) {
+    if (!h) return ADSG_EINVAL;
+    return solve_impl(h, slots);
+}
+
+int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags) {
+    if (!h || !form || nsteps < 0) return fail(h, ADSG_EINVAL, "step: bad argument");
+    if (!h->prepared) return fail(h, ADSG_ESTATE, "step: prepare_matrices first");
+    if (nsteps == 0) return ADSG_OK;
+    hipGraphExec_t g2 = nullptr, g1 = nullptr;
+    int rc;
+    if (nsteps >= 2) {
+        if ((rc = get_graph(h, form, flags, 2, &g2))) return rc;
+        for (int i = 0; i < nsteps / 2; ++i) HIP_TRY(h, hipGraphLaunch(g2, h->stream));
+    }
+    if (nsteps % 2) {
+        if ((rc = get_graph(h, form, flags, 1, &g1))) return rc;
+        HIP_TRY(h, hipGraphLaunch(g1, h->stream));
+        std::swap(h->vec[ADSG_U], h->vec[ADSG_U_PREV]);
+    }
+    return ADSG_OK;
+}
+
+int adsg_synchronize(adsg_sim* h) {
+    if (!h) return ADSG_EINVAL;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return ADSG_OK;
+}
+
+int adsg_band_factorize(int n, int kl, int ku, double* ab, int* ipiv) {
+    if (n < 0 || kl < 0 || ku < 0 || !ab || !ipiv) return ADSG_EINVAL;
+    std::vector<double> b(ab, ab + static_cast<size_t>(2 * kl + ku + 1) * n);
+    std::vector<int> p;
+    int info = adsg::band_lu(b, n, kl, ku, p);
+    std::memcpy(ab, b.data(), sizeof(double) * b.size());
+    std::memcpy(ipiv, p.data(), sizeof(int) * n);
+    return info;
+}
+
+}  // extern "C"

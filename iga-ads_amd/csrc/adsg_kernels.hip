@@ -1,0 +1,782 @@
+// HIP kernels of the MI355X ADS time step (gfx950).
+//
+//  * line sweeps: the dgbtrs 'N' substitution (band_solve.hpp:20-31) of every line of one
+//    direction, in place, in the natural first-index-fastest layout. y/z lines are read
+//    directly (consecutive lanes own consecutive lines, so each step is one coalesced
+//    access); x lines are contiguous and are staged through LDS in chunks, which is the
+//    transpose the reference performs with cyclic_transpose (cyclic_transpose.hpp:17-63)
+//    between its dgbtrs calls. Factors are uniform across lanes (scalar loads).
+//  * element-loop RHS (compute_rhs, validation.hpp:95-119): one workgroup per tile of
+//    elements, one lane per element, sum factorisation through the 1D tables, local
+//    contributions combined in LDS, and tiles of one parity colour (2^d colours) adding
+//    their DOF blocks into the global rhs without atomics.
+#include <hip/hip_runtime.h>
+
+#include "adsg_kernels.hpp"
+
+namespace adsg {
+
+namespace {
+
+constexpr int CH = 16;  // line-sweep chunk (values in flight per lane and per pass)
+
+// ------------------------------------------------------------------------------------
+// line sweeps
+// ------------------------------------------------------------------------------------
+
+// Forward elimination over one chunk [j0, j0+CH): w[0..KL] is the window b[j..j+KL]
+// (partially eliminated); cur holds the raw chunk, nxt the raw next chunk. Column order of
+// LAPACK's dgbtrs: interchange rows j and ipiv(j), then b[j+t] -= L(j+t,j) * b[j].
+template <typename T, typename F, int KL, bool PIV>
+__device__ __forceinline__ void forward_chunk(T (&cur)[CH], const T (&nxt)[CH], T (&w)[KL + 1],
+                                              int j0, int n, const dev_factors<F>& f) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        const int j = j0 + k;
+        if (j < n) {
+            if constexpr (PIV) {
+                const int off = f.poff[j];
+#pragma unroll
+                for (int t = 1; t <= KL; ++t) {
+                    if (off == t) {
+                        T tmp = w[0];
+                        w[0] = w[t];
+                        w[t] = tmp;
+                    }
+                }
+            }
+            const T y = w[0];
+#pragma unroll
+            for (int t = 1; t <= KL; ++t)
+                w[t] -= static_cast<T>(f.lower[static_cast<long>(j) * KL + t - 1]) * y;
+            cur[k] = y;
+#pragma unroll
+            for (int t = 0; t < KL; ++t) w[t] = w[t + 1];
+            const int src = k + KL + 1;
+            w[KL] = (src < CH) ? cur[src < CH ? src : 0] : nxt[src >= CH ? src - CH : 0];
+        }
+    }
+}
+
+// Back substitution over one chunk [j0, j0+CH), highest index first: v[KU] = b[j],
+// v[KU-t] = b[j-t]; x_j = b_j / U(j,j), then b[j-t] -= U(j-t, j) * x_j (dtbsv, upper,
+// column order). prv holds the raw chunk below.
+template <typename T, typename F, int KU>
+__device__ __forceinline__ void backward_chunk(T (&cur)[CH], const T (&prv)[CH], T (&v)[KU + 1],
+                                               int j0, const dev_factors<F>& f) {
+#pragma unroll
+    for (int k = CH - 1; k >= 0; --k) {
+        const int j = j0 + k;
+        if (j >= 0) {
+            const T x = v[KU] * static_cast<T>(f.rdiag[j]);
+#pragma unroll
+            for (int t = 1; t <= KU; ++t)
+                v[KU - t] -= static_cast<T>(f.upper[static_cast<long>(j) * KU + t - 1]) * x;
+            cur[k] = x;
+#pragma unroll
+            for (int t = KU; t > 0; --t) v[t] = v[t - 1];
+            const int src = k - KU - 1;
+            v[0] = (src >= 0) ? cur[src >= 0 ? src : 0] : prv[src < 0 ? src + CH : 0];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_chunk(T (&r)[CH], const T* p, long S, int j0, int n) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        const int j = j0 + k;
+        r[k] = (j >= 0 && j < n) ? p[static_cast<long>(j) * S] : T(0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_chunk(const T (&r)[CH], T* p, long S, int j0, int n) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        const int j = j0 + k;
+        if (j >= 0 && j < n) p[static_cast<long>(j) * S] = r[k];
+    }
+}
+
+// Lines with stride S > 1 (y and z directions): lane = line.
+template <typename T, typename F, int KL, int KU, bool PIV>
+__global__ __launch_bounds__(256) void k_sweep_strided(T* __restrict__ u, long n_lines, long A,
+                                                       long B, long S, dev_factors<F> f) {
+    const long l = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (l >= n_lines) return;
+    T* p = u + (l % A) + (l / A) * B;
+    const int n = f.n;
+    T cur[CH], other[CH];
+
+    load_chunk(cur, p, S, 0, n);
+    load_chunk(other, p, S, CH, n);
+    T w[KL + 1];
+#pragma unroll
+    for (int t = 0; t <= KL; ++t) w[t] = cur[t];
+    for (int j0 = 0; j0 < n; j0 += CH) {
+        forward_chunk<T, F, KL, PIV>(cur, other, w, j0, n, f);
+        store_chunk(cur, p, S, j0, n);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) cur[k] = other[k];
+        load_chunk(other, p, S, j0 + 2 * CH, n);
+    }
+
+    const int top = n - CH;
+    load_chunk(cur, p, S, top, n);
+    load_chunk(other, p, S, top - CH, n);
+    T v[KU + 1];
+#pragma unroll
+    for (int t = 0; t <= KU; ++t) v[KU - t] = cur[CH - 1 - t];
+    for (int j0 = top; j0 > -CH; j0 -= CH) {
+        backward_chunk<T, F, KU>(cur, other, v, j0, f);
+        store_chunk(cur, p, S, j0, n);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) cur[k] = other[k];
+        load_chunk(other, p, S, j0 - 2 * CH, n);
+    }
+}
+
+// Contiguous lines (x direction): one wave owns 64 lines; each chunk of CH values of its
+// lines is loaded coalesced (16 lanes per 128-byte line segment), transposed through LDS
+// into per-lane registers, and written back the same way.
+template <typename T>
+__device__ __forceinline__ void stage_in(T (&r)[CH], T (*tile)[CH + 1], const T* u, long line0,
+                                         long n_lines, int n, int j0) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int rr = 0; rr < CH; ++rr) {
+        const int idx = rr * 64 + lane;
+        const int ln = idx / CH, k = idx % CH;
+        const long line = line0 + ln;
+        const int j = j0 + k;
+        tile[ln][k] = (line < n_lines && j >= 0 && j < n) ? u[line * n + j] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CH; ++k) r[k] = tile[lane][k];
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void stage_out(const T (&r)[CH], T (*tile)[CH + 1], T* u, long line0,
+                                          long n_lines, int n, int j0) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) tile[lane][k] = r[k];
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < CH; ++rr) {
+        const int idx = rr * 64 + lane;
+        const int ln = idx / CH, k = idx % CH;
+        const long line = line0 + ln;
+        const int j = j0 + k;
+        if (line < n_lines && j >= 0 && j < n) u[line * n + j] = tile[ln][k];
+    }
+    __syncthreads();
+}
+
+template <typename T, typename F, int KL, int KU, bool PIV>
+__global__ __launch_bounds__(64) void k_sweep_contig(T* __restrict__ u, long n_lines,
+                                                     dev_factors<F> f) {
+    __shared__ T tile[64][CH + 1];
+    const long line0 = static_cast<long>(blockIdx.x) * 64;
+    const int n = f.n;
+    T cur[CH], other[CH];
+
+    stage_in(cur, tile, u, line0, n_lines, n, 0);
+    stage_in(other, tile, u, line0, n_lines, n, CH);
+    T w[KL + 1];
+#pragma unroll
+    for (int t = 0; t <= KL; ++t) w[t] = cur[t];
+    for (int j0 = 0; j0 < n; j0 += CH) {
+        forward_chunk<T, F, KL, PIV>(cur, other, w, j0, n, f);
+        stage_out(cur, tile, u, line0, n_lines, n, j0);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) cur[k] = other[k];
+        stage_in(other, tile, u, line0, n_lines, n, j0 + 2 * CH);
+    }
+
+    const int top = n - CH;
+    stage_in(cur, tile, u, line0, n_lines, n, top);
+    stage_in(other, tile, u, line0, n_lines, n, top - CH);
+    T v[KU + 1];
+#pragma unroll
+    for (int t = 0; t <= KU; ++t) v[KU - t] = cur[CH - 1 - t];
+    for (int j0 = top; j0 > -CH; j0 -= CH) {
+        backward_chunk<T, F, KU>(cur, other, v, j0, f);
+        stage_out(cur, tile, u, line0, n_lines, n, j0);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) cur[k] = other[k];
+        stage_in(other, tile, u, line0, n_lines, n, j0 - 2 * CH);
+    }
+}
+
+template <typename T, typename F, int KL, int KU, bool PIV>
+hipError_t sweep_dispatch_piv(T* u, const line_geometry& g, const dev_factors<F>& f,
+                              hipStream_t s) {
+    if (g.n_lines <= 0 || f.n <= 0) return hipSuccess;
+    if (g.S == 1) {
+        const unsigned blocks = static_cast<unsigned>((g.n_lines + 63) / 64);
+        hipLaunchKernelGGL((k_sweep_contig<T, F, KL, KU, PIV>), dim3(blocks), dim3(64), 0, s, u,
+                           g.n_lines, f);
+    } else {
+        const unsigned blocks = static_cast<unsigned>((g.n_lines + 255) / 256);
+        hipLaunchKernelGGL((k_sweep_strided<T, F, KL, KU, PIV>), dim3(blocks), dim3(256), 0, s, u,
+                           g.n_lines, g.A, g.B, g.S, f);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, typename F, int KL, int KU>
+hipError_t sweep_dispatch(T* u, const line_geometry& g, const dev_factors<F>& f, hipStream_t s) {
+    return f.pivoted ? sweep_dispatch_piv<T, F, KL, KU, true>(u, g, f, s)
+                     : sweep_dispatch_piv<T, F, KL, KU, false>(u, g, f, s);
+}
+
+// ------------------------------------------------------------------------------------
+// element-loop RHS, 3D
+// ------------------------------------------------------------------------------------
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC>
+__global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T* __restrict__ rhs,
+                                                     dev_tables<T> X, dev_tables<T> Y,
+                                                     dev_tables<T> Z, rhs_params<T> prm,
+                                                     int colour, int ntx2, int nty2) {
+    constexpr int M = P + 1, NT = TX * TY * TZ;
+    constexpr int BX = TX + P, BY = TY + P, BZ = TZ + P, NB = BX * BY * BZ;
+    constexpr int M3 = M * M * M;
+    __shared__ T blk[NB];
+    __shared__ T kb[KF ? NB : 1];
+    __shared__ T sb[SRC ? NB : 1];
+    __shared__ T tvx[TX * M * M], tdx[TX * M * M], twx[TX * M];
+    __shared__ T tvy[TY * M * M], tdy[TY * M * M], twy[TY * M];
+    __shared__ T tvz[TZ * M * M], tdz[TZ * M * M], twz[TZ * M];
+    __shared__ T Rs[M3 * NT];
+
+    const int tid = threadIdx.x;
+    int t = blockIdx.x;
+    const int tx = 2 * (t % ntx2) + (colour & 1);
+    t /= ntx2;
+    const int ty = 2 * (t % nty2) + ((colour >> 1) & 1);
+    t /= nty2;
+    const int tz = 2 * t + ((colour >> 2) & 1);
+    const int ex0 = tx * TX, ey0 = ty * TY, ez0 = tz * TZ;
+    const int nx = X.dofs, ny = Y.dofs, nz = Z.dofs;
+
+    // the tile's DOF block (first(e) = e for these knot vectors, so dof origin = element origin)
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, r = i / BX, by = r % BY, bz = r / BY;
+        const int gx = ex0 + bx, gy = ey0 + by, gz = ez0 + bz;
+        const bool in = gx < nx && gy < ny && gz < nz;
+        const long g = gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz);
+        blk[i] = in ? up[g] : T(0);
+        if constexpr (KF) kb[i] = in ? prm.kfield[g] : T(0);
+        if constexpr (SRC) sb[i] = in ? prm.source[g] : T(0);
+    }
+    for (int i = tid; i < TX * M * M; i += NT) {
+        const bool in = ex0 + i / (M * M) < X.elements;
+        tvx[i] = in ? X.val[static_cast<long>(ex0) * M * M + i] : T(0);
+        tdx[i] = in ? X.der[static_cast<long>(ex0) * M * M + i] : T(0);
+    }
+    for (int i = tid; i < TY * M * M; i += NT) {
+        const bool in = ey0 + i / (M * M) < Y.elements;
+        tvy[i] = in ? Y.val[static_cast<long>(ey0) * M * M + i] : T(0);
+        tdy[i] = in ? Y.der[static_cast<long>(ey0) * M * M + i] : T(0);
+    }
+    for (int i = tid; i < TZ * M * M; i += NT) {
+        const bool in = ez0 + i / (M * M) < Z.elements;
+        tvz[i] = in ? Z.val[static_cast<long>(ez0) * M * M + i] : T(0);
+        tdz[i] = in ? Z.der[static_cast<long>(ez0) * M * M + i] : T(0);
+    }
+    for (int i = tid; i < TX * M; i += NT)
+        twx[i] = (ex0 + i / M < X.elements) ? X.wj[static_cast<long>(ex0) * M + i] : T(0);
+    for (int i = tid; i < TY * M; i += NT)
+        twy[i] = (ey0 + i / M < Y.elements) ? Y.wj[static_cast<long>(ey0) * M + i] : T(0);
+    for (int i = tid; i < TZ * M; i += NT)
+        twz[i] = (ez0 + i / M < Z.elements) ? Z.wj[static_cast<long>(ez0) * M + i] : T(0);
+    __syncthreads();
+
+    const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
+    const bool valid = ex0 + lx < X.elements && ey0 + ly < Y.elements && ez0 + lz < Z.elements;
+    if (!valid) {
+#pragma unroll
+        for (int i = 0; i < M3; ++i) Rs[i * NT + tid] = T(0);
+    } else {
+        T Bz[M][M], Dz[M][M], wz[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            wz[q] = twz[lz * M + q];
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                Bz[q][c] = tvz[(lz * M + q) * M + c];
+                Dz[q][c] = tdz[(lz * M + q) * M + c];
+            }
+        }
+#pragma unroll 1
+        for (int qx = 0; qx < M; ++qx) {
+            T bx[M], dx[M];
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                bx[a] = tvx[(lx * M + qx) * M + a];
+                dx[a] = tdx[(lx * M + qx) * M + a];
+            }
+            const T wx = twx[lx * M + qx];
+            // stage x: contract the first index of the element's coefficient block
+            T A0[M][M], A1[M][M], KA[KF ? M : 1][KF ? M : 1], SA[SRC ? M : 1][SRC ? M : 1];
+#pragma unroll
+            for (int b = 0; b < M; ++b)
+#pragma unroll
+                for (int c = 0; c < M; ++c) {
+                    T s0 = 0, s1 = 0, sk = 0, ss = 0;
+#pragma unroll
+                    for (int a = 0; a < M; ++a) {
+                        const int li = (lx + a) + BX * ((ly + b) + BY * (lz + c));
+                        const T cv = blk[li];
+                        s0 += bx[a] * cv;
+                        s1 += dx[a] * cv;
+                        if constexpr (KF) sk += bx[a] * kb[li];
+                        if constexpr (SRC) ss += bx[a] * sb[li];
+                    }
+                    A0[b][c] = s0;
+                    A1[b][c] = s1;
+                    if constexpr (KF) KA[b][c] = sk;
+                    if constexpr (SRC) SA[b][c] = ss;
+                }
+            T K0[M][M], Kx[M][M];
+#pragma unroll
+            for (int b = 0; b < M; ++b)
+#pragma unroll
+                for (int c = 0; c < M; ++c) K0[b][c] = Kx[b][c] = T(0);
+#pragma unroll
+            for (int qy = 0; qy < M; ++qy) {
+                T by[M], dy[M];
+#pragma unroll
+                for (int b = 0; b < M; ++b) {
+                    by[b] = tvy[(ly * M + qy) * M + b];
+                    dy[b] = tdy[(ly * M + qy) * M + b];
+                }
+                const T wxy = wx * twy[ly * M + qy];
+                // stage y for this Gauss point
+                T a00[M], a01[M], a10[M], ka[KF ? M : 1], sa[SRC ? M : 1];
+#pragma unroll
+                for (int c = 0; c < M; ++c) {
+                    T s00 = 0, s01 = 0, s10 = 0, sk = 0, ss = 0;
+#pragma unroll
+                    for (int b = 0; b < M; ++b) {
+                        s00 += by[b] * A0[b][c];
+                        s01 += dy[b] * A0[b][c];
+                        s10 += by[b] * A1[b][c];
+                        if constexpr (KF) sk += by[b] * KA[b][c];
+                        if constexpr (SRC) ss += by[b] * SA[b][c];
+                    }
+                    a00[c] = s00;
+                    a01[c] = s01;
+                    a10[c] = s10;
+                    if constexpr (KF) ka[c] = sk;
+                    if constexpr (SRC) sa[c] = ss;
+                }
+                T H0[M], Hx[M], Hy[M];
+#pragma unroll
+                for (int c = 0; c < M; ++c) H0[c] = Hx[c] = Hy[c] = T(0);
+#pragma unroll
+                for (int qz = 0; qz < M; ++qz) {
+                    T u = 0, uz = 0, uy = 0, ux = 0, kv = 0, fv = 0;
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        u += Bz[qz][c] * a00[c];
+                        uz += Dz[qz][c] * a00[c];
+                        uy += Bz[qz][c] * a01[c];
+                        ux += Bz[qz][c] * a10[c];
+                        if constexpr (KF) kv += Bz[qz][c] * ka[c];
+                        if constexpr (SRC) fv += Bz[qz][c] * sa[c];
+                    }
+                    const T wJ = wxy * wz[qz];
+                    T g0 = prm.mass * u;
+                    if (prm.reaction != T(0)) g0 += prm.dt * prm.reaction * u * (T(1) - u);
+                    if constexpr (SRC) g0 += prm.dt * fv;
+                    g0 *= wJ;
+                    T sc = -prm.dt * wJ;
+                    if constexpr (KF) sc *= kv;
+                    const T gx = sc * prm.gx * ux, gy = sc * prm.gy * uy, gz = sc * prm.gz * uz;
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        H0[c] += Bz[qz][c] * g0 + Dz[qz][c] * gz;
+                        Hx[c] += Bz[qz][c] * gx;
+                        Hy[c] += Bz[qz][c] * gy;
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < M; ++b)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        K0[b][c] += by[b] * H0[c] + dy[b] * Hy[c];
+                        Kx[b][c] += by[b] * Hx[c];
+                    }
+            }
+            // stage x transposed: accumulate the element contribution in LDS
+#pragma unroll
+            for (int a = 0; a < M; ++a)
+#pragma unroll
+                for (int b = 0; b < M; ++b)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        const int idx = (a + M * (b + M * c)) * NT + tid;
+                        const T r = bx[a] * K0[b][c] + dx[a] * Kx[b][c];
+                        Rs[idx] = (qx == 0) ? r : Rs[idx] + r;
+                    }
+        }
+    }
+    __syncthreads();
+
+    // each DOF of the block sums the contributions of the tile's elements that support it
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, r = i / BX, by = r % BY, bz = r / BY;
+        const int gx = ex0 + bx, gy = ey0 + by, gz = ez0 + bz;
+        if (gx >= nx || gy >= ny || gz >= nz) continue;
+        T sum = 0;
+        for (int lz2 = (bz > P ? bz - P : 0); lz2 <= (bz < TZ - 1 ? bz : TZ - 1); ++lz2)
+            for (int ly2 = (by > P ? by - P : 0); ly2 <= (by < TY - 1 ? by : TY - 1); ++ly2)
+                for (int lx2 = (bx > P ? bx - P : 0); lx2 <= (bx < TX - 1 ? bx : TX - 1); ++lx2)
+                    sum += Rs[((bx - lx2) + M * ((by - ly2) + M * (bz - lz2))) * NT + lx2 +
+                              TX * (ly2 + TY * lz2)];
+        rhs[gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz)] += sum;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// element-loop RHS, 2D
+// ------------------------------------------------------------------------------------
+template <typename T, int P, int TX, int TY, bool KF, bool SRC>
+__global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __restrict__ rhs,
+                                                 dev_tables<T> X, dev_tables<T> Y,
+                                                 rhs_params<T> prm, int colour, int ntx2) {
+    constexpr int M = P + 1, NT = TX * TY;
+    constexpr int BX = TX + P, BY = TY + P, NB = BX * BY, M2 = M * M;
+    __shared__ T blk[NB];
+    __shared__ T kb[KF ? NB : 1];
+    __shared__ T sb[SRC ? NB : 1];
+    __shared__ T tvx[TX * M * M], tdx[TX * M * M], twx[TX * M];
+    __shared__ T tvy[TY * M * M], tdy[TY * M * M], twy[TY * M];
+    __shared__ T Rs[M2 * NT];
+
+    const int tid = threadIdx.x;
+    const int t = blockIdx.x;
+    const int tx = 2 * (t % ntx2) + (colour & 1);
+    const int ty = 2 * (t / ntx2) + ((colour >> 1) & 1);
+    const int ex0 = tx * TX, ey0 = ty * TY;
+    const int nx = X.dofs, ny = Y.dofs;
+
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, by = i / BX;
+        const int gx = ex0 + bx, gy = ey0 + by;
+        const bool in = gx < nx && gy < ny;
+        const long g = gx + static_cast<long>(nx) * gy;
+        blk[i] = in ? up[g] : T(0);
+        if constexpr (KF) kb[i] = in ? prm.kfield[g] : T(0);
+        if constexpr (SRC) sb[i] = in ? prm.source[g] : T(0);
+    }
+    for (int i = tid; i < TX * M * M; i += NT) {
+        const bool in = ex0 + i / (M * M) < X.elements;
+        tvx[i] = in ? X.val[static_cast<long>(ex0) * M * M + i] : T(0);
+        tdx[i] = in ? X.der[static_cast<long>(ex0) * M * M + i] : T(0);
+    }
+    for (int i = tid; i < TY * M * M; i += NT) {
+        const bool in = ey0 + i / (M * M) < Y.elements;
+        tvy[i] = in ? Y.val[static_cast<long>(ey0) * M * M + i] : T(0);
+        tdy[i] = in ? Y.der[static_cast<long>(ey0) * M * M + i] : T(0);
+    }
+    for (int i = tid; i < TX * M; i += NT)
+        twx[i] = (ex0 + i / M < X.elements) ? X.wj[static_cast<long>(ex0) * M + i] : T(0);
+    for (int i = tid; i < TY * M; i += NT)
+        twy[i] = (ey0 + i / M < Y.elements) ? Y.wj[static_cast<long>(ey0) * M + i] : T(0);
+    __syncthreads();
+
+    const int lx = tid % TX, ly = tid / TX;
+    const bool valid = ex0 + lx < X.elements && ey0 + ly < Y.elements;
+    if (!valid) {
+#pragma unroll
+        for (int i = 0; i < M2; ++i) Rs[i * NT + tid] = T(0);
+    } else {
+        T By[M][M], Dy[M][M], wy[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            wy[q] = twy[ly * M + q];
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                By[q][b] = tvy[(ly * M + q) * M + b];
+                Dy[q][b] = tdy[(ly * M + q) * M + b];
+            }
+        }
+#pragma unroll
+        for (int qx = 0; qx < M; ++qx) {
+            T bx[M], dx[M];
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                bx[a] = tvx[(lx * M + qx) * M + a];
+                dx[a] = tdx[(lx * M + qx) * M + a];
+            }
+            const T wx = twx[lx * M + qx];
+            T A0[M], A1[M], KA[KF ? M : 1], SA[SRC ? M : 1];
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                T s0 = 0, s1 = 0, sk = 0, ss = 0;
+#pragma unroll
+                for (int a = 0; a < M; ++a) {
+                    const int li = (lx + a) + BX * (ly + b);
+                    const T cv = blk[li];
+                    s0 += bx[a] * cv;
+                    s1 += dx[a] * cv;
+                    if constexpr (KF) sk += bx[a] * kb[li];
+                    if constexpr (SRC) ss += bx[a] * sb[li];
+                }
+                A0[b] = s0;
+                A1[b] = s1;
+                if constexpr (KF) KA[b] = sk;
+                if constexpr (SRC) SA[b] = ss;
+            }
+            T K0[M], Kx[M];
+#pragma unroll
+            for (int b = 0; b < M; ++b) K0[b] = Kx[b] = T(0);
+#pragma unroll
+            for (int qy = 0; qy < M; ++qy) {
+                T u = 0, uy = 0, ux = 0, kv = 0, fv = 0;
+#pragma unroll
+                for (int b = 0; b < M; ++b) {
+                    u += By[qy][b] * A0[b];
+                    uy += Dy[qy][b] * A0[b];
+                    ux += By[qy][b] * A1[b];
+                    if constexpr (KF) kv += By[qy][b] * KA[b];
+                    if constexpr (SRC) fv += By[qy][b] * SA[b];
+                }
+                const T wJ = wx * wy[qy];
+                T g0 = prm.mass * u;
+                if (prm.reaction != T(0)) g0 += prm.dt * prm.reaction * u * (T(1) - u);
+                if constexpr (SRC) g0 += prm.dt * fv;
+                g0 *= wJ;
+                T sc = -prm.dt * wJ;
+                if constexpr (KF) sc *= kv;
+                const T gx = sc * prm.gx * ux, gy = sc * prm.gy * uy;
+#pragma unroll
+                for (int b = 0; b < M; ++b) {
+                    K0[b] += By[qy][b] * g0 + Dy[qy][b] * gy;
+                    Kx[b] += By[qy][b] * gx;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < M; ++a)
+#pragma unroll
+                for (int b = 0; b < M; ++b) {
+                    const int idx = (a + M * b) * NT + tid;
+                    const T r = bx[a] * K0[b] + dx[a] * Kx[b];
+                    Rs[idx] = (qx == 0) ? r : Rs[idx] + r;
+                }
+        }
+    }
+    __syncthreads();
+
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, by = i / BX;
+        const int gx = ex0 + bx, gy = ey0 + by;
+        if (gx >= nx || gy >= ny) continue;
+        T sum = 0;
+        for (int ly2 = (by > P ? by - P : 0); ly2 <= (by < TY - 1 ? by : TY - 1); ++ly2)
+            for (int lx2 = (bx > P ? bx - P : 0); lx2 <= (bx < TX - 1 ? bx : TX - 1); ++lx2)
+                sum += Rs[((bx - lx2) + M * (by - ly2)) * NT + lx2 + TX * ly2];
+        rhs[gx + static_cast<long>(nx) * gy] += sum;
+    }
+}
+
+// tile shapes (elements) per dimension and degree: one lane per element
+template <int D, int P>
+struct tile;
+template <int P>
+struct tile<2, P> {
+    static constexpr int X = 16, Y = 16, Z = 1;
+};
+template <>
+struct tile<3, 1> {
+    static constexpr int X = 8, Y = 8, Z = 4;
+};
+template <>
+struct tile<3, 2> {
+    static constexpr int X = 8, Y = 8, Z = 4;
+};
+template <>
+struct tile<3, 3> {
+    static constexpr int X = 8, Y = 4, Z = 4;
+};
+
+template <typename T, int P, bool KF, bool SRC>
+hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
+                        hipStream_t s) {
+    using TL = tile<3, P>;
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X;
+    const int nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    const int ntz = (d[2].elements + TL::Z - 1) / TL::Z;
+    for (int c = 0; c < 8; ++c) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+        const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2, nz2 = (ntz - cz + 1) / 2;
+        const long blocks = static_cast<long>(nx2) * ny2 * nz2;
+        if (blocks == 0) continue;
+        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC>), dim3(blocks),
+                           dim3(TL::X * TL::Y * TL::Z), 0, s, up, rhs, d[0], d[1], d[2], prm, c,
+                           nx2, ny2);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T, int P, bool KF, bool SRC>
+hipError_t rhs2_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
+                        hipStream_t s) {
+    using TL = tile<2, P>;
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X;
+    const int nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    for (int c = 0; c < 4; ++c) {
+        const int cx = c & 1, cy = (c >> 1) & 1;
+        const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2;
+        const long blocks = static_cast<long>(nx2) * ny2;
+        if (blocks == 0) continue;
+        hipLaunchKernelGGL((k_rhs2<T, P, TL::X, TL::Y, KF, SRC>), dim3(blocks),
+                           dim3(TL::X * TL::Y), 0, s, up, rhs, d[0], d[1], prm, c, nx2);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T, int P>
+hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
+                      const rhs_params<T>& prm, hipStream_t s) {
+    const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
+    if (dim == 3) {
+        if (kf && src) return rhs3_colours<T, P, true, true>(up, rhs, d, prm, s);
+        if (kf) return rhs3_colours<T, P, true, false>(up, rhs, d, prm, s);
+        if (src) return rhs3_colours<T, P, false, true>(up, rhs, d, prm, s);
+        return rhs3_colours<T, P, false, false>(up, rhs, d, prm, s);
+    }
+    if (kf && src) return rhs2_colours<T, P, true, true>(up, rhs, d, prm, s);
+    if (kf) return rhs2_colours<T, P, true, false>(up, rhs, d, prm, s);
+    if (src) return rhs2_colours<T, P, false, true>(up, rhs, d, prm, s);
+    return rhs2_colours<T, P, false, false>(up, rhs, d, prm, s);
+}
+
+// ------------------------------------------------------------------------------------
+// small utilities
+// ------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, long n, T v) {
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T>
+__global__ void k_zero_boundary(T* __restrict__ u, int dim, int nx, int ny, int nz) {
+    const long total = static_cast<long>(nx) * ny * nz;
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += static_cast<long>(gridDim.x) * blockDim.x) {
+        const int x = static_cast<int>(i % nx);
+        const long r = i / nx;
+        const int y = static_cast<int>(r % ny), z = static_cast<int>(r / ny);
+        bool b = x == 0 || x == nx - 1 || y == 0 || y == ny - 1;
+        if (dim == 3) b = b || z == 0 || z == nz - 1;
+        if (b) u[i] = T(0);
+    }
+}
+
+__global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, long n) {
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x)
+        out[i] = static_cast<float>(in[i]);
+}
+
+__global__ void k_f2d(const float* __restrict__ in, double* __restrict__ out, long n) {
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x)
+        out[i] = static_cast<double>(in[i]);
+}
+
+inline unsigned grid_for(long n) {
+    long b = (n + 255) / 256;
+    return static_cast<unsigned>(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------
+template <typename T, typename F>
+hipError_t launch_line_sweep(T* u, const line_geometry& g, const dev_factors<F>& f,
+                             hipStream_t s) {
+    const int kl = f.kl, ku = f.ku;
+#define ADSG_SWEEP(KL_, KU_) \
+    if (kl == KL_ && ku == KU_) return sweep_dispatch<T, F, KL_, KU_>(u, g, f, s);
+    ADSG_SWEEP(1, 1)
+    ADSG_SWEEP(1, 2)
+    ADSG_SWEEP(1, 3)
+    ADSG_SWEEP(2, 2)
+    ADSG_SWEEP(2, 3)
+    ADSG_SWEEP(2, 4)
+    ADSG_SWEEP(3, 3)
+    ADSG_SWEEP(3, 4)
+    ADSG_SWEEP(3, 5)
+    ADSG_SWEEP(3, 6)
+#undef ADSG_SWEEP
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
+                      const rhs_params<T>& prm, hipStream_t s) {
+    switch (p) {
+        case 1:
+            return rhs_fields<T, 1>(dim, u_prev, rhs, dirs, prm, s);
+        case 2:
+            return rhs_fields<T, 2>(dim, u_prev, rhs, dirs, prm, s);
+        case 3:
+            return rhs_fields<T, 3>(dim, u_prev, rhs, dirs, prm, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_fill(T* p, long n, T v, hipStream_t s) {
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(n)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_zero_boundary(int dim, T* u, int nx, int ny, int nz, hipStream_t s) {
+    const long n = static_cast<long>(nx) * ny * nz;
+    hipLaunchKernelGGL((k_zero_boundary<T>), dim3(grid_for(n)), dim3(256), 0, s, u, dim, nx, ny, nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_d2f(const double* in, float* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_f2d(const float* in, double* out, long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_f2d, dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+template hipError_t launch_line_sweep<double, double>(double*, const line_geometry&,
+                                                      const dev_factors<double>&, hipStream_t);
+template hipError_t launch_line_sweep<float, float>(float*, const line_geometry&,
+                                                    const dev_factors<float>&, hipStream_t);
+template hipError_t launch_rhs<double>(int, int, const double*, double*, const dev_tables<double>*,
+                                       const rhs_params<double>&, hipStream_t);
+template hipError_t launch_rhs<float>(int, int, const float*, float*, const dev_tables<float>*,
+                                      const rhs_params<float>&, hipStream_t);
+template hipError_t launch_fill<double>(double*, long, double, hipStream_t);
+template hipError_t launch_fill<float>(float*, long, float, hipStream_t);
+template hipError_t launch_zero_boundary<double>(int, double*, int, int, int, hipStream_t);
+template hipError_t launch_zero_boundary<float>(int, float*, int, int, int, hipStream_t);
+
+}  // namespace adsg

@@ -1,0 +1,63 @@
+// Kernel launch interface of the MI355X ADS step (device code in adsg_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace adsg {
+
+// Factors of one band matrix for the line sweeps (layout documented in adsg_setup.hpp).
+template <typename F>
+struct dev_factors {
+    const F* lower = nullptr;  // [n*kl]
+    const F* upper = nullptr;  // [n*ku]
+    const F* rdiag = nullptr;  // [n]
+    const int* poff = nullptr; // [n], only read when pivoted
+    int n = 0, kl = 0, ku = 0;
+    bool pivoted = false;
+};
+
+// One direction of the line solve: n_lines lines of length f.n, line l starts at
+// (l % A) + (l / A) * B and has stride S. S == 1 selects the LDS-staged contiguous sweep.
+struct line_geometry {
+    long n_lines = 0;
+    long A = 1, B = 0, S = 1;
+};
+
+template <typename T, typename F>
+hipError_t launch_line_sweep(T* u, const line_geometry& g, const dev_factors<F>& f,
+                             hipStream_t s);
+
+// Tables of one direction on the device: val/der [elements][p+1][p+1] (element, Gauss
+// point, local function) and wj [elements][p+1].
+template <typename T>
+struct dev_tables {
+    const T* val = nullptr;
+    const T* der = nullptr;
+    const T* wj = nullptr;
+    int elements = 0;
+    int dofs = 0;
+};
+
+// Integrand of compute_rhs at a Gauss point:
+//   mass*u*v - dt*k*(gx*ux*vx + gy*uy*vy + gz*uz*vz) + dt*reaction*u*(1-u)*v + dt*f*v
+template <typename T>
+struct rhs_params {
+    T mass = 1, dt = 0, gx = 1, gy = 1, gz = 1, reaction = 0;
+    const T* kfield = nullptr;  // coefficients of k on the same basis, or null (k = 1)
+    const T* source = nullptr;  // coefficients of f on the same basis, or null (f = 0)
+};
+
+template <typename T>
+hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
+                      const rhs_params<T>& prm, hipStream_t s);
+
+template <typename T>
+hipError_t launch_fill(T* p, long n, T v, hipStream_t s);
+
+template <typename T>
+hipError_t launch_zero_boundary(int dim, T* u, int nx, int ny, int nz, hipStream_t s);
+
+hipError_t launch_convert_d2f(const double* in, float* out, long n, hipStream_t s);
+hipError_t launch_convert_f2d(const float* in, double* out, long n, hipStream_t s);
+
+}  // namespace adsg

@@ -1,0 +1,149 @@
+/*
+ * adsgpu -- C ABI of the MI355X-native alternating-direction (ADS) time step.
+ *
+ * One handle = one simulation_2d / simulation_3d of the reference (IGA-ADS 0.1.0) on one
+ * GPU: the B-spline dimensions, their 1D band matrices and FP64 factors, and the device
+ * state vectors u, u_prev and buffer. Each entry point replaces the reference call cited
+ * next to it; "file:line" refers to the reference tree.
+ *
+ * Conventions (all entry points):
+ *  - return value: ADSG_OK (0) on success; > 0 is a LAPACK-style zero-pivot index from a
+ *    factorisation; < 0 is one of the ADSG_E* codes. Nothing throws across this ABI.
+ *    adsg_last_error(h) gives the message of the handle's last failure,
+ *    adsg_last_error(NULL) the message of the last failed adsg_create on this thread.
+ *  - host arrays are borrowed for the duration of the call only; layouts are the
+ *    reference's: vectors first-index-fastest (lin/tensor/base.hpp:14,
+ *    util/multi_array/ordering/reverse.hpp:28-31), u(i0,i1,i2) at i0 + n0*(i1 + n1*i2);
+ *    band matrices in LAPACK band storage (lin/band_matrix.hpp:66-73).
+ *  - a handle owns one HIP stream; calls on a handle are asynchronous with respect to the
+ *    host except the ones that return host data (adsg_get_vector) and adsg_synchronize.
+ *    A handle is not thread-safe; different handles may be used concurrently.
+ */
+#ifndef ADSGPU_H
+#define ADSGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADSG_OK 0
+#define ADSG_EINVAL (-1)   /* bad argument or unsupported configuration */
+#define ADSG_EHIP (-2)     /* HIP runtime error (message in adsg_last_error) */
+#define ADSG_ENOMEM (-3)   /* device allocation failed */
+#define ADSG_ESTATE (-4)   /* call out of order (e.g. solve before prepare_matrices) */
+#define ADSG_ENONFINITE (-5) /* a state check found NaN/Inf */
+
+/* arithmetic of the step */
+#define ADSG_FP64 0 /* FP64 throughout */
+#define ADSG_FP32 1 /* FP32 contractions and substitution; factors computed in FP64 */
+
+/* vectors held by a handle */
+#define ADSG_U 0      /* the state / right-hand side being solved (simulation's u) */
+#define ADSG_U_PREV 1 /* previous state (simulation's u_prev) */
+#define ADSG_BUFFER 2 /* scratch of the same shape (simulation_2d.cpp:19 buffer) */
+#define ADSG_KFIELD 3 /* coefficients of a diffusion field k(x) on the same basis */
+#define ADSG_SOURCE 4 /* coefficients of a source f(x) on the same basis */
+
+typedef struct adsg_sim adsg_sim;
+
+/* ads::dim_config (simulation/config.hpp:11-29). quad_order <= 0 means p + 1. */
+typedef struct adsg_dim_config {
+    int p;
+    int elements;
+    double a;
+    double b;
+    int quad_order;
+    int repeated_nodes;
+} adsg_dim_config;
+
+/* ads::config_2d / config_3d (simulation/config.hpp:46-56) plus the dimension count and the
+ * arithmetic (ADSG_FP64 / ADSG_FP32). */
+typedef struct adsg_config {
+    int dim; /* 2 or 3 */
+    adsg_dim_config x, y, z;
+    int step_count;
+    double dt;
+    int derivatives;
+    int precision;
+} adsg_config;
+
+/* The integrand of compute_rhs at a Gauss point, for test function v and state u:
+ *   mass*u*v - dt*k*(grad[0]*u_x*v_x + grad[1]*u_y*v_y + grad[2]*u_z*v_z)
+ *            + dt*reaction*u*(1-u)*v + dt*f*v
+ * with k = 1 unless use_kfield (k from ADSG_KFIELD) and f = 0 unless use_source.
+ * Explicit heat (validation.hpp:110-115, heat_3d.hpp:60-63): mass 1, grad (1,1,1).
+ * Directional split terms (implicit.hpp:132-182): dt -> dt/2 and one grad entry. */
+typedef struct adsg_rhs_form {
+    double mass;
+    double dt;
+    double grad[3];
+    double reaction;
+    int use_kfield;
+    int use_source;
+} adsg_rhs_form;
+
+/* simulation_2d(config) / simulation_3d(config) (simulation_2d.cpp:8-19,
+ * simulation_3d.cpp:8-22): builds each dimension (dimension.cpp:8-18: knots, tabulated
+ * basis, Gram matrix) and allocates u, u_prev and buffer on the current HIP device. */
+int adsg_create(const adsg_config* cfg, adsg_sim** out);
+void adsg_destroy(adsg_sim* h);
+const char* adsg_last_error(const adsg_sim* h);
+
+int adsg_dofs(const adsg_sim* h, int dir);   /* dimension::dofs (dimension.hpp:35) */
+long adsg_size(const adsg_sim* h);           /* number of coefficients */
+int adsg_precision(const adsg_sim* h);
+
+/* dimension::fix_dof (dimension.cpp:23-29) on the direction's Gram matrix (slot 0). */
+int adsg_fix_dof(adsg_sim* h, int dir, int k);
+
+/* A further 1D matrix of direction dir in slot >= 1: mass*M + stiff*K built from the
+ * direction's tables like implicit.hpp:46-64 (matrix(): va*vb + h*da*db). */
+int adsg_set_matrix(adsg_sim* h, int dir, int slot, double mass, double stiff);
+
+/* A user band matrix for direction dir, slot >= 0: lin::band_matrix(kl, ku, n) full buffer
+ * (band_matrix.hpp:31-32, column size 2kl+ku+1), n = adsg_dofs(h, dir). */
+int adsg_set_band_matrix(adsg_sim* h, int dir, int slot, int kl, int ku, const double* ab);
+
+/* prepare_matrices (simulation_2d.hpp:56-59, simulation_3d.hpp:58-62): factor every slot
+ * in FP64 (lin::factorize -> dgbtrf, band_solve.hpp:16-18) and upload the factors.
+ * Returns the first nonzero LAPACK info (zero pivot) if any. */
+int adsg_prepare_matrices(adsg_sim* h);
+
+/* copy a vector in/out (FP64 host data whatever the precision). */
+int adsg_set_vector(adsg_sim* h, int which, const double* host);
+int adsg_get_vector(adsg_sim* h, int which, double* host);
+/* device pointer of a vector (double* or float* by precision), for zero-copy interop */
+void* adsg_device_ptr(adsg_sim* h, int which);
+
+/* before_step's swap(u, u_prev) (validation.hpp:73-76). */
+int adsg_swap(adsg_sim* h);
+
+/* compute_rhs: u := element-loop assembly over u_prev (validation.hpp:95-119). */
+int adsg_compute_rhs(adsg_sim* h, const adsg_rhs_form* form);
+
+/* validation.hpp:31-41: zero the boundary coefficients of u (all faces in 3D). */
+int adsg_zero_boundary(adsg_sim* h);
+
+/* ads_solve(u, buffer, dims...) (solver.hpp:222-226): u := (A_x (x) A_y [(x) A_z])^-1 u
+ * with A_d the factored matrix in slots[d] (NULL = slot 0 everywhere = solve(),
+ * simulation_2d.hpp:41). */
+int adsg_ads_solve(adsg_sim* h, const int* slots);
+
+/* nsteps of simulation_base::run's loop body (simulation_base.cpp:14-18) as in
+ * validation.hpp:73-81: swap; compute_rhs(form); [zero_boundary]; ads_solve(slot 0).
+ * Replayed from a HIP graph captured once per (form, flags). */
+#define ADSG_STEP_DIRICHLET 1
+int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags);
+
+int adsg_synchronize(adsg_sim* h);
+
+/* lin::factorize + lin::solve_with_factorized (band_solve.hpp:16-31) on the host in
+ * FP64, for callers that hold their own band matrices: factor ab (n x n, kl, ku, full
+ * buffer as above) in place, pivots 1-based like dgbtrf. Returns LAPACK info. */
+int adsg_band_factorize(int n, int kl, int ku, double* ab, int* ipiv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADSGPU_H */

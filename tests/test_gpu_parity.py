@@ -1,0 +1,264 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the same inputs.
+
+Bar: the FP64 step matches the oracle to <= 1e-12 relative L2 after every step; the FP32
+step is reported against the FP64 oracle with a stated bound. Full-size configurations
+are checked through size-independent properties (conservation, symmetry, determinism)
+and one oracle step.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import adsgpu as ag
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+FP64_STEP_TOL = 1e-12
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _pair(dim, p, ne, precision=ag.FP64, dirichlet=False):
+    g = ag.Simulation(dim, [p] * dim, list(ne), precision=precision)
+    o = po.Sim(dim, [p] * dim, list(ne))
+    if dirichlet:
+        g.fix_all_boundaries()
+        o.fix_all_boundaries()
+    assert g.prepare_matrices() == 0
+    assert o.prepare_matrices() == 0
+    assert g.shape == o.shape
+    return g, o
+
+
+# ------------------------------------------------------------------ solver known answers
+MX = {(0, 0): 1, (0, 1): 2, (1, 0): 2, (1, 1): 3, (1, 2): 1, (2, 1): -1, (2, 2): 4, (3, 2): 1, (3, 3): 3}
+MY = {(0, 0): 2, (0, 1): 3, (1, 0): 1, (1, 1): 2, (1, 2): -3, (2, 1): 2, (2, 2): 2}
+MZ = {(0, 0): 1, (0, 1): 2, (1, 0): 3, (1, 1): 4}
+
+
+def _band(n, ent, kl=1, ku=1):
+    ab = np.zeros((n, 2 * kl + ku + 1))
+    for (i, j), v in ent.items():
+        ab[j, kl + ku + i - j] = v
+    return ab
+
+
+def test_ads_known_answer_2d():
+    # solver_test.cpp:101-162 -- matrices that need row interchanges
+    _log("case: ads 2D known answer (4x3)")
+    g = ag.Simulation(2, [1, 1], [3, 2])
+    assert g.shape == (4, 3)
+    g.set_band_matrix(0, 0, 1, 1, _band(4, MX))
+    g.set_band_matrix(1, 0, 1, 1, _band(3, MY))
+    g.prepare_matrices()
+    rhs = np.array([61, 127, 86, 123, -48, -96, -48, -64, 92, 188, 112, 156], dtype=float)
+    g.set(ag.U, rhs)
+    g.ads_solve()
+    assert np.allclose(g.get(ag.U), np.arange(1, 13), rtol=1e-12)
+
+
+def test_ads_known_answer_3d():
+    # solver_test.cpp:164-255
+    _log("case: ads 3D known answer (4x3x2)")
+    g = ag.Simulation(3, [1, 1, 1], [3, 2, 1])
+    assert g.shape == (4, 3, 2)
+    g.set_band_matrix(0, 0, 1, 1, _band(4, MX))
+    g.set_band_matrix(1, 0, 1, 1, _band(3, MY))
+    g.set_band_matrix(2, 0, 1, 1, _band(2, MZ))
+    g.prepare_matrices()
+    rhs = np.array([543, 1101, 618, 849, -144, -288, -144, -192, 564, 1140, 624, 852,
+                    1147, 2329, 1322, 1821, -336, -672, -336, -448, 1220, 2468, 1360, 1860],
+                   dtype=float)
+    g.set(ag.U, rhs)
+    g.ads_solve()
+    assert np.allclose(g.get(ag.U), np.arange(1, 25), rtol=1e-12)
+
+
+# ------------------------------------------------------------------ solve and rhs parity
+SHAPES = [(2, 1, (37, 20)), (2, 2, (64, 64)), (2, 3, (33, 47)), (3, 1, (9, 17, 8)),
+          (3, 2, (16, 13, 11)), (3, 3, (10, 9, 12))]
+
+
+@pytest.mark.parametrize("dim,p,ne", SHAPES)
+def test_ads_solve_matches_oracle(dim, p, ne):
+    _log(f"case: ads_solve dim={dim} p={p} ne={ne}")
+    for dirichlet in (False, True):
+        g, o = _pair(dim, p, ne, dirichlet=dirichlet)
+        b = np.random.default_rng(dim * 100 + p).standard_normal(g.size)
+        g.set(ag.U, b)
+        g.ads_solve()
+        x = g.get(ag.U)
+        y = o.solve(b.copy())
+        assert po.rel_l2(x, y) < 1e-13, dirichlet
+
+
+@pytest.mark.parametrize("dim,p,ne", SHAPES)
+def test_rhs_heat_matches_oracle(dim, p, ne):
+    _log(f"case: rhs heat dim={dim} p={p} ne={ne}")
+    g, o = _pair(dim, p, ne)
+    c = np.random.default_rng(7 + p).standard_normal(g.size)
+    g.set(ag.U_PREV, c)
+    g.compute_rhs(ag.rhs_form(1e-3))
+    got = g.get(ag.U)
+    expect = o.compute_rhs(c, po.heat_form(1e-3, dim))
+    assert po.rel_l2(got, expect) < 1e-13
+
+
+@pytest.mark.parametrize("dim,p,ne", [(2, 2, (20, 17)), (3, 2, (9, 8, 10)), (2, 3, (11, 12)), (3, 1, (8, 9, 7))])
+def test_rhs_general_forms_match_oracle(dim, p, ne):
+    _log(f"case: rhs forms dim={dim} p={p} ne={ne}")
+    g, o = _pair(dim, p, ne)
+    rng = np.random.default_rng(11)
+    c = rng.uniform(0, 1, g.size)
+    kf = np.exp(rng.uniform(-0.5, 0.5, g.size))
+    src = rng.standard_normal(g.size)
+    g.set(ag.U_PREV, c)
+    g.set(ag.KFIELD, kf)
+    g.set(ag.SOURCE, src)
+    dt = 2e-3
+    cases = [
+        (dict(grad=(0.0, 1.0, 0.0)), dict(grad=(0.0, 1.0, 0.0))),
+        (dict(kfield=True), dict(kfield=kf)),
+        (dict(reaction=1.0), dict(reaction=1.0)),
+        (dict(source=True), dict(source=src)),
+        (dict(kfield=True, source=True, reaction=0.5), dict(kfield=kf, source=src, reaction=0.5)),
+    ]
+    for gk, ok in cases:
+        g.compute_rhs(ag.rhs_form(dt, **gk))
+        got = g.get(ag.U)
+        expect = o.compute_rhs(c, po.heat_form(dt, dim, **ok))
+        assert po.rel_l2(got, expect) < 1e-13, gk
+
+
+# ------------------------------------------------------------------ time stepping parity
+def test_config1_explicit_heat_2d_every_step():
+    """2D heat, p=2, 64x64 elements, dt=1e-5, 100 steps, zero Dirichlet (validation flow),
+    seeded Gaussian bump + noise: FP64 GPU vs oracle after every step."""
+    _log("case: 2D p=2 64^2 Dirichlet, 100 steps")
+    g, o = _pair(2, 2, (64, 64), dirichlet=True)
+    u = o.init_gaussians(seed=20240601, count=1, sigma=(0.1, 0.1), centre=(0.3, 0.7), noise=1e-3)
+    o.zero_boundary(u)
+    g.set(ag.U, u)
+    form = ag.rhs_form(1e-5)
+    worst = 0.0
+    for step in range(100):
+        rhs = o.compute_rhs(u, po.heat_form(1e-5, 2))
+        o.zero_boundary(rhs)
+        u = o.solve(rhs)
+        g.step(1, form, ag.STEP_DIRICHLET)
+        err = po.rel_l2(g.get(ag.U), u)
+        worst = max(worst, err)
+        assert err <= FP64_STEP_TOL, (step, err)
+    _log(f"worst per-step rel L2 = {worst:.3e}")
+
+
+def test_explicit_heat_3d_every_step():
+    _log("case: 3D p=2 24^3 natural BC, 10 steps")
+    g, o = _pair(3, 2, (24, 24, 24))
+    u = o.init_gaussians(seed=7, count=8, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+    g.set(ag.U, u)
+    form = ag.rhs_form(1e-4)
+    for step in range(10):
+        rhs = o.compute_rhs(u, po.heat_form(1e-4, 3), threads=8)
+        u = o.solve(rhs)
+        g.step(1, form)
+        err = po.rel_l2(g.get(ag.U), u)
+        assert err <= FP64_STEP_TOL, (step, err)
+
+
+def test_graph_replay_equals_eager_steps_bitwise():
+    _log("case: graph replay vs eager, 3D p=2 12^3")
+    a, _ = _pair(3, 2, (12, 12, 12))
+    b, o = _pair(3, 2, (12, 12, 12))
+    u = o.init_gaussians(seed=3, count=4, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+    a.set(ag.U, u)
+    b.set(ag.U, u)
+    form = ag.rhs_form(1e-4)
+    a.step(7, form)
+    for _ in range(7):
+        b.swap()
+        b.compute_rhs(form)
+        b.ads_solve()
+    assert np.array_equal(a.get(ag.U), b.get(ag.U))
+    # replaying the same graph again from the same state is deterministic
+    a.set(ag.U, u)
+    a.step(7, form)
+    assert np.array_equal(a.get(ag.U), b.get(ag.U))
+
+
+def test_split_implicit_step_2d():
+    """implicit_2d::step (implicit.hpp:102-111): rhs_1 (y explicit, dt/2) then
+    (Kx (x) My) solve, swap, rhs_2 (x explicit) then (Mx (x) Ky) solve."""
+    _log("case: 2D implicit split p=2 32^2")
+    dt = 1e-3
+    g, o = _pair(2, 2, (32, 32))
+    for d in range(2):
+        g.set_matrix(d, 1, 1.0, 0.5 * dt)
+        o.set_matrix(d, 1, 1.0, 0.5 * dt)
+    g.prepare_matrices()
+    o.prepare_matrices()
+    u = o.init_gaussians(seed=5, count=2, sigma=(0.05, 0.1), centre=(0.3, 0.7), noise=0.0)
+    g.set(ag.U, u)
+    for _ in range(5):
+        # oracle
+        r1 = o.compute_rhs(u, po.heat_form(0.5 * dt, 2, grad=(0.0, 1.0, 0.0)))
+        mid = o.solve(r1, slots=[1, 0])
+        r2 = o.compute_rhs(mid, po.heat_form(0.5 * dt, 2, grad=(1.0, 0.0, 0.0)))
+        u = o.solve(r2, slots=[0, 1])
+        # GPU
+        g.swap()
+        g.compute_rhs(ag.rhs_form(0.5 * dt, grad=(0.0, 1.0, 0.0)))
+        g.ads_solve([1, 0])
+        g.swap()
+        g.compute_rhs(ag.rhs_form(0.5 * dt, grad=(1.0, 0.0, 0.0)))
+        g.ads_solve([0, 1])
+        assert po.rel_l2(g.get(ag.U), u) <= FP64_STEP_TOL
+
+
+# ------------------------------------------------------------------ full-size properties
+def test_config2_full_size_one_step_and_conservation():
+    """3D p=2 128^3 (130^3 = 2,197,000 DOFs): one step against the threaded oracle, then
+    5 more steps checking that the integral of u (1^T M c) is conserved by the natural-BC
+    heat step and that a re-run is bitwise identical."""
+    _log("case: 3D p=2 128^3 one oracle step + conservation")
+    g, o = _pair(3, 2, (128, 128, 128))
+    u0 = o.init_gaussians(seed=2024, count=8, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+    g.set(ag.U, u0)
+    form = ag.rhs_form(1e-6)
+    g.step(1, form)
+    x1 = g.get(ag.U)
+    rhs = o.compute_rhs(u0, po.heat_form(1e-6, 3), threads=16)
+    y1 = o.solve(rhs)
+    assert po.rel_l2(x1, y1) <= FP64_STEP_TOL
+    ones = np.ones(g.size)
+    mass = o.compute_rhs(ones, po.heat_form(0.0, 3), threads=16)  # M 1
+    integral0 = float(mass @ u0)
+    g.step(5, form)
+    x6 = g.get(ag.U)
+    assert abs(float(mass @ x6) - integral0) <= 1e-12 * abs(integral0)
+    g.set(ag.U, u0)
+    g.step(6, form)
+    assert np.array_equal(g.get(ag.U), x6)
+
+
+def test_fp32_step_error_against_fp64_oracle():
+    """FP32 contractions + FP32 substitution on FP64-computed factors, 3D p=2 32^3,
+    20 steps: relative L2 error against the FP64 oracle stays below 1e-6 at every step."""
+    _log("case: FP32 3D p=2 32^3 20 steps")
+    g, o = _pair(3, 2, (32, 32, 32), precision=ag.FP32)
+    u = o.init_gaussians(seed=9, count=8, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+    g.set(ag.U, u)
+    form = ag.rhs_form(1e-6)
+    worst = 0.0
+    for step in range(20):
+        rhs = o.compute_rhs(u, po.heat_form(1e-6, 3), threads=8)
+        u = o.solve(rhs)
+        g.step(1, form)
+        err = po.rel_l2(g.get(ag.U), u)
+        worst = max(worst, err)
+        assert err <= 1e-6, (step, err)
+    _log(f"FP32 worst per-step rel L2 vs FP64 oracle = {worst:.3e}")
