@@ -60,7 +60,7 @@ EXPORTS = [
     "adsg_fix_dof", "adsg_set_matrix", "adsg_set_band_matrix", "adsg_prepare_matrices",
     "adsg_set_vector", "adsg_get_vector", "adsg_device_ptr", "adsg_swap", "adsg_compute_rhs",
     "adsg_zero_boundary", "adsg_ads_solve", "adsg_step", "adsg_synchronize",
-    "adsg_band_factorize",
+    "adsg_band_factorize", "adsg_time_phases",
 ]
 
 
@@ -94,6 +94,7 @@ def lib():
         L.adsg_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), C.c_int]
         L.adsg_synchronize.argtypes = [C.c_void_p]
         L.adsg_band_factorize.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip]
+        L.adsg_time_phases.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -196,6 +197,13 @@ class Simulation:
 
     def synchronize(self):
         self._check(lib().adsg_synchronize(self.h), "synchronize")
+
+    def time_phases(self, n, form, flags=0):
+        """Average device ms per step: rhs, boundary, sweep x, sweep y, sweep z, step."""
+        ms = np.zeros(6)
+        self._check(lib().adsg_time_phases(self.h, n, C.byref(form), flags, ms.ctypes.data_as(_dp)),
+                    "time_phases")
+        return ms
 
     def device_ptr(self, which):
         return lib().adsg_device_ptr(self.h, which)

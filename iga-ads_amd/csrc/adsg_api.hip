@@ -36,6 +36,7 @@ struct dev_dir {
     void* val = nullptr;  // [elements][nq][p+1] in the step precision
     void* der = nullptr;
     void* wj = nullptr;
+    void* rows = nullptr;  // FP32 mode: slot-0 matrix as n rows of 2p+1 entries
 };
 
 }  // namespace
@@ -48,7 +49,15 @@ struct adsg_sim {
     dev_dir dirs[3];
     long N = 0;
     hipStream_t stream = nullptr;
+    // u and u_prev are FP64 in both precisions; buffer, kfield and source are in the step
+    // precision. In FP32 mode shadow[0]/shadow[1] are FP32 copies of u/u_prev (they swap
+    // with them) and a step is computed as an increment: compute_rhs assembles
+    // rhs - M u_prev in FP32 into buffer, ads_solve turns it into M^-1 (rhs - M u_prev) with
+    // FP32 sweeps on FP64-computed factors and adds it to u_prev in FP64.
     void* vec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* shadow[2] = {nullptr, nullptr};
+    bool pending = false;            // FP32: buffer holds an increment based on u_prev
+    bool pending_dirichlet = false;  // FP32: boundary rows of the increment were fixed
     bool prepared = false;
     std::string err;
     // captured step graphs keyed by (form, flags, u pointer, steps in graph)
@@ -57,7 +66,11 @@ struct adsg_sim {
 
 namespace {
 
-size_t elt(const adsg_sim* h) { return h->precision == ADSG_FP32 ? sizeof(float) : sizeof(double); }
+// bytes per coefficient of a vector
+size_t elt_of(const adsg_sim* h, int which) {
+    if (which == ADSG_U || which == ADSG_U_PREV) return sizeof(double);
+    return h->precision == ADSG_FP32 ? sizeof(float) : sizeof(double);
+}
 
 int fail(adsg_sim* h, int code, const std::string& msg) {
     if (h) h->err = msg;
@@ -140,25 +153,31 @@ adsg::line_geometry geometry(const adsg_sim* h, int dir) {
 }
 
 template <typename T>
+int sweep_dir(adsg_sim* h, T* u, int d, int sl) {
+    if (sl < 0 || sl >= static_cast<int>(h->dirs[d].slots.size()) || !h->dirs[d].slots[sl].factored)
+        return fail(h, ADSG_ESTATE, "ads_solve: matrix slot not set or not factored");
+    const dev_slot& s = h->dirs[d].slots[sl];
+    adsg::dev_factors<T> f;
+    f.lower = static_cast<const T*>(s.lower);
+    f.upper = static_cast<const T*>(s.upper);
+    f.rdiag = static_cast<const T*>(s.rdiag);
+    f.poff = s.poff;
+    f.n = s.lf.n;
+    f.kl = s.lf.kl;
+    f.ku = s.lf.ku_eff;
+    f.pivoted = s.lf.pivoted;
+    hipError_t e = adsg::launch_line_sweep<T, T>(u, geometry(h, d), f, h->stream);
+    if (e == hipErrorInvalidValue) return fail(h, ADSG_EINVAL, "ads_solve: unsupported band widths (kl, ku)");
+    if (e != hipSuccess) return hip_fail(h, e, "line sweep");
+    return ADSG_OK;
+}
+
+// ads_solve: one sweep per direction in the order x, y, z (solver.hpp:35-41)
+template <typename T>
 int sweep_all(adsg_sim* h, T* u, const int* slots) {
     for (int d = 0; d < h->dim; ++d) {
-        const int sl = slots ? slots[d] : 0;
-        if (sl < 0 || sl >= static_cast<int>(h->dirs[d].slots.size()) || !h->dirs[d].slots[sl].factored)
-            return fail(h, ADSG_ESTATE, "ads_solve: matrix slot not set or not factored");
-        const dev_slot& s = h->dirs[d].slots[sl];
-        adsg::dev_factors<T> f;
-        f.lower = static_cast<const T*>(s.lower);
-        f.upper = static_cast<const T*>(s.upper);
-        f.rdiag = static_cast<const T*>(s.rdiag);
-        f.poff = s.poff;
-        f.n = s.lf.n;
-        f.kl = s.lf.kl;
-        f.ku = s.lf.ku_eff;
-        f.pivoted = s.lf.pivoted;
-        hipError_t e = adsg::launch_line_sweep<T, T>(u, geometry(h, d), f, h->stream);
-        if (e == hipErrorInvalidValue)
-            return fail(h, ADSG_EINVAL, "ads_solve: unsupported band widths (kl, ku)");
-        if (e != hipSuccess) return hip_fail(h, e, "line sweep");
+        int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0);
+        if (rc) return rc;
     }
     return ADSG_OK;
 }
@@ -188,27 +207,84 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form) {
         if (!h->vec[ADSG_SOURCE]) return fail(h, ADSG_ESTATE, "compute_rhs: source not set");
         prm.source = static_cast<const T*>(h->vec[ADSG_SOURCE]);
     }
-    T* rhs = static_cast<T*>(h->vec[ADSG_U]);
-    const T* up = static_cast<const T*>(h->vec[ADSG_U_PREV]);
+    T* rhs;
+    const T* up;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        // increment form: rhs - M u_prev, i.e. the integrand without its mass term
+        prm.mass = static_cast<T>(form->mass - 1.0);
+        rhs = static_cast<T*>(h->vec[ADSG_BUFFER]);
+        up = static_cast<const T*>(h->shadow[1]);
+    } else {
+        rhs = static_cast<T*>(h->vec[ADSG_U]);
+        up = static_cast<const T*>(h->vec[ADSG_U_PREV]);
+    }
     hipError_t e = adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
     if (e != hipSuccess) return hip_fail(h, e, "zero rhs");
     e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
+    if constexpr (sizeof(T) == sizeof(float)) {
+        h->pending = true;
+        h->pending_dirichlet = false;
+    }
     return ADSG_OK;
 }
 
+void extents(const adsg_sim* h, int& nx, int& ny, int& nz) {
+    nx = h->dirs[0].tab.dofs;
+    ny = h->dirs[1].tab.dofs;
+    nz = h->dim == 3 ? h->dirs[2].tab.dofs : 1;
+}
+
 int zero_boundary_impl(adsg_sim* h) {
-    const int nx = h->dirs[0].tab.dofs, ny = h->dirs[1].tab.dofs;
-    const int nz = h->dim == 3 ? h->dirs[2].tab.dofs : 1;
-    hipError_t e = h->precision == ADSG_FP32
-                       ? adsg::launch_zero_boundary<float>(h->dim, static_cast<float*>(h->vec[ADSG_U]), nx, ny, nz, h->stream)
-                       : adsg::launch_zero_boundary<double>(h->dim, static_cast<double*>(h->vec[ADSG_U]), nx, ny, nz, h->stream);
+    int nx, ny, nz;
+    extents(h, nx, ny, nz);
+    hipError_t e;
+    if (h->precision == ADSG_FP32) {
+        // fixed rows: the rhs entry is 0, so the increment entry is 0 - (M_f u_prev)_b
+        if (!h->pending) return fail(h, ADSG_ESTATE, "zero_boundary (FP32): call compute_rhs first");
+        for (int d = 0; d < h->dim; ++d)
+            if (!h->dirs[d].rows)
+                return fail(h, ADSG_ESTATE, "zero_boundary (FP32): slot 0 must be a factored degree-p band matrix");
+        e = adsg::launch_boundary_residual(
+            static_cast<float*>(h->vec[ADSG_BUFFER]), static_cast<const float*>(h->shadow[1]), h->dim,
+            nx, ny, nz, h->dirs[0].tab.p, static_cast<const float*>(h->dirs[0].rows),
+            static_cast<const float*>(h->dirs[1].rows),
+            h->dim == 3 ? static_cast<const float*>(h->dirs[2].rows) : nullptr, h->stream);
+        h->pending_dirichlet = true;
+    } else {
+        e = adsg::launch_zero_boundary<double>(h->dim, static_cast<double*>(h->vec[ADSG_U]), nx, ny,
+                                               nz, h->stream);
+    }
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "zero boundary");
 }
 
 int solve_impl(adsg_sim* h, const int* slots) {
-    if (h->precision == ADSG_FP32) return sweep_all<float>(h, static_cast<float*>(h->vec[ADSG_U]), slots);
-    return sweep_all<double>(h, static_cast<double*>(h->vec[ADSG_U]), slots);
+    if (h->precision == ADSG_FP64) return sweep_all<double>(h, static_cast<double*>(h->vec[ADSG_U]), slots);
+    // FP32: u = u_prev + A^-1 (rhs - M u_prev); valid when A is the Gram matrix of every
+    // direction (slot 0), which is what makes buffer = rhs - A u_prev
+    if (!h->pending) return fail(h, ADSG_ESTATE, "ads_solve (FP32): call compute_rhs first");
+    if (slots)
+        for (int d = 0; d < h->dim; ++d)
+            if (slots[d] != 0)
+                return fail(h, ADSG_EINVAL,
+                            "ads_solve (FP32): increments are defined against the Gram matrices (slot 0)");
+    int rc = sweep_all<float>(h, static_cast<float*>(h->vec[ADSG_BUFFER]), nullptr);
+    if (rc) return rc;
+    int nx, ny, nz;
+    extents(h, nx, ny, nz);
+    hipError_t e = adsg::launch_apply_increment(
+        static_cast<const double*>(h->vec[ADSG_U_PREV]), static_cast<const float*>(h->vec[ADSG_BUFFER]),
+        static_cast<double*>(h->vec[ADSG_U]), static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz,
+        h->pending_dirichlet, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "apply increment");
+    h->pending = false;
+    h->pending_dirichlet = false;
+    return ADSG_OK;
+}
+
+void swap_state(adsg_sim* h) {
+    std::swap(h->vec[ADSG_U], h->vec[ADSG_U_PREV]);
+    std::swap(h->shadow[0], h->shadow[1]);
 }
 
 int rhs_dispatch(adsg_sim* h, const adsg_rhs_form* form) {
@@ -217,7 +293,7 @@ int rhs_dispatch(adsg_sim* h, const adsg_rhs_form* form) {
 
 // one explicit step as validation.hpp:73-81 (before_step swap, compute_rhs, solve)
 int one_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
-    std::swap(h->vec[ADSG_U], h->vec[ADSG_U_PREV]);
+    swap_state(h);
     int rc = rhs_dispatch(h, form);
     if (rc) return rc;
     if (flags & ADSG_STEP_DIRICHLET) {
@@ -243,6 +319,8 @@ int get_graph(adsg_sim* h, const adsg_rhs_form* form, int flags, int steps, hipG
     }
     void* u0 = h->vec[ADSG_U];
     void* p0 = h->vec[ADSG_U_PREV];
+    void* s0 = h->shadow[0];
+    void* s1 = h->shadow[1];
     HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = ADSG_OK;
     for (int s = 0; s < steps && rc == ADSG_OK; ++s) rc = one_step(h, form, flags);
@@ -250,6 +328,9 @@ int get_graph(adsg_sim* h, const adsg_rhs_form* form, int flags, int steps, hipG
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     h->vec[ADSG_U] = u0;  // capture only recorded the swaps; restore the logical state
     h->vec[ADSG_U_PREV] = p0;
+    h->shadow[0] = s0;
+    h->shadow[1] = s1;
+    h->pending = false;
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -324,15 +405,25 @@ int adsg_create(const adsg_config* cfg, adsg_sim** out) {
         }
     }
     for (int v = ADSG_U; v <= ADSG_BUFFER; ++v) {
-        if (hipMalloc(&h->vec[v], h->N * elt(h)) != hipSuccess) {
+        if (hipMalloc(&h->vec[v], h->N * elt_of(h, v)) != hipSuccess) {
             g_create_error = "adsg_create: device allocation of the state vectors failed";
             adsg_destroy(h);
             return ADSG_ENOMEM;
         }
-        if (hipMemsetAsync(h->vec[v], 0, h->N * elt(h), h->stream) != hipSuccess) {
+        if (hipMemsetAsync(h->vec[v], 0, h->N * elt_of(h, v), h->stream) != hipSuccess) {
             g_create_error = "adsg_create: clearing the state vectors failed";
             adsg_destroy(h);
             return ADSG_EHIP;
+        }
+    }
+    if (h->precision == ADSG_FP32) {
+        for (auto& sp : h->shadow) {
+            if (hipMalloc(&sp, h->N * sizeof(float)) != hipSuccess ||
+                hipMemsetAsync(sp, 0, h->N * sizeof(float), h->stream) != hipSuccess) {
+                g_create_error = "adsg_create: device allocation of the FP32 shadows failed";
+                adsg_destroy(h);
+                return ADSG_ENOMEM;
+            }
         }
     }
     (void)hipStreamSynchronize(h->stream);
@@ -346,10 +437,12 @@ void adsg_destroy(adsg_sim* h) {
     drop_graphs(h);
     for (auto& d : h->dirs) {
         for (auto& s : d.slots) free_slot(s);
-        for (void** p : {&d.val, &d.der, &d.wj})
+        for (void** p : {&d.val, &d.der, &d.wj, &d.rows})
             if (*p) (void)hipFree(*p);
     }
     for (auto& v : h->vec)
+        if (v) (void)hipFree(v);
+    for (auto& v : h->shadow)
         if (v) (void)hipFree(v);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -437,6 +530,19 @@ int adsg_prepare_matrices(adsg_sim* h) {
             HIP_TRY(h, hipMemcpy(s.poff, s.lf.poff.data(), sizeof(int) * s.lf.n, hipMemcpyHostToDevice));
             s.factored = true;
         }
+        // FP32 increments need the rows of the (fixed) slot-0 matrix for boundary residuals
+        const dev_slot& s0 = d.slots[0];
+        if (h->precision == ADSG_FP32 && s0.factored && s0.kl == d.tab.p && s0.ku == d.tab.p) {
+            const int n = d.tab.dofs, p = d.tab.p, w = 2 * p + 1, ld = 3 * p + 1;
+            std::vector<double> rows(static_cast<size_t>(n) * w, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int t = 0; t < w; ++t) {
+                    const int j = i - p + t;
+                    if (j >= 0 && j < n) rows[static_cast<size_t>(i) * w + t] = s0.ab[static_cast<size_t>(j) * ld + 2 * p + i - j];
+                }
+            int rc = upload<float>(h, &d.rows, rows);
+            if (rc) return rc;
+        }
     }
     h->prepared = first_info == 0;
     drop_graphs(h);
@@ -446,18 +552,25 @@ int adsg_prepare_matrices(adsg_sim* h) {
 
 int adsg_set_vector(adsg_sim* h, int which, const double* host) {
     if (!h || !host || which < ADSG_U || which > ADSG_SOURCE) return fail(h, ADSG_EINVAL, "set_vector: bad argument");
+    const size_t es = elt_of(h, which);
     if (!h->vec[which]) {
-        if (hipMalloc(&h->vec[which], h->N * elt(h)) != hipSuccess)
+        if (hipMalloc(&h->vec[which], h->N * es) != hipSuccess)
             return fail(h, ADSG_ENOMEM, "set_vector: device allocation failed");
         drop_graphs(h);
     }
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    if (h->precision == ADSG_FP32) {
+    if (es == sizeof(float)) {
         std::vector<float> tmp(h->N);
         for (long i = 0; i < h->N; ++i) tmp[i] = static_cast<float>(host[i]);
         HIP_TRY(h, hipMemcpy(h->vec[which], tmp.data(), h->N * sizeof(float), hipMemcpyHostToDevice));
     } else {
         HIP_TRY(h, hipMemcpy(h->vec[which], host, h->N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (h->precision == ADSG_FP32 && (which == ADSG_U || which == ADSG_U_PREV)) {
+        hipError_t e = adsg::launch_convert_d2f(static_cast<const double*>(h->vec[which]),
+                                                static_cast<float*>(h->shadow[which]), h->N, h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "set_vector: shadow");
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
     }
     return ADSG_OK;
 }
@@ -466,7 +579,7 @@ int adsg_get_vector(adsg_sim* h, int which, double* host) {
     if (!h || !host || which < ADSG_U || which > ADSG_SOURCE || !h->vec[which])
         return fail(h, ADSG_EINVAL, "get_vector: bad argument");
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    if (h->precision == ADSG_FP32) {
+    if (elt_of(h, which) == sizeof(float)) {
         std::vector<float> tmp(h->N);
         HIP_TRY(h, hipMemcpy(tmp.data(), h->vec[which], h->N * sizeof(float), hipMemcpyDeviceToHost));
         for (long i = 0; i < h->N; ++i) host[i] = tmp[i];
@@ -483,7 +596,7 @@ void* adsg_device_ptr(adsg_sim* h, int which) {
 
 int adsg_swap(adsg_sim* h) {
     if (!h) return ADSG_EINVAL;
-    std::swap(h->vec[ADSG_U], h->vec[ADSG_U_PREV]);
+    swap_state(h);
     return ADSG_OK;
 }
 
@@ -515,7 +628,7 @@ int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags) {
     if (nsteps % 2) {
         if ((rc = get_graph(h, form, flags, 1, &g1))) return rc;
         HIP_TRY(h, hipGraphLaunch(g1, h->stream));
-        std::swap(h->vec[ADSG_U], h->vec[ADSG_U_PREV]);
+        swap_state(h);
     }
     return ADSG_OK;
 }
@@ -524,6 +637,53 @@ int adsg_synchronize(adsg_sim* h) {
     if (!h) return ADSG_EINVAL;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     return ADSG_OK;
+}
+
+int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags, double* ms) {
+    if (!h || !form || !ms || nsteps <= 0) return fail(h, ADSG_EINVAL, "time_phases: bad argument");
+    if (!h->prepared) return fail(h, ADSG_ESTATE, "time_phases: prepare_matrices first");
+    hipEvent_t ev[7];
+    for (auto& e : ev) HIP_TRY(h, hipEventCreate(&e));
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int rc = ADSG_OK;
+    for (int s = 0; s < nsteps && rc == ADSG_OK; ++s) {
+        swap_state(h);
+        (void)hipEventRecord(ev[0], h->stream);
+        rc = rhs_dispatch(h, form);
+        (void)hipEventRecord(ev[1], h->stream);
+        if (!rc && (flags & ADSG_STEP_DIRICHLET)) rc = zero_boundary_impl(h);
+        (void)hipEventRecord(ev[2], h->stream);
+        for (int d = 0; d < 3; ++d) {
+            if (!rc && d < h->dim) {
+                rc = h->precision == ADSG_FP32
+                         ? sweep_dir<float>(h, static_cast<float*>(h->vec[ADSG_BUFFER]), d, 0)
+                         : sweep_dir<double>(h, static_cast<double*>(h->vec[ADSG_U]), d, 0);
+            }
+            (void)hipEventRecord(ev[3 + d], h->stream);
+        }
+        if (!rc && h->precision == ADSG_FP32) {
+            int nx, ny, nz;
+            extents(h, nx, ny, nz);
+            hipError_t e = adsg::launch_apply_increment(
+                static_cast<const double*>(h->vec[ADSG_U_PREV]), static_cast<const float*>(h->vec[ADSG_BUFFER]),
+                static_cast<double*>(h->vec[ADSG_U]), static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz,
+                h->pending_dirichlet, h->stream);
+            if (e != hipSuccess) rc = hip_fail(h, e, "apply increment");
+            h->pending = h->pending_dirichlet = false;
+        }
+        (void)hipEventRecord(ev[6], h->stream);
+        HIP_TRY(h, hipEventSynchronize(ev[6]));
+        float t;
+        const int from[6] = {0, 1, 2, 3, 4, 0};
+        const int to[6] = {1, 2, 3, 4, 5, 6};
+        for (int i = 0; i < 6; ++i) {
+            (void)hipEventElapsedTime(&t, ev[from[i]], ev[to[i]]);
+            acc[i] += t;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int i = 0; i < 6; ++i) ms[i] = acc[i] / nsteps;
+    return rc;
 }
 
 int adsg_band_factorize(int n, int kl, int ku, double* ab, int* ipiv) {

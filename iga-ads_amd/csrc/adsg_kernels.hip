@@ -698,6 +698,62 @@ __global__ void k_f2d(const float* __restrict__ in, double* __restrict__ out, lo
         out[i] = static_cast<double>(in[i]);
 }
 
+__device__ __forceinline__ bool on_boundary(long i, int dim, int nx, int ny, int nz) {
+    const int x = static_cast<int>(i % nx);
+    const long r = i / nx;
+    const int y = static_cast<int>(r % ny), z = static_cast<int>(r / ny);
+    bool b = x == 0 || x == nx - 1 || y == 0 || y == ny - 1;
+    if (dim == 3) b = b || z == 0 || z == nz - 1;
+    return b;
+}
+
+__global__ void k_apply_increment(const double* __restrict__ xp, const float* __restrict__ d,
+                                  double* __restrict__ xn, float* __restrict__ sh, long n, int dim,
+                                  int nx, int ny, int nz, int dirichlet) {
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x) {
+        double v = xp[i] + static_cast<double>(d[i]);
+        if (dirichlet && on_boundary(i, dim, nx, ny, nz)) v = 0.0;
+        xn[i] = v;
+        sh[i] = static_cast<float>(v);
+    }
+}
+
+__global__ void k_boundary_residual(float* __restrict__ d, const float* __restrict__ sh, long n,
+                                    int dim, int nx, int ny, int nz, int p,
+                                    const float* __restrict__ rx, const float* __restrict__ ry,
+                                    const float* __restrict__ rz) {
+    const int w = 2 * p + 1;
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x) {
+        if (!on_boundary(i, dim, nx, ny, nz)) continue;
+        const int x = static_cast<int>(i % nx);
+        const long r = i / nx;
+        const int y = static_cast<int>(r % ny), z = static_cast<int>(r / ny);
+        float acc = 0.0f;
+        const int z0 = dim == 3 ? -p : 0, z1 = dim == 3 ? p : 0;
+        for (int c = z0; c <= z1; ++c) {
+            const int zz = z + c;
+            if (zz < 0 || zz >= nz) continue;
+            const float wz = dim == 3 ? rz[static_cast<long>(z) * w + c + p] : 1.0f;
+            if (wz == 0.0f) continue;
+            for (int b = -p; b <= p; ++b) {
+                const int yy = y + b;
+                if (yy < 0 || yy >= ny) continue;
+                const float wy = ry[static_cast<long>(y) * w + b + p];
+                if (wy == 0.0f) continue;
+                for (int a = -p; a <= p; ++a) {
+                    const int xx = x + a;
+                    if (xx < 0 || xx >= nx) continue;
+                    acc += rx[static_cast<long>(x) * w + a + p] * wy * wz *
+                           sh[xx + static_cast<long>(nx) * (yy + static_cast<long>(ny) * zz)];
+                }
+            }
+        }
+        d[i] = -acc;
+    }
+}
+
 inline unsigned grid_for(long n) {
     long b = (n + 255) / 256;
     return static_cast<unsigned>(b < 4096 ? (b > 0 ? b : 1) : 4096);
@@ -763,6 +819,24 @@ hipError_t launch_convert_d2f(const double* in, float* out, long n, hipStream_t 
 
 hipError_t launch_convert_f2d(const float* in, double* out, long n, hipStream_t s) {
     hipLaunchKernelGGL(k_f2d, dim3(grid_for(n)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_increment(const double* x_prev, const float* delta, double* x_new,
+                                  float* shadow, int dim, int nx, int ny, int nz, bool dirichlet,
+                                  hipStream_t s) {
+    const long n = static_cast<long>(nx) * ny * nz;
+    hipLaunchKernelGGL(k_apply_increment, dim3(grid_for(n)), dim3(256), 0, s, x_prev, delta, x_new,
+                       shadow, n, dim, nx, ny, nz, dirichlet ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary_residual(float* delta, const float* shadow, int dim, int nx, int ny,
+                                    int nz, int p, const float* rows_x, const float* rows_y,
+                                    const float* rows_z, hipStream_t s) {
+    const long n = static_cast<long>(nx) * ny * nz;
+    hipLaunchKernelGGL(k_boundary_residual, dim3(grid_for(n)), dim3(256), 0, s, delta, shadow, n,
+                       dim, nx, ny, nz, p, rows_x, rows_y, rows_z);
     return hipGetLastError();
 }
 

@@ -60,4 +60,17 @@ hipError_t launch_zero_boundary(int dim, T* u, int nx, int ny, int nz, hipStream
 hipError_t launch_convert_d2f(const double* in, float* out, long n, hipStream_t s);
 hipError_t launch_convert_f2d(const float* in, double* out, long n, hipStream_t s);
 
+// FP32 increment step: x_new = x_prev + delta (FP64), shadow = (float) x_new; with
+// dirichlet, boundary coefficients of x_new are set to 0 exactly.
+hipError_t launch_apply_increment(const double* x_prev, const float* delta, double* x_new,
+                                  float* shadow, int dim, int nx, int ny, int nz, bool dirichlet,
+                                  hipStream_t s);
+// Residual of the fixed boundary rows: delta_b = 0 - (M_f u)_b, where M_f is the Kronecker
+// product of the 1D Gram matrices with their boundary rows fixed (dimension.cpp:23-29).
+// rows_d holds the fixed 1D matrix of direction d as n_d rows of 2p+1 entries (columns
+// i-p .. i+p).
+hipError_t launch_boundary_residual(float* delta, const float* shadow, int dim, int nx, int ny,
+                                    int nz, int p, const float* rows_x, const float* rows_y,
+                                    const float* rows_z, hipStream_t s);
+
 }  // namespace adsg

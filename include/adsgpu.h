@@ -137,6 +137,13 @@ int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags);
 
 int adsg_synchronize(adsg_sim* h);
 
+/* Device time per phase of nsteps eagerly launched steps (same sequence as adsg_step),
+ * measured with HIP events on the handle's stream -- the counterpart of the reference's
+ * StatTimer around compute_rhs (examples/scalability/test3d.hpp:67-101). Averages per step
+ * in milliseconds: ms[0] rhs assembly (zero fill + colour passes), ms[1] boundary zeroing,
+ * ms[2], ms[3], ms[4] line sweeps in x, y, z, ms[5] the whole step. */
+int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags, double* ms);
+
 /* lin::factorize + lin::solve_with_factorized (band_solve.hpp:16-31) on the host in
  * FP64, for callers that hold their own band matrices: factor ab (n x n, kl, ku, full
  * buffer as above) in place, pivots 1-based like dgbtrf. Returns LAPACK info. */
