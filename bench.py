@@ -34,11 +34,15 @@ def f_elem(d, p):
     return 2 * c * 2 * n ** (d + 1) + (d + 2) * n ** d
 
 
-def run_form(ag, po, precision, ne, steps, warmup, u0, dt):
+def run_form(ag, po, precision, ne, steps, warmup, u0, dt, variant=None):
     sim = ag.Simulation(3, [2, 2, 2], [ne, ne, ne], precision=precision)
     sim.prepare_matrices()
     sim.set(ag.U, u0)
     form = ag.rhs_form(dt)
+    if variant is None:
+        variant = sim.autotune(form)  # picks the element-kernel variant; state is restored
+    else:
+        sim.set_tuning(ag.TUNE_RHS_VARIANT, variant)
     sim.step(warmup, form)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -49,7 +53,7 @@ def run_form(ag, po, precision, ne, steps, warmup, u0, dt):
     sim.synchronize()
     out = sim.get(ag.U)
     sim.close()
-    return (t1 - t0), phases, out
+    return (t1 - t0), phases, out, variant
 
 
 def main():
@@ -60,6 +64,8 @@ def main():
     ap.add_argument("--elements", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--rhs-variant", type=int, default=None,
+                    help="element-kernel variant (default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the ADS step here runs on a single MI355X "
@@ -80,7 +86,8 @@ def main():
     N = init.size
     E = ne ** 3
 
-    t64, ph64, x64 = run_form(ag, po, ag.FP64, ne, args.steps, args.warmup, u0, dt)
+    t64, ph64, x64, var64 = run_form(ag, po, ag.FP64, ne, args.steps, args.warmup, u0, dt,
+                                     args.rhs_variant)
     value = N * args.steps / t64
     ms_step = 1e3 * t64 / args.steps
 
@@ -122,6 +129,7 @@ def main():
         },
         "phases_ms": {"rhs": ph64[0], "sweep_x": ph64[2], "sweep_y": ph64[3],
                       "sweep_z": ph64[4], "step_eager": ph64[5]},
+        "rhs_variant": var64,
         "solve_roofline": {
             "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
             "achieved_xyz": sweep_gbs,
@@ -131,7 +139,8 @@ def main():
     }
 
     if not args.no_fp32:
-        t32, ph32, x32 = run_form(ag, po, ag.FP32, ne, args.steps, args.warmup, u0, dt)
+        t32, ph32, x32, var32 = run_form(ag, po, ag.FP32, ne, args.steps, args.warmup, u0, dt,
+                                         args.rhs_variant)
         err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
         result["forms"] = {
             "fp64": {"value": value, "ms_per_step": ms_step},
@@ -140,6 +149,7 @@ def main():
                 "ms_per_step": 1e3 * t32 / args.steps,
                 "speedup_vs_fp64": t64 / t32,
                 "rel_l2_vs_fp64_after_all_steps": err,
+                "rhs_variant": var32,
                 "phases_ms": {"rhs": ph32[0], "sweep_x": ph32[2], "sweep_y": ph32[3],
                               "sweep_z": ph32[4], "step_eager": ph32[5]},
             },

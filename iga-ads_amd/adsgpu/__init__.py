@@ -60,8 +60,11 @@ EXPORTS = [
     "adsg_fix_dof", "adsg_set_matrix", "adsg_set_band_matrix", "adsg_prepare_matrices",
     "adsg_set_vector", "adsg_get_vector", "adsg_device_ptr", "adsg_swap", "adsg_compute_rhs",
     "adsg_zero_boundary", "adsg_ads_solve", "adsg_step", "adsg_synchronize",
-    "adsg_band_factorize", "adsg_time_phases",
+    "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
+    "adsg_autotune",
 ]
+TUNE_RHS_VARIANT = 0
+TUNE_SWEEP_VARIANT = 1
 
 
 def lib():
@@ -95,6 +98,9 @@ def lib():
         L.adsg_synchronize.argtypes = [C.c_void_p]
         L.adsg_band_factorize.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip]
         L.adsg_time_phases.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), C.c_int, _dp]
+        L.adsg_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.adsg_get_tuning.argtypes = [C.c_void_p, C.c_int]
+        L.adsg_autotune.argtypes = [C.c_void_p, C.POINTER(RhsForm), C.c_int]
         _lib = L
     return _lib
 
@@ -204,6 +210,16 @@ class Simulation:
         self._check(lib().adsg_time_phases(self.h, n, C.byref(form), flags, ms.ctypes.data_as(_dp)),
                     "time_phases")
         return ms
+
+    def set_tuning(self, key, value):
+        self._check(lib().adsg_set_tuning(self.h, key, value), "set_tuning")
+
+    def get_tuning(self, key):
+        return lib().adsg_get_tuning(self.h, key)
+
+    def autotune(self, form, flags=0):
+        self._check(lib().adsg_autotune(self.h, C.byref(form), flags), "autotune")
+        return self.get_tuning(TUNE_RHS_VARIANT)
 
     def device_ptr(self, which):
         return lib().adsg_device_ptr(self.h, which)

@@ -28,6 +28,10 @@ struct dev_slot {
     void* upper = nullptr;
     void* rdiag = nullptr;
     int* poff = nullptr;
+    // segmented sweep (unpivoted factors only)
+    adsg::line_segments seg;
+    void* gf = nullptr;
+    void* hb = nullptr;
 };
 
 struct dev_dir {
@@ -59,6 +63,8 @@ struct adsg_sim {
     bool pending = false;            // FP32: buffer holds an increment based on u_prev
     bool pending_dirichlet = false;  // FP32: boundary rows of the increment were fixed
     bool prepared = false;
+    int rhs_variant = 0;    // element-kernel tuning choice (adsg_set_tuning)
+    int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
     std::string err;
     // captured step graphs keyed by (form, flags, u pointer, steps in graph)
     std::map<std::tuple<std::string, int, void*, int>, hipGraphExec_t> graphs;
@@ -116,7 +122,7 @@ int upload_tables(adsg_sim* h, dev_dir& d) {
 }
 
 void free_slot(dev_slot& s) {
-    for (void** p : {&s.lower, &s.upper, &s.rdiag}) {
+    for (void** p : {&s.lower, &s.upper, &s.rdiag, &s.gf, &s.hb}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
@@ -166,7 +172,19 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl) {
     f.kl = s.lf.kl;
     f.ku = s.lf.ku_eff;
     f.pivoted = s.lf.pivoted;
-    hipError_t e = adsg::launch_line_sweep<T, T>(u, geometry(h, d), f, h->stream);
+    hipError_t e;
+    if (h->sweep_variant == 0 && s.seg.nseg > 0) {
+        adsg::seg_factors<T> sf;
+        sf.f = f;
+        sf.gf = static_cast<const T*>(s.gf);
+        sf.hb = static_cast<const T*>(s.hb);
+        sf.m = s.seg.m;
+        sf.nseg = s.seg.nseg;
+        sf.mseg = s.seg.mseg;
+        e = adsg::launch_line_sweep_seg<T, T>(u, geometry(h, d), sf, h->stream);
+    } else {
+        e = adsg::launch_line_sweep<T, T>(u, geometry(h, d), f, h->stream);
+    }
     if (e == hipErrorInvalidValue) return fail(h, ADSG_EINVAL, "ads_solve: unsupported band widths (kl, ku)");
     if (e != hipSuccess) return hip_fail(h, e, "line sweep");
     return ADSG_OK;
@@ -220,7 +238,7 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form) {
     }
     hipError_t e = adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
     if (e != hipSuccess) return hip_fail(h, e, "zero rhs");
-    e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->stream);
+    e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->rhs_variant, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     if constexpr (sizeof(T) == sizeof(float)) {
         h->pending = true;
@@ -525,6 +543,18 @@ int adsg_prepare_matrices(adsg_sim* h) {
                 if ((rc = upload<double>(h, &s.upper, s.lf.upper))) return rc;
                 if ((rc = upload<double>(h, &s.rdiag, s.lf.rdiag))) return rc;
             }
+            s.seg = adsg::line_segments{};
+            if (adsg::segment_lines(s.lf, s.seg)) {
+                if (h->precision == ADSG_FP32) {
+                    if ((rc = upload<float>(h, &s.gf, s.seg.gf))) return rc;
+                    if ((rc = upload<float>(h, &s.hb, s.seg.hb))) return rc;
+                } else {
+                    if ((rc = upload<double>(h, &s.gf, s.seg.gf))) return rc;
+                    if ((rc = upload<double>(h, &s.hb, s.seg.hb))) return rc;
+                }
+            } else {
+                s.seg = adsg::line_segments{};
+            }
             if (hipMalloc(reinterpret_cast<void**>(&s.poff), sizeof(int) * s.lf.n) != hipSuccess)
                 return fail(h, ADSG_ENOMEM, "hipMalloc failed for pivots");
             HIP_TRY(h, hipMemcpy(s.poff, s.lf.poff.data(), sizeof(int) * s.lf.n, hipMemcpyHostToDevice));
@@ -683,6 +713,80 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int fla
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (int i = 0; i < 6; ++i) ms[i] = acc[i] / nsteps;
+    return rc;
+}
+
+int adsg_set_tuning(adsg_sim* h, int key, int value) {
+    if (!h) return ADSG_EINVAL;
+    int* slot = key == ADSG_TUNE_RHS_VARIANT ? &h->rhs_variant
+                : key == ADSG_TUNE_SWEEP_VARIANT ? &h->sweep_variant
+                                                 : nullptr;
+    if (!slot || value < 0 || value > 1) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
+    if (*slot != value) drop_graphs(h);
+    *slot = value;
+    return ADSG_OK;
+}
+
+int adsg_get_tuning(const adsg_sim* h, int key) {
+    if (!h) return ADSG_EINVAL;
+    if (key == ADSG_TUNE_RHS_VARIANT) return h->rhs_variant;
+    if (key == ADSG_TUNE_SWEEP_VARIANT) return h->sweep_variant;
+    return ADSG_EINVAL;
+}
+
+int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, int flags) {
+    if (!h || !form) return fail(h, ADSG_EINVAL, "autotune: bad argument");
+    if (!h->prepared) return fail(h, ADSG_ESTATE, "autotune: prepare_matrices first");
+    // keep the caller's state: copies of u, u_prev (and their FP32 shadows)
+    std::vector<void*> saved;
+    std::vector<std::pair<void*, size_t>> what = {{h->vec[ADSG_U], h->N * sizeof(double)},
+                                                  {h->vec[ADSG_U_PREV], h->N * sizeof(double)}};
+    if (h->precision == ADSG_FP32) {
+        what.push_back({h->shadow[0], h->N * sizeof(float)});
+        what.push_back({h->shadow[1], h->N * sizeof(float)});
+    }
+    void* ptrs[4] = {h->vec[ADSG_U], h->vec[ADSG_U_PREV], h->shadow[0], h->shadow[1]};
+    for (auto& w : what) {
+        void* p = nullptr;
+        if (hipMalloc(&p, w.second) != hipSuccess) {
+            for (void* q : saved) (void)hipFree(q);
+            return fail(h, ADSG_ENOMEM, "autotune: no memory for the state copy");
+        }
+        (void)hipMemcpyAsync(p, w.first, w.second, hipMemcpyDeviceToDevice, h->stream);
+        saved.push_back(p);
+    }
+    int best = h->rhs_variant, best_sweep = h->sweep_variant;
+    double best_ms = 1e30, best_sweep_ms = 1e30;
+    int rc = ADSG_OK;
+    for (int v = 0; v <= 1 && rc == ADSG_OK; ++v) {
+        h->rhs_variant = v;
+        h->sweep_variant = v;
+        double ms[6];
+        rc = adsg_time_phases(h, 1, form, flags, ms);  // warm-up (code load, caches)
+        if (!rc) rc = adsg_time_phases(h, 3, form, flags, ms);
+        if (!rc && ms[0] < best_ms) {
+            best_ms = ms[0];
+            best = v;
+        }
+        const double sweeps = ms[2] + ms[3] + ms[4];
+        if (!rc && sweeps < best_sweep_ms) {
+            best_sweep_ms = sweeps;
+            best_sweep = v;
+        }
+    }
+    // restore the state and its pointer assignment
+    h->vec[ADSG_U] = ptrs[0];
+    h->vec[ADSG_U_PREV] = ptrs[1];
+    h->shadow[0] = ptrs[2];
+    h->shadow[1] = ptrs[3];
+    for (size_t i = 0; i < what.size(); ++i)
+        (void)hipMemcpyAsync(what[i].first, saved[i], what[i].second, hipMemcpyDeviceToDevice, h->stream);
+    (void)hipStreamSynchronize(h->stream);
+    for (void* q : saved) (void)hipFree(q);
+    h->pending = h->pending_dirichlet = false;
+    h->rhs_variant = best;
+    h->sweep_variant = best_sweep;
+    drop_graphs(h);
     return rc;
 }
 

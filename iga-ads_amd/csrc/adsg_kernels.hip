@@ -235,9 +235,231 @@ hipError_t sweep_dispatch(T* u, const line_geometry& g, const dev_factors<F>& f,
 }
 
 // ------------------------------------------------------------------------------------
+// segmented line sweeps (unpivoted factors)
+//
+// A workgroup owns 64 lines; wave w owns segment w of all 64 (lane = line), so every
+// line is solved by nseg waves at once and each coefficient is read and written once.
+// Forward: y = y0 + G t, where y0 solves the segment with a zero tail and t is the true
+// tail (last kl values of y) of the previous segment; the tails are chained through LDS
+// by one wave. Backward likewise with the heads (first ku values of x) of the next
+// segment. Recurrences are in row form: y_j = b_j - sum_t L(j, j-t) y_{j-t},
+// x_j = (y_j - sum_t U(j, j+t) x_{j+t}) / U(j, j).
+// ------------------------------------------------------------------------------------
+template <typename T, typename F, int KL, int KU, int MSEG, bool CONTIG>
+__global__ __launch_bounds__(1024) void k_sweep_seg(T* __restrict__ u, long n_lines, long A,
+                                                    long B, long S, seg_factors<F> sf) {
+    extern __shared__ double smem_raw[];
+    constexpr int KM = KL > KU ? KL : KU;
+    const dev_factors<F>& f = sf.f;
+    const int n = f.n, m = sf.m, nseg = sf.nseg;
+    const int lane = threadIdx.x & 63;
+    const int seg = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long line0 = static_cast<long>(blockIdx.x) * 64;
+    const long line = line0 + lane;
+    const int j0 = seg * m;
+    const int len = n - j0 < m ? n - j0 : m;  // >= KM by construction
+
+    T* bound = reinterpret_cast<T*>(smem_raw);  // [nseg][KM][64] tails, then heads
+    T(*stage)[MSEG + 1] = reinterpret_cast<T(*)[MSEG + 1]>(bound + nseg * KM * 64) + seg * 64;
+
+    T vals[MSEG];
+    T* p = nullptr;
+    if constexpr (CONTIG) {
+#pragma unroll
+        for (int r = 0; r < MSEG; ++r) {
+            const int idx = r * 64 + lane;
+            const int ln = idx / MSEG, k = idx % MSEG;
+            const long g = line0 + ln;
+            stage[ln][k] = (g < n_lines && k < len) ? u[g * n + j0 + k] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < MSEG; ++k) vals[k] = stage[lane][k];
+    } else {
+        p = u + (line % A) + (line / A) * B;
+        const bool act = line < n_lines;
+#pragma unroll
+        for (int k = 0; k < MSEG; ++k)
+            vals[k] = (act && k < len) ? p[static_cast<long>(j0 + k) * S] : T(0);
+    }
+
+    // forward, zero tail
+    {
+        T hist[KL];
+#pragma unroll
+        for (int t = 0; t < KL; ++t) hist[t] = T(0);
+#pragma unroll
+        for (int k = 0; k < MSEG; ++k) {
+            if (k < len) {
+                const int j = j0 + k;
+                T acc = vals[k];
+#pragma unroll
+                for (int t = 1; t <= KL; ++t)
+                    if (j - t >= 0) acc -= static_cast<T>(f.lower[static_cast<long>(j - t) * KL + t - 1]) * hist[t - 1];
+#pragma unroll
+                for (int t = KL - 1; t > 0; --t) hist[t] = hist[t - 1];
+                hist[0] = acc;
+                vals[k] = acc;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < KL; ++t) {  // tail position len-KL+t
+            T tv = T(0);
+#pragma unroll
+            for (int k = 0; k < MSEG; ++k)
+                if (k == len - KL + t) tv = vals[k];
+            bound[(seg * KM + t) * 64 + lane] = tv;
+        }
+    }
+    __syncthreads();
+    if (seg == 0) {
+        for (int s = 1; s < nseg; ++s) {
+            const int jt = s * m + (n - s * m < m ? n - s * m : m) - KL;  // first tail position of s
+            T prev[KL];
+#pragma unroll
+            for (int t = 0; t < KL; ++t) prev[t] = bound[((s - 1) * KM + t) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < KL; ++t) {
+                T acc = bound[(s * KM + t) * 64 + lane];
+#pragma unroll
+                for (int q = 0; q < KL; ++q)
+                    acc += static_cast<T>(sf.gf[static_cast<long>(jt + t) * KL + q]) * prev[q];
+                bound[(s * KM + t) * 64 + lane] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    if (seg > 0) {
+        T tail[KL];
+#pragma unroll
+        for (int t = 0; t < KL; ++t) tail[t] = bound[((seg - 1) * KM + t) * 64 + lane];
+#pragma unroll
+        for (int k = 0; k < MSEG; ++k) {
+            if (k < len) {
+                T acc = vals[k];
+#pragma unroll
+                for (int q = 0; q < KL; ++q)
+                    acc += static_cast<T>(sf.gf[static_cast<long>(j0 + k) * KL + q]) * tail[q];
+                vals[k] = acc;
+            }
+        }
+    }
+    __syncthreads();  // every wave has read its tail before the heads overwrite it
+
+    // backward, zero head
+    {
+        T hist[KU];
+#pragma unroll
+        for (int t = 0; t < KU; ++t) hist[t] = T(0);
+#pragma unroll
+        for (int k = MSEG - 1; k >= 0; --k) {
+            if (k < len) {
+                const int j = j0 + k;
+                T acc = vals[k];
+#pragma unroll
+                for (int t = 1; t <= KU; ++t)
+                    if (j + t < n) acc -= static_cast<T>(f.upper[static_cast<long>(j + t) * KU + t - 1]) * hist[t - 1];
+                acc *= static_cast<T>(f.rdiag[j]);
+#pragma unroll
+                for (int t = KU - 1; t > 0; --t) hist[t] = hist[t - 1];
+                hist[0] = acc;
+                vals[k] = acc;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < KU; ++t) bound[(seg * KM + t) * 64 + lane] = vals[t];
+    }
+    __syncthreads();
+    if (seg == 0) {
+        for (int s = nseg - 2; s >= 0; --s) {
+            const int jh = s * m;  // first head position of segment s
+            T next[KU];
+#pragma unroll
+            for (int t = 0; t < KU; ++t) next[t] = bound[((s + 1) * KM + t) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < KU; ++t) {
+                T acc = bound[(s * KM + t) * 64 + lane];
+#pragma unroll
+                for (int q = 0; q < KU; ++q)
+                    acc += static_cast<T>(sf.hb[static_cast<long>(jh + t) * KU + q]) * next[q];
+                bound[(s * KM + t) * 64 + lane] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    if (seg < nseg - 1) {
+        T head[KU];
+#pragma unroll
+        for (int t = 0; t < KU; ++t) head[t] = bound[((seg + 1) * KM + t) * 64 + lane];
+#pragma unroll
+        for (int k = 0; k < MSEG; ++k) {
+            if (k < len) {
+                T acc = vals[k];
+#pragma unroll
+                for (int q = 0; q < KU; ++q)
+                    acc += static_cast<T>(sf.hb[static_cast<long>(j0 + k) * KU + q]) * head[q];
+                vals[k] = acc;
+            }
+        }
+    }
+
+    if constexpr (CONTIG) {
+#pragma unroll
+        for (int k = 0; k < MSEG; ++k) stage[lane][k] = vals[k];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < MSEG; ++r) {
+            const int idx = r * 64 + lane;
+            const int ln = idx / MSEG, k = idx % MSEG;
+            const long g = line0 + ln;
+            if (g < n_lines && k < len) u[g * n + j0 + k] = stage[ln][k];
+        }
+    } else {
+        if (line < n_lines) {
+#pragma unroll
+            for (int k = 0; k < MSEG; ++k)
+                if (k < len) p[static_cast<long>(j0 + k) * S] = vals[k];
+        }
+    }
+}
+
+template <typename T, typename F, int KL, int KU, int MSEG>
+hipError_t seg_dispatch_m(T* u, const line_geometry& g, const seg_factors<F>& sf, hipStream_t s) {
+    constexpr int KM = KL > KU ? KL : KU;
+    const unsigned blocks = static_cast<unsigned>((g.n_lines + 63) / 64);
+    const unsigned threads = static_cast<unsigned>(sf.nseg * 64);
+    size_t shm = static_cast<size_t>(sf.nseg) * KM * 64 * sizeof(T);
+    if (g.S == 1) {
+        shm += static_cast<size_t>(sf.nseg) * 64 * (MSEG + 1) * sizeof(T);
+        // staging would not fit the 160 KiB of LDS: one lane per whole line instead
+        if (shm > 150 * 1024) return sweep_dispatch<T, F, KL, KU>(u, g, sf.f, s);
+        auto kern = k_sweep_seg<T, F, KL, KU, MSEG, true>;
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm));
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), shm, s, u, g.n_lines, g.A, g.B, g.S, sf);
+    } else {
+        auto kern = k_sweep_seg<T, F, KL, KU, MSEG, false>;
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm));
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), shm, s, u, g.n_lines, g.A, g.B, g.S, sf);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, typename F, int KL, int KU>
+hipError_t seg_dispatch(T* u, const line_geometry& g, const seg_factors<F>& sf, hipStream_t s) {
+    if (g.n_lines <= 0 || sf.f.n <= 0) return hipSuccess;
+    if (sf.mseg == 16) return seg_dispatch_m<T, F, KL, KU, 16>(u, g, sf, s);
+    if (sf.mseg == 32) return seg_dispatch_m<T, F, KL, KU, 32>(u, g, sf, s);
+    return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------------------------
 // element-loop RHS, 3D
 // ------------------------------------------------------------------------------------
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC>
+// CREG: hold the element's coefficient block in registers (fewer LDS reads) or re-read it
+// from LDS for every x Gauss point (fewer registers, more waves per SIMD).
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool CREG>
 __global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T* __restrict__ rhs,
                                                      dev_tables<T> X, dev_tables<T> Y,
                                                      dev_tables<T> Z, rhs_params<T> prm,
@@ -245,12 +467,13 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T
     constexpr int M = P + 1, NT = TX * TY * TZ;
     constexpr int BX = TX + P, BY = TY + P, BZ = TZ + P, NB = BX * BY * BZ;
     constexpr int M3 = M * M * M;
+    // one wave = one z-layer of the tile, so everything indexed by z is wave-uniform
+    static_assert(TX * TY == 64, "a wave must cover exactly one z-layer of the tile");
     __shared__ T blk[NB];
     __shared__ T kb[KF ? NB : 1];
     __shared__ T sb[SRC ? NB : 1];
     __shared__ T tvx[TX * M * M], tdx[TX * M * M], twx[TX * M];
     __shared__ T tvy[TY * M * M], tdy[TY * M * M], twy[TY * M];
-    __shared__ T tvz[TZ * M * M], tdz[TZ * M * M], twz[TZ * M];
     __shared__ T Rs[M3 * NT];
 
     const int tid = threadIdx.x;
@@ -283,34 +506,56 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T
         tvy[i] = in ? Y.val[static_cast<long>(ey0) * M * M + i] : T(0);
         tdy[i] = in ? Y.der[static_cast<long>(ey0) * M * M + i] : T(0);
     }
-    for (int i = tid; i < TZ * M * M; i += NT) {
-        const bool in = ez0 + i / (M * M) < Z.elements;
-        tvz[i] = in ? Z.val[static_cast<long>(ez0) * M * M + i] : T(0);
-        tdz[i] = in ? Z.der[static_cast<long>(ez0) * M * M + i] : T(0);
-    }
     for (int i = tid; i < TX * M; i += NT)
         twx[i] = (ex0 + i / M < X.elements) ? X.wj[static_cast<long>(ex0) * M + i] : T(0);
     for (int i = tid; i < TY * M; i += NT)
         twy[i] = (ey0 + i / M < Y.elements) ? Y.wj[static_cast<long>(ey0) * M + i] : T(0);
-    for (int i = tid; i < TZ * M; i += NT)
-        twz[i] = (ez0 + i / M < Z.elements) ? Z.wj[static_cast<long>(ez0) * M + i] : T(0);
     __syncthreads();
 
-    const int lx = tid % TX, ly = (tid / TX) % TY, lz = tid / (TX * TY);
-    const bool valid = ex0 + lx < X.elements && ey0 + ly < Y.elements && ez0 + lz < Z.elements;
-    if (!valid) {
+    const int lx = tid % TX, ly = (tid / TX) % TY;
+    const int lz = __builtin_amdgcn_readfirstlane(tid / (TX * TY));
+    const int ez = ez0 + lz;
+
+    T R[M][M][M];  // the element's contribution, (a, b, c) = local (x, y, z) function
 #pragma unroll
-        for (int i = 0; i < M3; ++i) Rs[i * NT + tid] = T(0);
-    } else {
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = 0; b < M; ++b)
+#pragma unroll
+            for (int c = 0; c < M; ++c) R[a][b][c] = T(0);
+
+    if (ez < Z.elements) {  // wave-uniform
+        // z tables of the wave's element layer: uniform addresses in uniform control flow,
+        // so they are read with scalar loads into scalar registers
         T Bz[M][M], Dz[M][M], wz[M];
+        const T* zv = Z.val + static_cast<long>(ez) * M * M;
+        const T* zd = Z.der + static_cast<long>(ez) * M * M;
 #pragma unroll
         for (int q = 0; q < M; ++q) {
-            wz[q] = twz[lz * M + q];
+            wz[q] = Z.wj[static_cast<long>(ez) * M + q];
 #pragma unroll
             for (int c = 0; c < M; ++c) {
-                Bz[q][c] = tvz[(lz * M + q) * M + c];
-                Dz[q][c] = tdz[(lz * M + q) * M + c];
+                Bz[q][c] = zv[q * M + c];
+                Dz[q][c] = zd[q * M + c];
             }
+        }
+        if (ex0 + lx < X.elements && ey0 + ly < Y.elements) {
+        // the element's coefficients
+        T cf[CREG ? M : 1][CREG ? M : 1][CREG ? M : 1];
+        T kc[KF && CREG ? M : 1][KF && CREG ? M : 1][KF && CREG ? M : 1];
+        T sc[SRC && CREG ? M : 1][SRC && CREG ? M : 1][SRC && CREG ? M : 1];
+        if constexpr (CREG) {
+#pragma unroll
+            for (int a = 0; a < M; ++a)
+#pragma unroll
+                for (int b = 0; b < M; ++b)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        const int li = (lx + a) + BX * ((ly + b) + BY * (lz + c));
+                        cf[a][b][c] = blk[li];
+                        if constexpr (KF) kc[a][b][c] = kb[li];
+                        if constexpr (SRC) sc[a][b][c] = sb[li];
+                    }
         }
 #pragma unroll 1
         for (int qx = 0; qx < M; ++qx) {
@@ -321,7 +566,7 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T
                 dx[a] = tdx[(lx * M + qx) * M + a];
             }
             const T wx = twx[lx * M + qx];
-            // stage x: contract the first index of the element's coefficient block
+            // stage x: contract the element's first index
             T A0[M][M], A1[M][M], KA[KF ? M : 1][KF ? M : 1], SA[SRC ? M : 1][SRC ? M : 1];
 #pragma unroll
             for (int b = 0; b < M; ++b)
@@ -331,11 +576,13 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T
 #pragma unroll
                     for (int a = 0; a < M; ++a) {
                         const int li = (lx + a) + BX * ((ly + b) + BY * (lz + c));
-                        const T cv = blk[li];
+                        const T cv = CREG ? cf[CREG ? a : 0][CREG ? b : 0][CREG ? c : 0] : blk[li];
                         s0 += bx[a] * cv;
                         s1 += dx[a] * cv;
-                        if constexpr (KF) sk += bx[a] * kb[li];
-                        if constexpr (SRC) ss += bx[a] * sb[li];
+                        if constexpr (KF)
+                            sk += bx[a] * (CREG ? kc[KF && CREG ? a : 0][KF && CREG ? b : 0][KF && CREG ? c : 0] : kb[li]);
+                        if constexpr (SRC)
+                            ss += bx[a] * (CREG ? sc[SRC && CREG ? a : 0][SRC && CREG ? b : 0][SRC && CREG ? c : 0] : sb[li]);
                     }
                     A0[b][c] = s0;
                     A1[b][c] = s1;
@@ -347,7 +594,7 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T
             for (int b = 0; b < M; ++b)
 #pragma unroll
                 for (int c = 0; c < M; ++c) K0[b][c] = Kx[b][c] = T(0);
-#pragma unroll
+#pragma unroll 1
             for (int qy = 0; qy < M; ++qy) {
                 T by[M], dy[M];
 #pragma unroll
@@ -395,9 +642,9 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T
                     if (prm.reaction != T(0)) g0 += prm.dt * prm.reaction * u * (T(1) - u);
                     if constexpr (SRC) g0 += prm.dt * fv;
                     g0 *= wJ;
-                    T sc = -prm.dt * wJ;
-                    if constexpr (KF) sc *= kv;
-                    const T gx = sc * prm.gx * ux, gy = sc * prm.gy * uy, gz = sc * prm.gz * uz;
+                    T sc2 = -prm.dt * wJ;
+                    if constexpr (KF) sc2 *= kv;
+                    const T gx = sc2 * prm.gx * ux, gy = sc2 * prm.gy * uy, gz = sc2 * prm.gz * uz;
 #pragma unroll
                     for (int c = 0; c < M; ++c) {
                         H0[c] += Bz[qz][c] * g0 + Dz[qz][c] * gz;
@@ -413,19 +660,22 @@ __global__ __launch_bounds__(TX* TY* TZ) void k_rhs3(const T* __restrict__ up, T
                         Kx[b][c] += by[b] * Hx[c];
                     }
             }
-            // stage x transposed: accumulate the element contribution in LDS
+            // stage x transposed
 #pragma unroll
             for (int a = 0; a < M; ++a)
 #pragma unroll
                 for (int b = 0; b < M; ++b)
 #pragma unroll
-                    for (int c = 0; c < M; ++c) {
-                        const int idx = (a + M * (b + M * c)) * NT + tid;
-                        const T r = bx[a] * K0[b][c] + dx[a] * Kx[b][c];
-                        Rs[idx] = (qx == 0) ? r : Rs[idx] + r;
-                    }
+                    for (int c = 0; c < M; ++c) R[a][b][c] += bx[a] * K0[b][c] + dx[a] * Kx[b][c];
+        }
         }
     }
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = 0; b < M; ++b)
+#pragma unroll
+            for (int c = 0; c < M; ++c) Rs[(a + M * (b + M * c)) * NT + tid] = R[a][b][c];
     __syncthreads();
 
     // each DOF of the block sums the contributions of the tile's elements that support it
@@ -603,10 +853,10 @@ struct tile<3, 2> {
 };
 template <>
 struct tile<3, 3> {
-    static constexpr int X = 8, Y = 4, Z = 4;
+    static constexpr int X = 8, Y = 8, Z = 3;
 };
 
-template <typename T, int P, bool KF, bool SRC>
+template <typename T, int P, bool KF, bool SRC, bool CREG>
 hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                         hipStream_t s) {
     using TL = tile<3, P>;
@@ -618,7 +868,7 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
         const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2, nz2 = (ntz - cz + 1) / 2;
         const long blocks = static_cast<long>(nx2) * ny2 * nz2;
         if (blocks == 0) continue;
-        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC>), dim3(blocks),
+        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, CREG>), dim3(blocks),
                            dim3(TL::X * TL::Y * TL::Z), 0, s, up, rhs, d[0], d[1], d[2], prm, c,
                            nx2, ny2);
         hipError_t e = hipGetLastError();
@@ -646,15 +896,23 @@ hipError_t rhs2_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
     return hipSuccess;
 }
 
+template <typename T, int P, bool CREG>
+hipError_t rhs3_fields(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
+                       hipStream_t s) {
+    const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
+    if (kf && src) return rhs3_colours<T, P, true, true, CREG>(up, rhs, d, prm, s);
+    if (kf) return rhs3_colours<T, P, true, false, CREG>(up, rhs, d, prm, s);
+    if (src) return rhs3_colours<T, P, false, true, CREG>(up, rhs, d, prm, s);
+    return rhs3_colours<T, P, false, false, CREG>(up, rhs, d, prm, s);
+}
+
 template <typename T, int P>
 hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
-                      const rhs_params<T>& prm, hipStream_t s) {
+                      const rhs_params<T>& prm, int variant, hipStream_t s) {
     const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
     if (dim == 3) {
-        if (kf && src) return rhs3_colours<T, P, true, true>(up, rhs, d, prm, s);
-        if (kf) return rhs3_colours<T, P, true, false>(up, rhs, d, prm, s);
-        if (src) return rhs3_colours<T, P, false, true>(up, rhs, d, prm, s);
-        return rhs3_colours<T, P, false, false>(up, rhs, d, prm, s);
+        if (variant == 1) return rhs3_fields<T, P, false>(up, rhs, d, prm, s);
+        return rhs3_fields<T, P, true>(up, rhs, d, prm, s);
     }
     if (kf && src) return rhs2_colours<T, P, true, true>(up, rhs, d, prm, s);
     if (kf) return rhs2_colours<T, P, true, false>(up, rhs, d, prm, s);
@@ -784,16 +1042,30 @@ hipError_t launch_line_sweep(T* u, const line_geometry& g, const dev_factors<F>&
     return hipErrorInvalidValue;
 }
 
+template <typename T, typename F>
+hipError_t launch_line_sweep_seg(T* u, const line_geometry& g, const seg_factors<F>& f,
+                                 hipStream_t s) {
+    const int kl = f.f.kl, ku = f.f.ku;
+    if (f.f.pivoted) return hipErrorInvalidValue;
+#define ADSG_SEG(KL_, KU_) \
+    if (kl == KL_ && ku == KU_) return seg_dispatch<T, F, KL_, KU_>(u, g, f, s);
+    ADSG_SEG(1, 1)
+    ADSG_SEG(2, 2)
+    ADSG_SEG(3, 3)
+#undef ADSG_SEG
+    return hipErrorInvalidValue;
+}
+
 template <typename T>
 hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
-                      const rhs_params<T>& prm, hipStream_t s) {
+                      const rhs_params<T>& prm, int variant, hipStream_t s) {
     switch (p) {
         case 1:
-            return rhs_fields<T, 1>(dim, u_prev, rhs, dirs, prm, s);
+            return rhs_fields<T, 1>(dim, u_prev, rhs, dirs, prm, variant, s);
         case 2:
-            return rhs_fields<T, 2>(dim, u_prev, rhs, dirs, prm, s);
+            return rhs_fields<T, 2>(dim, u_prev, rhs, dirs, prm, variant, s);
         case 3:
-            return rhs_fields<T, 3>(dim, u_prev, rhs, dirs, prm, s);
+            return rhs_fields<T, 3>(dim, u_prev, rhs, dirs, prm, variant, s);
         default:
             return hipErrorInvalidValue;
     }
@@ -844,10 +1116,14 @@ template hipError_t launch_line_sweep<double, double>(double*, const line_geomet
                                                       const dev_factors<double>&, hipStream_t);
 template hipError_t launch_line_sweep<float, float>(float*, const line_geometry&,
                                                     const dev_factors<float>&, hipStream_t);
+template hipError_t launch_line_sweep_seg<double, double>(double*, const line_geometry&,
+                                                          const seg_factors<double>&, hipStream_t);
+template hipError_t launch_line_sweep_seg<float, float>(float*, const line_geometry&,
+                                                        const seg_factors<float>&, hipStream_t);
 template hipError_t launch_rhs<double>(int, int, const double*, double*, const dev_tables<double>*,
-                                       const rhs_params<double>&, hipStream_t);
+                                       const rhs_params<double>&, int, hipStream_t);
 template hipError_t launch_rhs<float>(int, int, const float*, float*, const dev_tables<float>*,
-                                      const rhs_params<float>&, hipStream_t);
+                                      const rhs_params<float>&, int, hipStream_t);
 template hipError_t launch_fill<double>(double*, long, double, hipStream_t);
 template hipError_t launch_fill<float>(float*, long, float, hipStream_t);
 template hipError_t launch_zero_boundary<double>(int, double*, int, int, int, hipStream_t);

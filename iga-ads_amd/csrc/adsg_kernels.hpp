@@ -27,6 +27,22 @@ template <typename T, typename F>
 hipError_t launch_line_sweep(T* u, const line_geometry& g, const dev_factors<F>& f,
                              hipStream_t s);
 
+// Segmented sweep of unpivoted factors: every line is cut into nseg segments of length m
+// (the last one shorter) solved concurrently; gf[j*kl + t] is the influence of the t-th
+// value of the previous segment's tail (positions j0-kl .. j0-1) on y_j, hb[j*ku + t] the
+// influence of the t-th value of the next segment's head (positions j1 .. j1+ku-1) on x_j.
+template <typename F>
+struct seg_factors {
+    dev_factors<F> f;
+    const F* gf = nullptr;  // [n*kl]
+    const F* hb = nullptr;  // [n*ku]
+    int m = 0, nseg = 0, mseg = 0;  // mseg: register capacity per lane (16 or 32)
+};
+
+template <typename T, typename F>
+hipError_t launch_line_sweep_seg(T* u, const line_geometry& g, const seg_factors<F>& f,
+                                 hipStream_t s);
+
 // Tables of one direction on the device: val/der [elements][p+1][p+1] (element, Gauss
 // point, local function) and wj [elements][p+1].
 template <typename T>
@@ -47,9 +63,11 @@ struct rhs_params {
     const T* source = nullptr;  // coefficients of f on the same basis, or null (f = 0)
 };
 
+// variant: kernel tuning choice (0 = coefficient block held in registers, 1 = re-read from
+// LDS); both compute the same sums in the same order.
 template <typename T>
 hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
-                      const rhs_params<T>& prm, hipStream_t s);
+                      const rhs_params<T>& prm, int variant, hipStream_t s);
 
 template <typename T>
 hipError_t launch_fill(T* p, long n, T v, hipStream_t s);

@@ -286,4 +286,63 @@ int factor_for_lines(const std::vector<double>& ab_in, int n, int kl, int ku, li
     return info;
 }
 
+bool segment_lines(const line_factors& f, line_segments& out) {
+    const int n = f.n, kl = f.kl, ku = f.ku_eff;
+    const int km = std::max(kl, ku);
+    if (f.pivoted || kl < 1 || ku < 1 || kl != ku) return false;
+    int mseg = 0, nseg = 0, m = 0;
+    for (int cap : {16, 32}) {
+        int s = (n + cap - 1) / cap;
+        if (s < 2 || s > 16) continue;
+        int mm = (n + s - 1) / s;  // balanced segments
+        if (mm > cap || n - (s - 1) * mm < km || mm < km) continue;
+        mseg = cap;
+        nseg = s;
+        m = mm;
+        break;
+    }
+    if (!mseg) return false;
+    out.m = m;
+    out.nseg = nseg;
+    out.mseg = mseg;
+    out.gf.assign(static_cast<size_t>(n) * kl, 0.0);
+    out.hb.assign(static_cast<size_t>(n) * ku, 0.0);
+    auto L = [&](int j, int t) { return f.lower[static_cast<size_t>(j - t) * kl + t - 1]; };  // L(j, j-t)
+    auto U = [&](int j, int t) { return f.upper[static_cast<size_t>(j + t) * ku + t - 1]; };  // U(j, j+t)
+    std::vector<double> y(n + km + 1);
+    for (int s = 0; s < nseg; ++s) {
+        const int j0 = s * m, j1 = std::min(n, j0 + m);
+        if (s > 0) {
+            // forward: unit value at tail position j0-kl+q, zero right-hand side
+            for (int q = 0; q < kl; ++q) {
+                std::fill(y.begin(), y.end(), 0.0);
+                y[j0 - kl + q] = 1.0;
+                for (int j = j0; j < j1; ++j) {
+                    double acc = 0;
+                    for (int t = 1; t <= kl; ++t)
+                        if (j - t >= j0 - kl) acc -= L(j, t) * y[j - t];
+                    y[j] = acc;
+                    out.gf[static_cast<size_t>(j) * kl + q] = acc;
+                }
+            }
+        }
+        if (s < nseg - 1) {
+            // backward: unit value at head position j1+q
+            for (int q = 0; q < ku; ++q) {
+                std::fill(y.begin(), y.end(), 0.0);
+                y[j1 + q] = 1.0;
+                for (int j = j1 - 1; j >= j0; --j) {
+                    double acc = 0;
+                    for (int t = 1; t <= ku; ++t)
+                        if (j + t < std::min(n, j1 + ku)) acc -= U(j, t) * y[j + t];
+                    acc *= f.rdiag[j];
+                    y[j] = acc;
+                    out.hb[static_cast<size_t>(j) * ku + q] = acc;
+                }
+            }
+        }
+    }
+    return true;
+}
+
 }  // namespace adsg

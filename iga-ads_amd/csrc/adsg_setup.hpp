@@ -59,4 +59,15 @@ struct line_factors {
 
 int factor_for_lines(const std::vector<double>& ab_in, int n, int kl, int ku, line_factors& out);
 
+// Segmentation of the lines of unpivoted factors for the segmented sweep: nseg segments of
+// length m (mseg = register capacity 16 or 32, nseg <= 16), and the influence
+// coefficients gf [n*kl] (tail of the previous segment on y_j) and hb [n*ku] (head of the
+// next segment on x_j), computed in FP64 from the factors. Returns false when the line
+// does not fit the scheme (pivoted factors, too short or too long lines).
+struct line_segments {
+    int m = 0, nseg = 0, mseg = 0;
+    std::vector<double> gf, hb;
+};
+bool segment_lines(const line_factors& f, line_segments& out);
+
 }  // namespace adsg

@@ -144,6 +144,20 @@ int adsg_synchronize(adsg_sim* h);
  * ms[2], ms[3], ms[4] line sweeps in x, y, z, ms[5] the whole step. */
 int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags, double* ms);
 
+/* Kernel tuning (no counterpart in the reference, whose CPU loops have no launch shape):
+ * ADSG_TUNE_RHS_VARIANT selects the element-kernel variant (0: element coefficients held in
+ * registers, 1: re-read from LDS); both compute the same sums in the same order.
+ * ADSG_TUNE_SWEEP_VARIANT selects the line sweep (0: lines cut into segments solved by
+ * separate waves and joined through FP64-computed influence coefficients, where the factors
+ * are unpivoted and the lines are 2..16 segments long; 1: one lane per whole line).
+ * adsg_autotune times every variant on the handle's current state with the given form and
+ * keeps the fastest of each; the state vectors are restored afterwards. */
+#define ADSG_TUNE_RHS_VARIANT 0
+#define ADSG_TUNE_SWEEP_VARIANT 1
+int adsg_set_tuning(adsg_sim* h, int key, int value);
+int adsg_get_tuning(const adsg_sim* h, int key);
+int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, int flags);
+
 /* lin::factorize + lin::solve_with_factorized (band_solve.hpp:16-31) on the host in
  * FP64, for callers that hold their own band matrices: factor ab (n x n, kl, ku, full
  * buffer as above) in place, pivots 1-based like dgbtrf. Returns LAPACK info. */

@@ -287,3 +287,58 @@ def test_fp32_step_error_against_fp64_oracle():
         worst = max(worst, err)
         assert err <= 1e-6, (step, err)
     _log(f"FP32 worst per-step rel L2 vs FP64 oracle = {worst:.3e}")
+
+
+# ------------------------------------------------------------------ tuning variants
+@pytest.mark.parametrize("rhs_variant,sweep_variant", [(0, 0), (1, 1), (0, 1), (1, 0)])
+@pytest.mark.parametrize("dim,p,ne", [(3, 2, (20, 17, 19)), (3, 3, (13, 11, 12)), (2, 2, (70, 45)),
+                                      (3, 1, (30, 9, 8)), (2, 3, (129, 33))])
+def test_every_kernel_variant_matches_oracle(dim, p, ne, rhs_variant, sweep_variant):
+    """Each element-kernel and line-sweep variant (what the autotuner chooses between)
+    gives the oracle's step to <= 1e-12, with and without fixed boundary rows."""
+    _log(f"case: variants rhs={rhs_variant} sweep={sweep_variant} dim={dim} p={p} ne={ne}")
+    for dirichlet in (False, True):
+        g, o = _pair(dim, p, ne, dirichlet=dirichlet)
+        g.set_tuning(ag.TUNE_RHS_VARIANT, rhs_variant)
+        g.set_tuning(ag.TUNE_SWEEP_VARIANT, sweep_variant)
+        u = o.init_gaussians(seed=41, count=3, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+        g.set(ag.U, u)
+        flags = ag.STEP_DIRICHLET if dirichlet else 0
+        for _ in range(3):
+            rhs = o.compute_rhs(u, po.heat_form(1e-4, dim), threads=8)
+            if dirichlet:
+                o.zero_boundary(rhs)
+            u = o.solve(rhs)
+        g.step(3, ag.rhs_form(1e-4), flags)
+        assert po.rel_l2(g.get(ag.U), u) <= FP64_STEP_TOL
+
+
+def test_segmented_sweep_on_implicit_matrices():
+    """The segmented sweep on M + c K factors (implicit directions, p=3, 195 DOFs per line
+    as in the 192^3 configuration's lines) against the oracle's dgbtrs restatement."""
+    _log("case: segmented sweep implicit p=3 192x6x5")
+    g, o = _pair(3, 3, (192, 6, 5))
+    for d in range(3):
+        g.set_matrix(d, 1, 1.0, 0.5e-4)
+        o.set_matrix(d, 1, 1.0, 0.5e-4)
+    g.prepare_matrices()
+    o.prepare_matrices()
+    b = np.random.default_rng(8).standard_normal(g.size)
+    for slots in ([1, 0, 0], [0, 1, 0], [1, 1, 1]):
+        g.set(ag.U, b)
+        g.ads_solve(slots)
+        assert po.rel_l2(g.get(ag.U), o.solve(b.copy(), slots=slots)) < 1e-13
+
+
+def test_autotune_keeps_state_and_picks_a_variant():
+    _log("case: autotune 3D p=2 24^3")
+    g, o = _pair(3, 2, (24, 24, 24))
+    u = o.init_gaussians(seed=2, count=2, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+    g.set(ag.U, u)
+    v = g.autotune(ag.rhs_form(1e-4))
+    assert v in (0, 1) and g.get_tuning(ag.TUNE_SWEEP_VARIANT) in (0, 1)
+    assert np.array_equal(g.get(ag.U), u)
+    g.step(2, ag.rhs_form(1e-4))
+    ref = o.solve(o.compute_rhs(u, po.heat_form(1e-4, 3), threads=8))
+    ref = o.solve(o.compute_rhs(ref, po.heat_form(1e-4, 3), threads=8))
+    assert po.rel_l2(g.get(ag.U), ref) <= FP64_STEP_TOL
