@@ -56,6 +56,26 @@ def run_form(ag, po, precision, ne, steps, warmup, u0, dt, variant=None):
     return (t1 - t0), phases, out, variant
 
 
+def copy_calibration(mib, reps):
+    """A plain device-to-device copy of known size through the HIP runtime (its copy
+    kernel reads and writes mib MiB per launch): the reference point for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of the step's kernels."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_mib": mib, "launches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,7 +86,12 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--rhs-variant", type=int, default=None,
                     help="element-kernel variant (default: autotuned)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain D2D copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 4)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the ADS step here runs on a single MI355X "
               "(--gpus 1); multi-GPU slab decomposition is described but not built.",
