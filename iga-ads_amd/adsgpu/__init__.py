@@ -16,6 +16,7 @@ OK, EINVAL, EHIP, ENOMEM, ESTATE, ENONFINITE = 0, -1, -2, -3, -4, -5
 FP64, FP32 = 0, 1
 U, U_PREV, BUFFER, KFIELD, SOURCE = 0, 1, 2, 3, 4
 STEP_DIRICHLET = 1
+STEP_REFINE = 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -59,7 +60,8 @@ EXPORTS = [
     "adsg_create", "adsg_destroy", "adsg_last_error", "adsg_dofs", "adsg_size", "adsg_precision",
     "adsg_fix_dof", "adsg_set_matrix", "adsg_set_band_matrix", "adsg_prepare_matrices",
     "adsg_set_vector", "adsg_get_vector", "adsg_device_ptr", "adsg_swap", "adsg_compute_rhs",
-    "adsg_zero_boundary", "adsg_ads_solve", "adsg_step", "adsg_synchronize",
+    "adsg_zero_boundary", "adsg_ads_solve", "adsg_ads_solve_transposed", "adsg_step",
+    "adsg_step_delta", "adsg_adjoint_step_delta", "adsg_synchronize",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
     "adsg_autotune",
 ]
@@ -95,6 +97,9 @@ def lib():
         L.adsg_zero_boundary.argtypes = [C.c_void_p]
         L.adsg_ads_solve.argtypes = [C.c_void_p, _ip]
         L.adsg_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), C.c_int]
+        L.adsg_ads_solve_transposed.argtypes = [C.c_void_p, _ip]
+        L.adsg_step_delta.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), _ip, C.c_int]
+        L.adsg_adjoint_step_delta.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), _ip, C.c_int]
         L.adsg_synchronize.argtypes = [C.c_void_p]
         L.adsg_band_factorize.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip]
         L.adsg_time_phases.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), C.c_int, _dp]
@@ -191,15 +196,32 @@ class Simulation:
     def zero_boundary(self):
         self._check(lib().adsg_zero_boundary(self.h), "zero_boundary")
 
-    def ads_solve(self, slots=None):
+    @staticmethod
+    def _slots(slots):
         if slots is None:
-            self._check(lib().adsg_ads_solve(self.h, None), "ads_solve")
-        else:
-            s = (C.c_int * 3)(*(list(slots) + [0] * (3 - len(slots))))
-            self._check(lib().adsg_ads_solve(self.h, s), "ads_solve")
+            return None
+        return (C.c_int * 3)(*(list(slots) + [0] * (3 - len(slots))))
+
+    def ads_solve(self, slots=None):
+        self._check(lib().adsg_ads_solve(self.h, self._slots(slots)), "ads_solve")
+
+    def ads_solve_transposed(self, slots=None):
+        """u := (A_x (x) A_y [(x) A_z])^-T u (dgbtrs 'T' per direction)."""
+        self._check(lib().adsg_ads_solve_transposed(self.h, self._slots(slots)),
+                    "ads_solve_transposed")
 
     def step(self, n, form, flags=0):
         self._check(lib().adsg_step(self.h, n, C.byref(form), flags), "step")
+
+    def step_delta(self, n, form, slots, flags=0):
+        """n steps of u := u_prev + A_slots^-1 r(u_prev), r the integrand of form."""
+        self._check(lib().adsg_step_delta(self.h, n, C.byref(form), self._slots(slots), flags),
+                    "step_delta")
+
+    def adjoint_step_delta(self, n, form, slots, flags=0):
+        """n adjoint steps of step_delta applied to u in place (u_prev, buffer clobbered)."""
+        self._check(lib().adsg_adjoint_step_delta(self.h, n, C.byref(form), self._slots(slots),
+                                                  flags), "adjoint_step_delta")
 
     def synchronize(self):
         self._check(lib().adsg_synchronize(self.h), "synchronize")

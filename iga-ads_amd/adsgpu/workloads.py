@@ -1,0 +1,106 @@
+"""Seeded synthetic inputs of the benchmark workloads, as B-spline coefficient vectors.
+
+Coefficients are the field sampled at the Greville abscissae of each direction's clamped
+uniform knot vector (bspline.cpp:26-43 builds the knots; lerp as util.hpp:17-26), first
+index fastest. Random draws come from splitmix64 (Steele, Lea, Flood 2014), mapped to
+[0, 1) as (z >> 11) * 2^-53; the k-th draw of a stream seeded with s is mix(s + k*gamma),
+so the noise draws are vectorised.
+"""
+import numpy as np
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z):
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+class Stream:
+    """splitmix64 stream: uniform() draws in [0, 1)."""
+
+    def __init__(self, seed):
+        self.k = 0
+        self.seed = np.uint64(seed)
+
+    def uniform(self, n=None):
+        count = 1 if n is None else int(n)
+        with np.errstate(over="ignore"):
+            ks = np.arange(self.k + 1, self.k + 1 + count, dtype=np.uint64)
+            z = _mix(self.seed + ks * _GAMMA)
+        self.k += count
+        u = (z >> np.uint64(11)).astype(np.float64) * 2.0**-53
+        return float(u[0]) if n is None else u
+
+
+def knots(p, elements, a=0.0, b=1.0):
+    """Clamped uniform knot vector: p+1 copies of a, the interior points, p+1 copies of b."""
+    t = np.arange(1, elements, dtype=np.float64) / float(elements)
+    interior = (1.0 - t) * a + t * b
+    return np.concatenate([np.full(p + 1, a), interior, np.full(p + 1, b)])
+
+
+def greville(p, elements, a=0.0, b=1.0):
+    """Greville abscissae (averages of p consecutive interior knots), one per dof."""
+    kn = knots(p, elements, a, b)
+    n = len(kn) - p - 1
+    if p == 0:
+        return kn[:n].copy()
+    out = np.zeros(n)
+    for i in range(n):
+        s = 0.0
+        for k in range(1, p + 1):
+            s += kn[i + k]
+        out[i] = s / p
+    return out
+
+
+def gaussians(p, elements, seed, count, sigma=(0.1, 0.1), centre=(0.3, 0.7), noise=1e-3):
+    """Sum of `count` Gaussians exp(-|x-c|^2 / (2 sigma^2)) plus uniform noise in
+    [-noise, noise]. Draw order: per Gaussian its centre (x, y[, z]) then sigma; then one
+    noise draw per coefficient in memory order."""
+    dim = len(p)
+    st = Stream(seed)
+    cs, sg = [], []
+    for _ in range(count):
+        c = [centre[0] + (centre[1] - centre[0]) * st.uniform() for _ in range(dim)]
+        cs.append(c)
+        sg.append(sigma[0] + (sigma[1] - sigma[0]) * st.uniform())
+    g = [greville(p[d], elements[d]) for d in range(dim)]
+    shape = tuple(len(x) for x in g)
+    u = np.zeros(shape[::-1])  # (z,) y, x: first index fastest in memory
+    for c, s in zip(cs, sg):
+        r2 = 0.0
+        for d in range(dim):
+            axis = [1] * dim
+            axis[dim - 1 - d] = shape[d]
+            dx = (g[d] - c[d]).reshape(axis)
+            r2 = r2 + dx * dx
+        u += np.exp(-r2 / (2 * s * s))
+    u = u.reshape(-1)
+    u += noise * (2 * st.uniform(u.size) - 1)
+    return u
+
+
+def sine_source(p, elements, seed, terms=4):
+    """Smooth separable source: sum over `terms` of a * prod_d sin(k_d pi x_d), with
+    amplitude a uniform in [-1, 1] and wave numbers k_d uniform in {1, 2, 3}. Draw order:
+    per term a, then k_x, k_y[, k_z]."""
+    dim = len(p)
+    st = Stream(seed)
+    g = [greville(p[d], elements[d]) for d in range(dim)]
+    shape = tuple(len(x) for x in g)
+    f = np.zeros(shape[::-1])
+    for _ in range(terms):
+        a = 2 * st.uniform() - 1
+        prod = a
+        for d in range(dim):
+            k = 1 + min(int(3 * st.uniform()), 2)
+            axis = [1] * dim
+            axis[dim - 1 - d] = shape[d]
+            prod = prod * np.sin(k * np.pi * g[d]).reshape(axis)
+        f += prod
+    return f.reshape(-1)

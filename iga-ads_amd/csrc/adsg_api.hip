@@ -32,6 +32,9 @@ struct dev_slot {
     adsg::line_segments seg;
     void* gf = nullptr;
     void* hb = nullptr;
+    void* gft = nullptr;  // transposed solve
+    void* hbt = nullptr;
+    double* arow = nullptr;  // FP64 rows of the unfactored matrix, for refinement
 };
 
 struct dev_dir {
@@ -122,12 +125,14 @@ int upload_tables(adsg_sim* h, dev_dir& d) {
 }
 
 void free_slot(dev_slot& s) {
-    for (void** p : {&s.lower, &s.upper, &s.rdiag, &s.gf, &s.hb}) {
+    for (void** p : {&s.lower, &s.upper, &s.rdiag, &s.gf, &s.hb, &s.gft, &s.hbt}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
     if (s.poff) (void)hipFree(s.poff);
     s.poff = nullptr;
+    if (s.arow) (void)hipFree(s.arow);
+    s.arow = nullptr;
 }
 
 void drop_graphs(adsg_sim* h) {
@@ -158,8 +163,9 @@ adsg::line_geometry geometry(const adsg_sim* h, int dir) {
     return g;
 }
 
+// mode: 0 = A^-1, 1 = A^-T, 2 = A^-1 with one FP64-residual refinement (FP32 data)
 template <typename T>
-int sweep_dir(adsg_sim* h, T* u, int d, int sl) {
+int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0) {
     if (sl < 0 || sl >= static_cast<int>(h->dirs[d].slots.size()) || !h->dirs[d].slots[sl].factored)
         return fail(h, ADSG_ESTATE, "ads_solve: matrix slot not set or not factored");
     const dev_slot& s = h->dirs[d].slots[sl];
@@ -172,19 +178,30 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl) {
     f.kl = s.lf.kl;
     f.ku = s.lf.ku_eff;
     f.pivoted = s.lf.pivoted;
-    hipError_t e;
-    if (h->sweep_variant == 0 && s.seg.nseg > 0) {
+    const bool seg_ok = s.seg.nseg > 0;
+    if (mode != 0 && !seg_ok)
+        return fail(h, ADSG_EINVAL,
+                    "ads_solve: transposed and refined solves need unpivoted factors and lines of "
+                    "2..16 segments");
+    hipError_t e = hipErrorInvalidValue;
+    if (seg_ok && (mode != 0 || h->sweep_variant == 0)) {
         adsg::seg_factors<T> sf;
         sf.f = f;
         sf.gf = static_cast<const T*>(s.gf);
         sf.hb = static_cast<const T*>(s.hb);
+        sf.gft = static_cast<const T*>(s.gft);
+        sf.hbt = static_cast<const T*>(s.hbt);
+        sf.arow = s.arow;
         sf.m = s.seg.m;
         sf.nseg = s.seg.nseg;
         sf.mseg = s.seg.mseg;
+        sf.transposed = mode == 1;
+        sf.refine = mode == 2;
         e = adsg::launch_line_sweep_seg<T, T>(u, geometry(h, d), sf, h->stream);
-    } else {
-        e = adsg::launch_line_sweep<T, T>(u, geometry(h, d), f, h->stream);
+        if (e == hipErrorInvalidValue && mode != 0)
+            return fail(h, ADSG_EINVAL, "ads_solve: line staging does not fit the LDS for this solve");
     }
+    if (e == hipErrorInvalidValue && mode == 0) e = adsg::launch_line_sweep<T, T>(u, geometry(h, d), f, h->stream);
     if (e == hipErrorInvalidValue) return fail(h, ADSG_EINVAL, "ads_solve: unsupported band widths (kl, ku)");
     if (e != hipSuccess) return hip_fail(h, e, "line sweep");
     return ADSG_OK;
@@ -192,16 +209,18 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl) {
 
 // ads_solve: one sweep per direction in the order x, y, z (solver.hpp:35-41)
 template <typename T>
-int sweep_all(adsg_sim* h, T* u, const int* slots) {
+int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0) {
     for (int d = 0; d < h->dim; ++d) {
-        int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0);
+        int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode);
         if (rc) return rc;
     }
     return ADSG_OK;
 }
 
+// delta: the integrand exactly as given, from u_prev (its FP32 shadow in FP32 mode) into
+// buffer; otherwise compute_rhs semantics (FP64: into u; FP32: rhs - M u_prev into buffer)
 template <typename T>
-int rhs_impl(adsg_sim* h, const adsg_rhs_form* form) {
+int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
     adsg::dev_tables<T> t[3];
     for (int d = 0; d < h->dim; ++d) {
         t[d].val = static_cast<const T*>(h->dirs[d].val);
@@ -229,11 +248,11 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form) {
     const T* up;
     if constexpr (sizeof(T) == sizeof(float)) {
         // increment form: rhs - M u_prev, i.e. the integrand without its mass term
-        prm.mass = static_cast<T>(form->mass - 1.0);
+        if (!delta) prm.mass = static_cast<T>(form->mass - 1.0);
         rhs = static_cast<T*>(h->vec[ADSG_BUFFER]);
         up = static_cast<const T*>(h->shadow[1]);
     } else {
-        rhs = static_cast<T*>(h->vec[ADSG_U]);
+        rhs = static_cast<T*>(h->vec[delta ? ADSG_BUFFER : ADSG_U]);
         up = static_cast<const T*>(h->vec[ADSG_U_PREV]);
     }
     hipError_t e = adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
@@ -241,7 +260,7 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form) {
     e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->rhs_variant, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     if constexpr (sizeof(T) == sizeof(float)) {
-        h->pending = true;
+        h->pending = !delta;
         h->pending_dirichlet = false;
     }
     return ADSG_OK;
@@ -285,7 +304,8 @@ int solve_impl(adsg_sim* h, const int* slots) {
         for (int d = 0; d < h->dim; ++d)
             if (slots[d] != 0)
                 return fail(h, ADSG_EINVAL,
-                            "ads_solve (FP32): increments are defined against the Gram matrices (slot 0)");
+                            "ads_solve (FP32): increments are defined against the Gram matrices (slot 0); "
+                            "use adsg_step_delta for other matrices");
     int rc = sweep_all<float>(h, static_cast<float*>(h->vec[ADSG_BUFFER]), nullptr);
     if (rc) return rc;
     int nx, ny, nz;
@@ -321,6 +341,84 @@ int one_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
     return solve_impl(h, nullptr);
 }
 
+// one delta step: u = u_prev + A^-1 r(u_prev)
+int one_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags) {
+    swap_state(h);
+    int nx, ny, nz;
+    extents(h, nx, ny, nz);
+    const bool dir = flags & ADSG_STEP_DIRICHLET;
+    int rc;
+    hipError_t e;
+    if (h->precision == ADSG_FP32) {
+        if ((rc = rhs_impl<float>(h, form, true))) return rc;
+        float* r = static_cast<float*>(h->vec[ADSG_BUFFER]);
+        if (dir && (e = adsg::launch_zero_boundary<float>(h->dim, r, nx, ny, nz, h->stream)) != hipSuccess)
+            return hip_fail(h, e, "zero boundary");
+        if ((rc = sweep_all<float>(h, r, slots, (flags & ADSG_STEP_REFINE) ? 2 : 0))) return rc;
+        e = adsg::launch_apply_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
+                                         static_cast<double*>(h->vec[ADSG_U]),
+                                         static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz, dir, h->stream);
+    } else {
+        if ((rc = rhs_impl<double>(h, form, true))) return rc;
+        double* r = static_cast<double*>(h->vec[ADSG_BUFFER]);
+        if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, r, nx, ny, nz, h->stream)) != hipSuccess)
+            return hip_fail(h, e, "zero boundary");
+        if ((rc = sweep_all<double>(h, r, slots))) return rc;
+        e = adsg::launch_add_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
+                                       static_cast<double*>(h->vec[ADSG_U]), h->dim, nx, ny, nz, dir,
+                                       h->stream);
+    }
+    return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "apply increment");
+}
+
+// One step of the adjoint of one_delta_step (FP64) for a linear form r(u) = R u + c:
+//   forward  u' = Z (u + A^-1 Z r(u))           (Z: zero the boundary rows when dirichlet)
+//   adjoint  lam := mu + R^T Z A^-T mu, mu = Z lam
+// R is symmetric (mass*M - dt*K, with K the k-weighted stiffness), so R^T v is the integrand
+// of form without its source term, assembled from v. u_prev is used as scratch.
+int one_adjoint_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags) {
+    int nx, ny, nz;
+    extents(h, nx, ny, nz);
+    const bool dir = flags & ADSG_STEP_DIRICHLET;
+    double* lam = static_cast<double*>(h->vec[ADSG_U]);
+    double* v = static_cast<double*>(h->vec[ADSG_BUFFER]);
+    double* w = static_cast<double*>(h->vec[ADSG_U_PREV]);
+    hipError_t e;
+    if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, lam, nx, ny, nz, h->stream)) != hipSuccess)
+        return hip_fail(h, e, "zero boundary");
+    HIP_TRY(h, hipMemcpyAsync(v, lam, sizeof(double) * h->N, hipMemcpyDeviceToDevice, h->stream));
+    int rc = sweep_all<double>(h, v, slots, 1);
+    if (rc) return rc;
+    if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, v, nx, ny, nz, h->stream)) != hipSuccess)
+        return hip_fail(h, e, "zero boundary");
+    adsg_rhs_form lin = *form;
+    lin.use_source = 0;
+    adsg::dev_tables<double> t[3];
+    for (int d = 0; d < h->dim; ++d) {
+        t[d].val = static_cast<const double*>(h->dirs[d].val);
+        t[d].der = static_cast<const double*>(h->dirs[d].der);
+        t[d].wj = static_cast<const double*>(h->dirs[d].wj);
+        t[d].elements = h->dirs[d].tab.elements;
+        t[d].dofs = h->dirs[d].tab.dofs;
+    }
+    adsg::rhs_params<double> prm;
+    prm.mass = lin.mass;
+    prm.dt = lin.dt;
+    prm.gx = lin.grad[0];
+    prm.gy = lin.grad[1];
+    prm.gz = lin.grad[2];
+    prm.reaction = 0.0;
+    if (lin.use_kfield) {
+        if (!h->vec[ADSG_KFIELD]) return fail(h, ADSG_ESTATE, "adjoint: k field not set");
+        prm.kfield = static_cast<const double*>(h->vec[ADSG_KFIELD]);
+    }
+    if ((e = adsg::launch_fill<double>(w, h->N, 0.0, h->stream)) != hipSuccess) return hip_fail(h, e, "zero");
+    e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, w, t, prm, h->rhs_variant, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
+    e = adsg::launch_add_increment(lam, w, lam, h->dim, nx, ny, nz, false, h->stream);
+    return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "adjoint update");
+}
+
 std::string form_key(const adsg_rhs_form* f) {
     char buf[256];
     std::snprintf(buf, sizeof buf, "%a,%a,%a,%a,%a,%a,%d,%d", f->mass, f->dt, f->grad[0], f->grad[1],
@@ -328,8 +426,25 @@ std::string form_key(const adsg_rhs_form* f) {
     return buf;
 }
 
-int get_graph(adsg_sim* h, const adsg_rhs_form* form, int flags, int steps, hipGraphExec_t* out) {
-    auto key = std::make_tuple(form_key(form), flags, h->vec[ADSG_U], steps);
+enum step_kind { EXPLICIT_STEP, DELTA_STEP, ADJOINT_DELTA_STEP };
+
+int step_of(adsg_sim* h, step_kind kind, const adsg_rhs_form* form, const int* slots, int flags) {
+    switch (kind) {
+        case DELTA_STEP:
+            return one_delta_step(h, form, slots, flags);
+        case ADJOINT_DELTA_STEP:
+            return one_adjoint_delta_step(h, form, slots, flags);
+        default:
+            return one_step(h, form, flags);
+    }
+}
+
+int get_graph(adsg_sim* h, step_kind kind, const adsg_rhs_form* form, int flags, int steps,
+              const int* slots, hipGraphExec_t* out) {
+    std::string k = form_key(form) + "|" + std::to_string(static_cast<int>(kind));
+    if (slots) k += " " + std::to_string(slots[0]) + " " + std::to_string(slots[1]) + " " +
+                    std::to_string(h->dim == 3 ? slots[2] : 0);
+    auto key = std::make_tuple(k, flags, h->vec[ADSG_U], steps);
     auto it = h->graphs.find(key);
     if (it != h->graphs.end()) {
         *out = it->second;
@@ -341,7 +456,8 @@ int get_graph(adsg_sim* h, const adsg_rhs_form* form, int flags, int steps, hipG
     void* s1 = h->shadow[1];
     HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = ADSG_OK;
-    for (int s = 0; s < steps && rc == ADSG_OK; ++s) rc = one_step(h, form, flags);
+    for (int s = 0; s < steps && rc == ADSG_OK; ++s)
+        rc = step_of(h, kind, form, slots, flags);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     h->vec[ADSG_U] = u0;  // capture only recorded the swaps; restore the logical state
@@ -360,6 +476,24 @@ int get_graph(adsg_sim* h, const adsg_rhs_form* form, int flags, int steps, hipG
     if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
     h->graphs[key] = ex;
     *out = ex;
+    return ADSG_OK;
+}
+
+// explicit and delta steps swap u and u_prev once per step, so a two-step graph returns
+// the pointers to where they were; the adjoint step updates u in place
+int run_graphs(adsg_sim* h, step_kind kind, int nsteps, const adsg_rhs_form* form, int flags,
+               const int* slots) {
+    hipGraphExec_t g2 = nullptr, g1 = nullptr;
+    int rc;
+    if (nsteps >= 2) {
+        if ((rc = get_graph(h, kind, form, flags, 2, slots, &g2))) return rc;
+        for (int i = 0; i < nsteps / 2; ++i) HIP_TRY(h, hipGraphLaunch(g2, h->stream));
+    }
+    if (nsteps % 2) {
+        if ((rc = get_graph(h, kind, form, flags, 1, slots, &g1))) return rc;
+        HIP_TRY(h, hipGraphLaunch(g1, h->stream));
+        if (kind != ADJOINT_DELTA_STEP) swap_state(h);
+    }
     return ADSG_OK;
 }
 
@@ -548,10 +682,25 @@ int adsg_prepare_matrices(adsg_sim* h) {
                 if (h->precision == ADSG_FP32) {
                     if ((rc = upload<float>(h, &s.gf, s.seg.gf))) return rc;
                     if ((rc = upload<float>(h, &s.hb, s.seg.hb))) return rc;
+                    if ((rc = upload<float>(h, &s.gft, s.seg.gft))) return rc;
+                    if ((rc = upload<float>(h, &s.hbt, s.seg.hbt))) return rc;
                 } else {
                     if ((rc = upload<double>(h, &s.gf, s.seg.gf))) return rc;
                     if ((rc = upload<double>(h, &s.hb, s.seg.hb))) return rc;
+                    if ((rc = upload<double>(h, &s.gft, s.seg.gft))) return rc;
+                    if ((rc = upload<double>(h, &s.hbt, s.seg.hbt))) return rc;
                 }
+                // rows of the unfactored matrix: arow[j*w + t + kl] = A(j, j+t), t in [-kl, ku]
+                const int n = s.lf.n, w = s.kl + s.ku + 1, ld = 2 * s.kl + s.ku + 1;
+                std::vector<double> rows(static_cast<size_t>(n) * w, 0.0);
+                for (int j = 0; j < n; ++j)
+                    for (int t = -s.kl; t <= s.ku; ++t)
+                        if (j + t >= 0 && j + t < n)
+                            rows[static_cast<size_t>(j) * w + t + s.kl] =
+                                s.ab[static_cast<size_t>(j + t) * ld + s.kl + s.ku - t];
+                void* ar = nullptr;
+                if ((rc = upload<double>(h, &ar, rows))) return rc;
+                s.arow = static_cast<double*>(ar);
             } else {
                 s.seg = adsg::line_segments{};
             }
@@ -649,18 +798,34 @@ int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags) {
     if (!h || !form || nsteps < 0) return fail(h, ADSG_EINVAL, "step: bad argument");
     if (!h->prepared) return fail(h, ADSG_ESTATE, "step: prepare_matrices first");
     if (nsteps == 0) return ADSG_OK;
-    hipGraphExec_t g2 = nullptr, g1 = nullptr;
-    int rc;
-    if (nsteps >= 2) {
-        if ((rc = get_graph(h, form, flags, 2, &g2))) return rc;
-        for (int i = 0; i < nsteps / 2; ++i) HIP_TRY(h, hipGraphLaunch(g2, h->stream));
-    }
-    if (nsteps % 2) {
-        if ((rc = get_graph(h, form, flags, 1, &g1))) return rc;
-        HIP_TRY(h, hipGraphLaunch(g1, h->stream));
-        swap_state(h);
-    }
-    return ADSG_OK;
+    return run_graphs(h, EXPLICIT_STEP, nsteps, form, flags, nullptr);
+}
+
+int adsg_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots, int flags) {
+    if (!h || !form || !slots || nsteps < 0) return fail(h, ADSG_EINVAL, "step_delta: bad argument");
+    if (!h->prepared) return fail(h, ADSG_ESTATE, "step_delta: prepare_matrices first");
+    if ((flags & ADSG_STEP_REFINE) && h->precision != ADSG_FP32)
+        return fail(h, ADSG_EINVAL, "step_delta: refinement applies to FP32 handles");
+    if (nsteps == 0) return ADSG_OK;
+    return run_graphs(h, DELTA_STEP, nsteps, form, flags, slots);
+}
+
+int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots,
+                            int flags) {
+    if (!h || !form || !slots || nsteps < 0) return fail(h, ADSG_EINVAL, "adjoint_step_delta: bad argument");
+    if (!h->prepared) return fail(h, ADSG_ESTATE, "adjoint_step_delta: prepare_matrices first");
+    if (h->precision != ADSG_FP64) return fail(h, ADSG_EINVAL, "adjoint_step_delta: FP64 handles only");
+    if (form->reaction != 0.0)
+        return fail(h, ADSG_EINVAL, "adjoint_step_delta: the form must be linear in u (reaction = 0)");
+    if (!h->vec[ADSG_BUFFER] || !h->vec[ADSG_U_PREV]) return fail(h, ADSG_ESTATE, "adjoint_step_delta: vectors not allocated");
+    if (nsteps == 0) return ADSG_OK;
+    return run_graphs(h, ADJOINT_DELTA_STEP, nsteps, form, flags, slots);
+}
+
+int adsg_ads_solve_transposed(adsg_sim* h, const int* slots) {
+    if (!h) return ADSG_EINVAL;
+    if (h->precision != ADSG_FP64) return fail(h, ADSG_EINVAL, "ads_solve_transposed: FP64 handles only");
+    return sweep_all<double>(h, static_cast<double*>(h->vec[ADSG_U]), slots, 1);
 }
 
 int adsg_synchronize(adsg_sim* h) {

@@ -31,12 +31,19 @@ hipError_t launch_line_sweep(T* u, const line_geometry& g, const dev_factors<F>&
 // (the last one shorter) solved concurrently; gf[j*kl + t] is the influence of the t-th
 // value of the previous segment's tail (positions j0-kl .. j0-1) on y_j, hb[j*ku + t] the
 // influence of the t-th value of the next segment's head (positions j1 .. j1+ku-1) on x_j.
+// For the transposed solve (A^T = U^T L^T) gft [n*ku] and hbt [n*kl] play the same roles.
+// refine: one refinement of every line solve against the FP64 residual w - A z, with arow
+// the unfactored matrix as rows of kl+ku+1 entries (columns j-kl .. j+ku).
 template <typename F>
 struct seg_factors {
     dev_factors<F> f;
-    const F* gf = nullptr;  // [n*kl]
-    const F* hb = nullptr;  // [n*ku]
+    const F* gf = nullptr;   // [n*kl]
+    const F* hb = nullptr;   // [n*ku]
+    const F* gft = nullptr;  // [n*ku]
+    const F* hbt = nullptr;  // [n*kl]
+    const double* arow = nullptr;  // [n*(kl+ku+1)]
     int m = 0, nseg = 0, mseg = 0;  // mseg: register capacity per lane (16 or 32)
+    bool transposed = false, refine = false;
 };
 
 template <typename T, typename F>
@@ -83,6 +90,9 @@ hipError_t launch_convert_f2d(const float* in, double* out, long n, hipStream_t 
 hipError_t launch_apply_increment(const double* x_prev, const float* delta, double* x_new,
                                   float* shadow, int dim, int nx, int ny, int nz, bool dirichlet,
                                   hipStream_t s);
+// FP64 form: x_new = x_prev + delta, boundary coefficients set to 0 with dirichlet.
+hipError_t launch_add_increment(const double* x_prev, const double* delta, double* x_new, int dim,
+                                int nx, int ny, int nz, bool dirichlet, hipStream_t s);
 // Residual of the fixed boundary rows: delta_b = 0 - (M_f u)_b, where M_f is the Kronecker
 // product of the 1D Gram matrices with their boundary rows fixed (dimension.cpp:23-29).
 // rows_d holds the fixed 1D matrix of direction d as n_d rows of 2p+1 entries (columns

@@ -1,0 +1,549 @@
+// HIP element-loop RHS of the MI355X ADS step (gfx950): compute_rhs (validation.hpp:95-119)
+// with one workgroup per tile of elements, one lane per element, sum factorisation through
+// the 1D tables, local contributions combined in LDS, and tiles of one parity colour
+// (2^d colours) adding their DOF blocks into the global rhs without atomics.
+#include <hip/hip_runtime.h>
+
+#include "adsg_kernels.hpp"
+
+namespace adsg {
+
+namespace {
+
+// ------------------------------------------------------------------------------------
+// element-loop RHS, 3D
+// ------------------------------------------------------------------------------------
+// CREG: hold the element's coefficient block in registers (fewer LDS reads) or re-read it
+// from LDS for every x Gauss point (fewer registers: two waves per SIMD).
+//
+// One lane per element; the x/y tables of the tile's elements are staged in LDS, the
+// forward ones as tabulated and the backward ones with the quadrature weight (and, for the
+// derivative tables, the constant -dt*grad_d) folded in, so that the only per-point work
+// is the integrand itself. The element contributions are summed per wave into private DOF
+// planes with LDS atomic adds: a wave's adds are executed in instruction order, so the sums
+// are deterministic, and waves never share a plane. One barrier later the planes of the
+// (P+1) waves that touch a DOF plane are added and the tile's block is added into the rhs.
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool CREG>
+__global__ __launch_bounds__(TX* TY* TZ, CREG ? 1 : 2) void k_rhs3(
+    const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
+    dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
+    constexpr int M = P + 1, NT = TX * TY * TZ;
+    constexpr int BX = TX + P, BY = TY + P, BZ = TZ + P, NB = BX * BY * BZ, PL = BX * BY;
+    static_assert(TX * TY == 64, "a wave must cover exactly one z-layer of the tile");
+    __shared__ T blk[NB];
+    __shared__ T kb[KF ? NB : 1];
+    __shared__ T sb[SRC ? NB : 1];
+    __shared__ T fvx[TX * M * M], fdx[TX * M * M], bvx[TX * M * M], bdx[TX * M * M];
+    __shared__ T fvy[TY * M * M], fdy[TY * M * M], bvy[TY * M * M], bdy[TY * M * M];
+    // wave lz owns planes lz*M .. lz*M+P (DOF z = lz + c); FP64 in both precisions (the FP64
+    // LDS add is the fast one on this part)
+    __shared__ double acc[TZ * M * PL];
+
+    const int tid = threadIdx.x;
+    int t = blockIdx.x;
+    const int tx = 2 * (t % ntx2) + (colour & 1);
+    t /= ntx2;
+    const int ty = 2 * (t % nty2) + ((colour >> 1) & 1);
+    t /= nty2;
+    const int tz = 2 * t + ((colour >> 2) & 1);
+    const int ex0 = tx * TX, ey0 = ty * TY, ez0 = tz * TZ;
+    const int nx = X.dofs, ny = Y.dofs, nz = Z.dofs;
+
+    // the tile's DOF block (first(e) = e for these knot vectors, so dof origin = element origin)
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, r = i / BX, by = r % BY, bz = r / BY;
+        const int gx = ex0 + bx, gy = ey0 + by, gz = ez0 + bz;
+        const bool in = gx < nx && gy < ny && gz < nz;
+        const long g = gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz);
+        blk[i] = in ? up[g] : T(0);
+        if constexpr (KF) kb[i] = in ? prm.kfield[g] : T(0);
+        if constexpr (SRC) sb[i] = in ? prm.source[g] : T(0);
+    }
+    const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy;
+    for (int i = tid; i < TX * M * M; i += NT) {
+        const int e = ex0 + i / (M * M), q = (i / M) % M;
+        const bool in = e < X.elements;
+        const T v = in ? X.val[static_cast<long>(ex0) * M * M + i] : T(0);
+        const T d = in ? X.der[static_cast<long>(ex0) * M * M + i] : T(0);
+        const T w = in ? X.wj[static_cast<long>(e) * M + q] : T(0);
+        fvx[i] = v;
+        fdx[i] = d;
+        bvx[i] = v * w;
+        bdx[i] = cx * (d * w);
+    }
+    for (int i = tid; i < TY * M * M; i += NT) {
+        const int e = ey0 + i / (M * M), q = (i / M) % M;
+        const bool in = e < Y.elements;
+        const T v = in ? Y.val[static_cast<long>(ey0) * M * M + i] : T(0);
+        const T d = in ? Y.der[static_cast<long>(ey0) * M * M + i] : T(0);
+        const T w = in ? Y.wj[static_cast<long>(e) * M + q] : T(0);
+        fvy[i] = v;
+        fdy[i] = d;
+        bvy[i] = v * w;
+        bdy[i] = cy * (d * w);
+    }
+    for (int i = tid; i < TZ * M * PL; i += NT) acc[i] = 0.0;
+    __syncthreads();
+
+    const int lx = tid % TX, ly = (tid / TX) % TY;
+    const int lz = __builtin_amdgcn_readfirstlane(tid / (TX * TY));
+    const int ez = ez0 + lz;
+
+    T R[M][M][M];  // the element's contribution, (a, b, c) = local (x, y, z) function
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = 0; b < M; ++b)
+#pragma unroll
+            for (int c = 0; c < M; ++c) R[a][b][c] = T(0);
+
+    if (ez < Z.elements) {  // wave-uniform
+        // z tables of the wave's element layer: uniform addresses in uniform control flow,
+        // read with scalar loads
+        T Bz[M][M], Dz[M][M], wz[M];
+        const T* zv = Z.val + static_cast<long>(ez) * M * M;
+        const T* zd = Z.der + static_cast<long>(ez) * M * M;
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            wz[q] = Z.wj[static_cast<long>(ez) * M + q];
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                Bz[q][c] = zv[q * M + c];
+                Dz[q][c] = zd[q * M + c];
+            }
+        }
+        const T mass = prm.mass, cz = -prm.dt * prm.gz;
+        if (ex0 + lx < X.elements && ey0 + ly < Y.elements) {
+            T cf[CREG ? M : 1][CREG ? M : 1][CREG ? M : 1];
+            T kc[KF && CREG ? M : 1][KF && CREG ? M : 1][KF && CREG ? M : 1];
+            T sc[SRC && CREG ? M : 1][SRC && CREG ? M : 1][SRC && CREG ? M : 1];
+            if constexpr (CREG) {
+#pragma unroll
+                for (int a = 0; a < M; ++a)
+#pragma unroll
+                    for (int b = 0; b < M; ++b)
+#pragma unroll
+                        for (int c = 0; c < M; ++c) {
+                            const int li = (lx + a) + BX * ((ly + b) + BY * (lz + c));
+                            cf[a][b][c] = blk[li];
+                            if constexpr (KF) kc[a][b][c] = kb[li];
+                            if constexpr (SRC) sc[a][b][c] = sb[li];
+                        }
+            }
+#pragma unroll 1
+            for (int qx = 0; qx < M; ++qx) {
+                T bx[M], dx[M];
+#pragma unroll
+                for (int a = 0; a < M; ++a) {
+                    bx[a] = fvx[(lx * M + qx) * M + a];
+                    dx[a] = fdx[(lx * M + qx) * M + a];
+                }
+                // stage x: contract the element's first index
+                T A0[M][M], A1[M][M], KA[KF ? M : 1][KF ? M : 1], SA[SRC ? M : 1][SRC ? M : 1];
+#pragma unroll
+                for (int b = 0; b < M; ++b)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        T s0 = 0, s1 = 0, sk = 0, ss = 0;
+#pragma unroll
+                        for (int a = 0; a < M; ++a) {
+                            const int li = (lx + a) + BX * ((ly + b) + BY * (lz + c));
+                            const T cv = CREG ? cf[CREG ? a : 0][CREG ? b : 0][CREG ? c : 0] : blk[li];
+                            s0 += bx[a] * cv;
+                            s1 += dx[a] * cv;
+                            if constexpr (KF)
+                                sk += bx[a] * (CREG ? kc[KF && CREG ? a : 0][KF && CREG ? b : 0][KF && CREG ? c : 0] : kb[li]);
+                            if constexpr (SRC)
+                                ss += bx[a] * (CREG ? sc[SRC && CREG ? a : 0][SRC && CREG ? b : 0][SRC && CREG ? c : 0] : sb[li]);
+                        }
+                        A0[b][c] = s0;
+                        A1[b][c] = s1;
+                        if constexpr (KF) KA[b][c] = sk;
+                        if constexpr (SRC) SA[b][c] = ss;
+                    }
+                T K0[M][M], Kx[M][M];
+#pragma unroll
+                for (int b = 0; b < M; ++b)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) K0[b][c] = Kx[b][c] = T(0);
+#pragma unroll 1
+                for (int qy = 0; qy < M; ++qy) {
+                    T by[M], dy[M];
+#pragma unroll
+                    for (int b = 0; b < M; ++b) {
+                        by[b] = fvy[(ly * M + qy) * M + b];
+                        dy[b] = fdy[(ly * M + qy) * M + b];
+                    }
+                    // stage y for this Gauss point
+                    T a00[M], a01[M], a10[M], ka[KF ? M : 1], sa[SRC ? M : 1];
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        T s00 = 0, s01 = 0, s10 = 0, sk = 0, ss = 0;
+#pragma unroll
+                        for (int b = 0; b < M; ++b) {
+                            s00 += by[b] * A0[b][c];
+                            s01 += dy[b] * A0[b][c];
+                            s10 += by[b] * A1[b][c];
+                            if constexpr (KF) sk += by[b] * KA[b][c];
+                            if constexpr (SRC) ss += by[b] * SA[b][c];
+                        }
+                        a00[c] = s00;
+                        a01[c] = s01;
+                        a10[c] = s10;
+                        if constexpr (KF) ka[c] = sk;
+                        if constexpr (SRC) sa[c] = ss;
+                    }
+                    T H0[M], Hx[M], Hy[M];
+#pragma unroll
+                    for (int c = 0; c < M; ++c) H0[c] = Hx[c] = Hy[c] = T(0);
+#pragma unroll
+                    for (int qz = 0; qz < M; ++qz) {
+                        // stage z, the integrand, and stage z transposed
+                        T u = 0, uz = 0, uy = 0, ux = 0, kv = 0, fv = 0;
+#pragma unroll
+                        for (int c = 0; c < M; ++c) {
+                            u += Bz[qz][c] * a00[c];
+                            uz += Dz[qz][c] * a00[c];
+                            uy += Bz[qz][c] * a01[c];
+                            ux += Bz[qz][c] * a10[c];
+                            if constexpr (KF) kv += Bz[qz][c] * ka[c];
+                            if constexpr (SRC) fv += Bz[qz][c] * sa[c];
+                        }
+                        T g0 = mass * u;
+                        if (prm.reaction != T(0)) g0 += prm.dt * prm.reaction * u * (T(1) - u);
+                        if constexpr (SRC) g0 += prm.dt * fv;
+                        T gw = wz[qz];
+                        if constexpr (KF) gw *= kv;
+                        g0 *= wz[qz];
+                        const T gx = gw * ux, gy = gw * uy, gz = (cz * gw) * uz;
+#pragma unroll
+                        for (int c = 0; c < M; ++c) {
+                            H0[c] += Bz[qz][c] * g0 + Dz[qz][c] * gz;
+                            Hx[c] += Bz[qz][c] * gx;
+                            Hy[c] += Bz[qz][c] * gy;
+                        }
+                    }
+                    // stage y transposed (weights and -dt*grad_y folded into the tables)
+#pragma unroll
+                    for (int b = 0; b < M; ++b) {
+                        const T wb = bvy[(ly * M + qy) * M + b], wd = bdy[(ly * M + qy) * M + b];
+#pragma unroll
+                        for (int c = 0; c < M; ++c) {
+                            K0[b][c] += wb * H0[c] + wd * Hy[c];
+                            Kx[b][c] += wb * Hx[c];
+                        }
+                    }
+                }
+                // stage x transposed
+#pragma unroll
+                for (int a = 0; a < M; ++a) {
+                    const T wb = bvx[(lx * M + qx) * M + a], wd = bdx[(lx * M + qx) * M + a];
+#pragma unroll
+                    for (int b = 0; b < M; ++b)
+#pragma unroll
+                        for (int c = 0; c < M; ++c) R[a][b][c] += wb * K0[b][c] + wd * Kx[b][c];
+                }
+            }
+        }
+    }
+    // sum into the wave's planes: in one instruction the 64 lanes hit 64 distinct entries
+    double* mine = acc + lz * M * PL;
+#pragma unroll
+    for (int c = 0; c < M; ++c)
+#pragma unroll
+        for (int b = 0; b < M; ++b)
+#pragma unroll
+            for (int a = 0; a < M; ++a)
+                __hip_atomic_fetch_add(mine + c * PL + (lx + a) + BX * (ly + b),
+                                       static_cast<double>(R[a][b][c]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __syncthreads();
+
+    // DOF plane k of the block collects plane k - lz of the waves lz = k-P .. k
+    for (int i = tid; i < NB; i += NT) {
+        const int bxy = i % PL, bz = i / PL;
+        const int gx = ex0 + bxy % BX, gy = ey0 + bxy / BX, gz = ez0 + bz;
+        if (gx >= nx || gy >= ny || gz >= nz) continue;
+        double sum = 0;
+#pragma unroll
+        for (int d = 0; d < M; ++d) {
+            const int w = bz - d;
+            if (w >= 0 && w < TZ) sum += acc[(w * M + d) * PL + bxy];
+        }
+        rhs[gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz)] += static_cast<T>(sum);
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// element-loop RHS, 2D
+// ------------------------------------------------------------------------------------
+template <typename T, int P, int TX, int TY, bool KF, bool SRC>
+__global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __restrict__ rhs,
+                                                 dev_tables<T> X, dev_tables<T> Y,
+                                                 rhs_params<T> prm, int colour, int ntx2) {
+    constexpr int M = P + 1, NT = TX * TY;
+    constexpr int BX = TX + P, BY = TY + P, NB = BX * BY;
+    constexpr int WY = 64 / TX;  // element rows per wave
+    constexpr int NW = NT / 64;  // waves
+    constexpr int WR = WY + P;   // DOF rows touched by one wave
+    static_assert(TX <= 64 && 64 % TX == 0 && TY % WY == 0, "waves must cover whole element rows");
+    __shared__ T blk[NB];
+    __shared__ T kb[KF ? NB : 1];
+    __shared__ T sb[SRC ? NB : 1];
+    __shared__ T fvx[TX * M * M], fdx[TX * M * M], bvx[TX * M * M], bdx[TX * M * M];
+    __shared__ T fvy[TY * M * M], fdy[TY * M * M], bvy[TY * M * M], bdy[TY * M * M];
+    __shared__ double acc[NW * WR * BX];  // wave w owns DOF rows w*WY .. w*WY+WR-1
+
+    const int tid = threadIdx.x;
+    const int t = blockIdx.x;
+    const int tx = 2 * (t % ntx2) + (colour & 1);
+    const int ty = 2 * (t / ntx2) + ((colour >> 1) & 1);
+    const int ex0 = tx * TX, ey0 = ty * TY;
+    const int nx = X.dofs, ny = Y.dofs;
+
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, by = i / BX;
+        const int gx = ex0 + bx, gy = ey0 + by;
+        const bool in = gx < nx && gy < ny;
+        const long g = gx + static_cast<long>(nx) * gy;
+        blk[i] = in ? up[g] : T(0);
+        if constexpr (KF) kb[i] = in ? prm.kfield[g] : T(0);
+        if constexpr (SRC) sb[i] = in ? prm.source[g] : T(0);
+    }
+    const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy;
+    for (int i = tid; i < TX * M * M; i += NT) {
+        const int e = ex0 + i / (M * M), q = (i / M) % M;
+        const bool in = e < X.elements;
+        const T v = in ? X.val[static_cast<long>(ex0) * M * M + i] : T(0);
+        const T d = in ? X.der[static_cast<long>(ex0) * M * M + i] : T(0);
+        const T w = in ? X.wj[static_cast<long>(e) * M + q] : T(0);
+        fvx[i] = v;
+        fdx[i] = d;
+        bvx[i] = v * w;
+        bdx[i] = cx * (d * w);
+    }
+    for (int i = tid; i < TY * M * M; i += NT) {
+        const int e = ey0 + i / (M * M), q = (i / M) % M;
+        const bool in = e < Y.elements;
+        const T v = in ? Y.val[static_cast<long>(ey0) * M * M + i] : T(0);
+        const T d = in ? Y.der[static_cast<long>(ey0) * M * M + i] : T(0);
+        const T w = in ? Y.wj[static_cast<long>(e) * M + q] : T(0);
+        fvy[i] = v;
+        fdy[i] = d;
+        bvy[i] = v * w;
+        bdy[i] = d * w;
+    }
+    for (int i = tid; i < NW * WR * BX; i += NT) acc[i] = 0.0;
+    __syncthreads();
+
+    const int lx = tid % TX, ly = tid / TX;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int wy = ly - wave * WY;  // element row inside the wave
+    T R[M][M];
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = 0; b < M; ++b) R[a][b] = T(0);
+    if (ex0 + lx < X.elements && ey0 + ly < Y.elements) {
+        T By[M][M], Dy[M][M];
+#pragma unroll
+        for (int q = 0; q < M; ++q)
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                By[q][b] = fvy[(ly * M + q) * M + b];
+                Dy[q][b] = fdy[(ly * M + q) * M + b];
+            }
+        const T mass = prm.mass;
+#pragma unroll
+        for (int qx = 0; qx < M; ++qx) {
+            T bx[M], dx[M];
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                bx[a] = fvx[(lx * M + qx) * M + a];
+                dx[a] = fdx[(lx * M + qx) * M + a];
+            }
+            T A0[M], A1[M], KA[KF ? M : 1], SA[SRC ? M : 1];
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                T s0 = 0, s1 = 0, sk = 0, ss = 0;
+#pragma unroll
+                for (int a = 0; a < M; ++a) {
+                    const int li = (lx + a) + BX * (ly + b);
+                    const T cv = blk[li];
+                    s0 += bx[a] * cv;
+                    s1 += dx[a] * cv;
+                    if constexpr (KF) sk += bx[a] * kb[li];
+                    if constexpr (SRC) ss += bx[a] * sb[li];
+                }
+                A0[b] = s0;
+                A1[b] = s1;
+                if constexpr (KF) KA[b] = sk;
+                if constexpr (SRC) SA[b] = ss;
+            }
+            T K0[M], Kx[M];
+#pragma unroll
+            for (int b = 0; b < M; ++b) K0[b] = Kx[b] = T(0);
+#pragma unroll
+            for (int qy = 0; qy < M; ++qy) {
+                T u = 0, uy = 0, ux = 0, kv = 0, fv = 0;
+#pragma unroll
+                for (int b = 0; b < M; ++b) {
+                    u += By[qy][b] * A0[b];
+                    uy += Dy[qy][b] * A0[b];
+                    ux += By[qy][b] * A1[b];
+                    if constexpr (KF) kv += By[qy][b] * KA[b];
+                    if constexpr (SRC) fv += By[qy][b] * SA[b];
+                }
+                T g0 = mass * u;
+                if (prm.reaction != T(0)) g0 += prm.dt * prm.reaction * u * (T(1) - u);
+                if constexpr (SRC) g0 += prm.dt * fv;
+                T gx = ux, gy = cy * uy;
+                if constexpr (KF) {
+                    gx *= kv;
+                    gy *= kv;
+                }
+                // stage y transposed (weights folded into the backward y tables)
+#pragma unroll
+                for (int b = 0; b < M; ++b) {
+                    const T wb = bvy[(ly * M + qy) * M + b], wd = bdy[(ly * M + qy) * M + b];
+                    K0[b] += wb * g0 + wd * gy;
+                    Kx[b] += wb * gx;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                const T wb = bvx[(lx * M + qx) * M + a], wd = bdx[(lx * M + qx) * M + a];
+#pragma unroll
+                for (int b = 0; b < M; ++b) R[a][b] += wb * K0[b] + wd * Kx[b];
+            }
+        }
+    }
+    double* mine = acc + wave * WR * BX;
+#pragma unroll
+    for (int b = 0; b < M; ++b)
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+            __hip_atomic_fetch_add(mine + (wy + b) * BX + lx + a, static_cast<double>(R[a][b]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __syncthreads();
+
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, by = i / BX;
+        const int gx = ex0 + bx, gy = ey0 + by;
+        if (gx >= nx || gy >= ny) continue;
+        double sum = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int r = by - w * WY;
+            if (r >= 0 && r < WR) sum += acc[(w * WR + r) * BX + bx];
+        }
+        rhs[gx + static_cast<long>(nx) * gy] += static_cast<T>(sum);
+    }
+}
+
+// tile shapes (elements) per dimension and degree: one lane per element
+template <int D, int P>
+struct tile;
+template <int P>
+struct tile<2, P> {
+    static constexpr int X = 16, Y = 16, Z = 1;
+};
+template <>
+struct tile<3, 1> {
+    static constexpr int X = 8, Y = 8, Z = 4;
+};
+template <>
+struct tile<3, 2> {
+    static constexpr int X = 8, Y = 8, Z = 4;
+};
+template <>
+struct tile<3, 3> {
+    static constexpr int X = 8, Y = 8, Z = 3;
+};
+
+template <typename T, int P, bool KF, bool SRC, bool CREG>
+hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
+                        hipStream_t s) {
+    using TL = tile<3, P>;
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X;
+    const int nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    const int ntz = (d[2].elements + TL::Z - 1) / TL::Z;
+    for (int c = 0; c < 8; ++c) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+        const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2, nz2 = (ntz - cz + 1) / 2;
+        const long blocks = static_cast<long>(nx2) * ny2 * nz2;
+        if (blocks == 0) continue;
+        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, CREG>), dim3(blocks),
+                           dim3(TL::X * TL::Y * TL::Z), 0, s, up, rhs, d[0], d[1], d[2], prm, c,
+                           nx2, ny2);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T, int P, bool KF, bool SRC>
+hipError_t rhs2_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
+                        hipStream_t s) {
+    using TL = tile<2, P>;
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X;
+    const int nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    for (int c = 0; c < 4; ++c) {
+        const int cx = c & 1, cy = (c >> 1) & 1;
+        const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2;
+        const long blocks = static_cast<long>(nx2) * ny2;
+        if (blocks == 0) continue;
+        hipLaunchKernelGGL((k_rhs2<T, P, TL::X, TL::Y, KF, SRC>), dim3(blocks),
+                           dim3(TL::X * TL::Y), 0, s, up, rhs, d[0], d[1], prm, c, nx2);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T, int P, bool CREG>
+hipError_t rhs3_fields(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
+                       hipStream_t s) {
+    const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
+    if (kf && src) return rhs3_colours<T, P, true, true, CREG>(up, rhs, d, prm, s);
+    if (kf) return rhs3_colours<T, P, true, false, CREG>(up, rhs, d, prm, s);
+    if (src) return rhs3_colours<T, P, false, true, CREG>(up, rhs, d, prm, s);
+    return rhs3_colours<T, P, false, false, CREG>(up, rhs, d, prm, s);
+}
+
+template <typename T, int P>
+hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
+                      const rhs_params<T>& prm, int variant, hipStream_t s) {
+    const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
+    if (dim == 3) {
+        if (variant == 1) return rhs3_fields<T, P, false>(up, rhs, d, prm, s);
+        return rhs3_fields<T, P, true>(up, rhs, d, prm, s);
+    }
+    if (kf && src) return rhs2_colours<T, P, true, true>(up, rhs, d, prm, s);
+    if (kf) return rhs2_colours<T, P, true, false>(up, rhs, d, prm, s);
+    if (src) return rhs2_colours<T, P, false, true>(up, rhs, d, prm, s);
+    return rhs2_colours<T, P, false, false>(up, rhs, d, prm, s);
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
+                      const rhs_params<T>& prm, int variant, hipStream_t s) {
+    switch (p) {
+        case 1:
+            return rhs_fields<T, 1>(dim, u_prev, rhs, dirs, prm, variant, s);
+        case 2:
+            return rhs_fields<T, 2>(dim, u_prev, rhs, dirs, prm, variant, s);
+        case 3:
+            return rhs_fields<T, 3>(dim, u_prev, rhs, dirs, prm, variant, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_rhs<double>(int, int, const double*, double*, const dev_tables<double>*,
+                                       const rhs_params<double>&, int, hipStream_t);
+template hipError_t launch_rhs<float>(int, int, const float*, float*, const dev_tables<float>*,
+                                      const rhs_params<float>&, int, hipStream_t);
+
+}  // namespace adsg

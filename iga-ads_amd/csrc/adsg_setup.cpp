@@ -307,6 +307,8 @@ bool segment_lines(const line_factors& f, line_segments& out) {
     out.mseg = mseg;
     out.gf.assign(static_cast<size_t>(n) * kl, 0.0);
     out.hb.assign(static_cast<size_t>(n) * ku, 0.0);
+    out.gft.assign(static_cast<size_t>(n) * ku, 0.0);
+    out.hbt.assign(static_cast<size_t>(n) * kl, 0.0);
     auto L = [&](int j, int t) { return f.lower[static_cast<size_t>(j - t) * kl + t - 1]; };  // L(j, j-t)
     auto U = [&](int j, int t) { return f.upper[static_cast<size_t>(j + t) * ku + t - 1]; };  // U(j, j+t)
     std::vector<double> y(n + km + 1);
@@ -323,6 +325,35 @@ bool segment_lines(const line_factors& f, line_segments& out) {
                         if (j - t >= j0 - kl) acc -= L(j, t) * y[j - t];
                     y[j] = acc;
                     out.gf[static_cast<size_t>(j) * kl + q] = acc;
+                }
+            }
+        }
+        if (s > 0) {
+            // transposed forward (U^T): unit value at tail position j0-ku+q
+            for (int q = 0; q < ku; ++q) {
+                std::fill(y.begin(), y.end(), 0.0);
+                y[j0 - ku + q] = 1.0;
+                for (int j = j0; j < j1; ++j) {
+                    double acc = 0;
+                    for (int t = 1; t <= ku; ++t)
+                        if (j - t >= j0 - ku) acc -= f.upper[static_cast<size_t>(j) * ku + t - 1] * y[j - t];
+                    acc *= f.rdiag[j];
+                    y[j] = acc;
+                    out.gft[static_cast<size_t>(j) * ku + q] = acc;
+                }
+            }
+        }
+        if (s < nseg - 1) {
+            // transposed backward (L^T): unit value at head position j1+q
+            for (int q = 0; q < kl; ++q) {
+                std::fill(y.begin(), y.end(), 0.0);
+                y[j1 + q] = 1.0;
+                for (int j = j1 - 1; j >= j0; --j) {
+                    double acc = 0;
+                    for (int t = 1; t <= kl; ++t)
+                        if (j + t < std::min(n, j1 + kl)) acc -= f.lower[static_cast<size_t>(j) * kl + t - 1] * y[j + t];
+                    y[j] = acc;
+                    out.hbt[static_cast<size_t>(j) * kl + q] = acc;
                 }
             }
         }

@@ -66,7 +66,8 @@ int factor_for_lines(const std::vector<double>& ab_in, int n, int kl, int ku, li
 // does not fit the scheme (pivoted factors, too short or too long lines).
 struct line_segments {
     int m = 0, nseg = 0, mseg = 0;
-    std::vector<double> gf, hb;
+    std::vector<double> gf, hb;    // A = L U
+    std::vector<double> gft, hbt;  // A^T = U^T L^T: gft [n*ku], hbt [n*kl]
 };
 bool segment_lines(const line_factors& f, line_segments& out);
 

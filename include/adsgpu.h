@@ -129,11 +129,40 @@ int adsg_zero_boundary(adsg_sim* h);
  * simulation_2d.hpp:41). */
 int adsg_ads_solve(adsg_sim* h, const int* slots);
 
+/* The transpose of ads_solve: u := (A_x (x) A_y [(x) A_z])^-T u, direction by direction
+ * with dgbtrs 'T' semantics on the same FP64 factors (band_solve.hpp:20-31 with
+ * trans = 'T'; the reference calls dgbtrs only with 'N'). Needed by adjoint steps. FP64 handles; every
+ * slot used must have unpivoted factors and lines of 2..16 segments (true for M and
+ * M + h K of degree-p B-splines with 32 <= n <= 512). */
+int adsg_ads_solve_transposed(adsg_sim* h, const int* slots);
+
 /* nsteps of simulation_base::run's loop body (simulation_base.cpp:14-18) as in
  * validation.hpp:73-81: swap; compute_rhs(form); [zero_boundary]; ads_solve(slot 0).
  * Replayed from a HIP graph captured once per (form, flags). */
 #define ADSG_STEP_DIRICHLET 1
 int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags);
+
+/* nsteps of the linearly implicit step in increment (delta) form:
+ *   swap; buffer := r(u_prev) (the integrand of form, mass term included as given);
+ *   [zero the boundary entries of buffer]; buffer := (A_x (x) A_y [(x) A_z])^-1 buffer with
+ *   A_d the matrix in slots[d]; u := u_prev + buffer [boundary coefficients of u := 0].
+ * With form {mass = 0, dt, source} and slots holding M_d + theta*dt*K_d this is the
+ * approximately factored theta scheme, the all-direction counterpart of the directional
+ * splitting of implicit.hpp:102-111 and elasticity/implicit.hpp:464-511. In FP32 the
+ * increment is assembled and solved in FP32 on the FP64 factors; ADSG_STEP_REFINE adds one
+ * refinement of every line solve against its FP64 residual. Replayed from captured graphs. */
+#define ADSG_STEP_REFINE 2
+int adsg_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots, int flags);
+
+/* nsteps of the adjoint of adsg_step_delta with the same form, slots and flags, applied to
+ * u in place (no swap; u_prev and buffer are overwritten): for the step
+ * u' = Z (u + A^-1 Z r(u)) with r(u) = R u + c linear (reaction = 0) and Z the boundary
+ * zeroing of ADSG_STEP_DIRICHLET, u := Z u + R Z A^-T Z u. R = mass*M - dt*K is symmetric,
+ * so R applies as the form's integrand without its source. Starting from dL/du(T) this
+ * yields dL/du(0) after the same number of steps. No counterpart in the reference (it has
+ * no adjoint code); FP64 handles. */
+int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots,
+                            int flags);
 
 int adsg_synchronize(adsg_sim* h);
 

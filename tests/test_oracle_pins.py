@@ -397,3 +397,35 @@ def test_ads_matches_reference_build():
             o2 = np.zeros_like(t)
             REF.ref_cyclic_transpose(rank, po._i(s), po._d(t), po._d(o2))
             assert np.array_equal(o1, o2)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sim_transposed_solve_equals_dense_transposed_solve(dim):
+    # The adjoint needs (A_x (x) A_y (x) A_z)^-T: pinned against a dense solve with the
+    # transposed Kronecker matrix, on non-symmetric matrices (a fixed dof breaks symmetry)
+    # and on the implicit M + c K slot.
+    p, ne = [2, 3, 2][:dim], [4, 3, 5][:dim]
+    o = po.Sim(dim, p, ne)
+    for d in range(dim):
+        o.fix_dof(d, 0 if d % 2 == 0 else o.shape[d] - 1)
+        o.set_matrix(d, 1, 1.0, 0.37)
+    mats = {(d, sl): _dense_from_band(o.matrix(d, sl)[0], o.shape[d], o.p[d], o.p[d])
+            for d in range(dim) for sl in (0, 1)}
+    assert o.prepare_matrices() == 0
+    for slots in ([0] * dim, [1] * dim, [1, 0, 1][:dim]):
+        K = mats[(0, slots[0])]
+        for d in range(1, dim):
+            K = np.kron(mats[(d, slots[d])], K)
+        if 0 in slots:  # slot 1 (unfixed M + c K) is symmetric, slot 0 is not
+            assert not np.allclose(K, K.T)
+        rng = np.random.default_rng(11 + dim)
+        b = rng.standard_normal(o.size)
+        x = b.copy()
+        o.solve(x, slots=slots + [0] * (3 - dim), transposed=True)
+        # cond(K) reaches 1e5 here: compare in relative L2
+        assert po.rel_l2(x, np.linalg.solve(K.T, b)) < 1e-11
+        y = b.copy()
+        o.solve(y, slots=slots + [0] * (3 - dim))
+        assert po.rel_l2(y, np.linalg.solve(K, b)) < 1e-11
+        if 0 in slots:
+            assert po.rel_l2(x, y) > 1e-3  # the transpose is not the plain solve here
