@@ -34,7 +34,9 @@ def f_elem(d, p):
     return 2 * c * 2 * n ** (d + 1) + (d + 2) * n ** d
 
 
-def run_form(ag, po, precision, ne, steps, warmup, u0, dt, variant=None):
+def run_form(ag, precision, ne, steps, warmup, u0, dt, variant=None):
+    """Plain explicit heat step (c2 form) in the given precision: autotune, warm up, time
+    `steps` graph-replayed steps, then time the phases of 3 eager steps."""
     sim = ag.Simulation(3, [2, 2, 2], [ne, ne, ne], precision=precision)
     sim.prepare_matrices()
     sim.set(ag.U, u0)
@@ -49,31 +51,121 @@ def run_form(ag, po, precision, ne, steps, warmup, u0, dt, variant=None):
     sim.step(steps, form)
     sim.synchronize()
     t1 = time.perf_counter()
-    phases = sim.time_phases(3, form)  # eager, events per phase
-    sim.synchronize()
     out = sim.get(ag.U)
+    phases = sim.time_phases(3, form)  # eager, events per phase
     sim.close()
     return (t1 - t0), phases, out, variant
 
 
-def copy_calibration(mib, reps):
-    """A plain device-to-device copy of known size through the HIP runtime (its copy
-    kernel reads and writes mib MiB per launch): the reference point for scaling the
-    FETCH_SIZE / WRITE_SIZE counters of the step's kernels."""
-    import ctypes as C
+def run_implicit(ag, precision, ne, steps, warmup, u0, f, dt, flags):
+    """Implicit delta-form heat step with source (p=3): u += A^-1 dt (f - K u),
+    A = (x)_d (M_d + dt/2 K_d)."""
+    sim = ag.Simulation(3, [3, 3, 3], [ne, ne, ne], precision=precision)
+    for d in range(3):
+        sim.set_matrix(d, 1, 1.0, 0.5 * dt)
+    sim.prepare_matrices()
+    sim.set(ag.U, u0)
+    sim.set(ag.SOURCE, f)
+    form = ag.rhs_form(dt, mass=0.0, source=True)
+    slots = [1, 1, 1]
+    variant = sim.autotune(form, slots, flags)
+    sim.step_delta(warmup, form, slots, flags)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step_delta(steps, form, slots, flags)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    out = sim.get(ag.U)
+    phases = sim.time_phases(2, form, slots, flags)
+    return sim, (t1 - t0), phases, out, variant
 
-    hip = C.CDLL("libamdhip64.so")
-    n = mib << 20
-    a, b = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
-    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
-    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
-    for _ in range(reps):
-        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
-    print(json.dumps({"copy_calibration_mib": mib, "launches": reps}), flush=True)
+
+def implicit_block(ag, np, ne, steps, warmup):
+    """3D implicit heat with source, p=3, ne^3, dt=1e-4: FP64, FP32 + refinement, and the
+    adjoint of ||u(T)||^2 with respect to u(0)."""
+    from adsgpu import workloads as wl
+
+    dt = 1e-4
+    p3 = [3, 3, 3]
+    u0 = wl.gaussians(p3, [ne] * 3, seed=2025, count=8, sigma=(0.05, 0.15), centre=(0.2, 0.8),
+                      noise=1e-3)
+    f = wl.sine_source(p3, [ne] * 3, seed=2026)
+    N = u0.size
+    total = warmup + steps
+    sim64, t64, ph64, x64, v64 = run_implicit(ag, ag.FP64, ne, steps, warmup, u0, f, dt, 0)
+    res = {
+        "workload": (f"3D implicit heat with source, p=3, {ne}^3 elements ({N} DOFs), dt=1e-4, "
+                     f"{steps} timed steps after {warmup}; u += A^-1 dt (f - K u), "
+                     f"A = (x)_d (M_d + dt/2 K_d); natural BC"),
+        "fp64": {"value": N * steps / t64, "ms_per_step": 1e3 * t64 / steps, "phases_ms": ph64,
+                 "rhs_variant": v64,
+                 "rhs_tflops": f_elem_source(3, 3) * ne ** 3 / (ph64["rhs"] * 1e-3) / 1e12},
+    }
+    for name, flags in (("fp32", 0), ("fp32_refined", ag.STEP_REFINE)):
+        s32, t32, ph32, x32, v32 = run_implicit(ag, ag.FP32, ne, steps, warmup, u0, f, dt, flags)
+        s32.close()
+        res[name] = {"value": N * steps / t32, "ms_per_step": 1e3 * t32 / steps,
+                     "speedup_vs_fp64": t64 / t32, "phases_ms": ph32, "rhs_variant": v32,
+                     f"rel_l2_vs_fp64_after_{total}_steps":
+                         float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))}
+    # adjoint: lam(T) = 2 u(T) for L = ||u(T)||^2, run back over the same number of steps
+    form = ag.rhs_form(dt, mass=0.0, source=True)
+    slots = [1, 1, 1]
+    sim64.set(ag.U, 2.0 * x64)
+    sim64.adjoint_step_delta(1, form, slots)  # graph capture and code load
+    sim64.set(ag.U, 2.0 * x64)
+    sim64.synchronize()
+    ta = time.perf_counter()
+    sim64.adjoint_step_delta(total, form, slots)
+    sim64.synchronize()
+    ta = time.perf_counter() - ta
+    grad = sim64.get(ag.U)
+
+    def loss(u):
+        sim64.set(ag.U, u)
+        sim64.step_delta(total, form, slots)
+        v = sim64.get(ag.U)
+        return float(v @ v)
+
+    # L is exactly quadratic in u(0), so the central difference is exact up to rounding for
+    # any h; h = 1e-3 keeps the rounding of L (about 1e6) below 1e-8 relative
+    d = np.random.default_rng(2027).standard_normal(N)
+    d /= np.linalg.norm(d)
+    h = 1e-3
+    fd = (loss(u0 + h * d) - loss(u0 - h * d)) / (2 * h)
+    ad = float(grad @ d)
+    sim64.close()
+    res["adjoint_fp64"] = {"value": N * total / ta, "ms_per_step": 1e3 * ta / total,
+                           "steps": total, "fd_direction_rel_err": abs(fd - ad) / abs(ad),
+                           "fd_h": h}
+    return res
+
+
+def f_elem_source(d, p):
+    """Flops per element of the delta-form integrand dt (f v - grad u . grad v), 3D: the
+    forward tree needs 8 one-dimensional contractions for grad u (value and derivative in x;
+    three mixed y stages; three z stages) and 3 for the value of f; the transposed tree
+    needs 9 (the value path carries f v). Each contraction costs 2 n^(d+1); the pointwise
+    work is (d+2) n^d."""
+    assert d == 3
+    n = p + 1
+    return 2 * (8 + 3 + 9) * n ** (d + 1) + (d + 2) * n ** d
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_rhs3_heat3d_p2_128.md")
+
+
+def launch_traffic():
+    """HBM bytes per k_rhs3 launch at 128^3 from the committed counter summary (the line
+    `traffic_bytes_per_launch: <bytes>`), or None when there is none."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -86,6 +178,10 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--rhs-variant", type=int, default=None,
                     help="element-kernel variant (default: autotuned)")
+    ap.add_argument("--no-implicit", action="store_true",
+                    help="skip the 3D p=3 implicit / adjoint block")
+    ap.add_argument("--implicit-elements", type=int, default=192)
+    ap.add_argument("--implicit-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain D2D copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -101,26 +197,27 @@ def main():
     import numpy as np
 
     import adsgpu as ag
-
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po  # initial state generator and the CPU baseline only
+    from adsgpu import workloads as wl
 
     ne, dt = args.elements, 1e-6
-    init = po.Sim(3, [2, 2, 2], [ne, ne, ne])
-    u0 = init.init_gaussians(seed=2024, count=8, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
-    N = init.size
+    u0 = wl.gaussians([2, 2, 2], [ne] * 3, seed=2024, count=8, sigma=(0.05, 0.15),
+                      centre=(0.2, 0.8), noise=1e-3)
+    N = u0.size
     E = ne ** 3
 
-    t64, ph64, x64, var64 = run_form(ag, po, ag.FP64, ne, args.steps, args.warmup, u0, dt,
+    t64, ph64, x64, var64 = run_form(ag, ag.FP64, ne, args.steps, args.warmup, u0, dt,
                                      args.rhs_variant)
     value = N * args.steps / t64
     ms_step = 1e3 * t64 / args.steps
 
-    flops_step = f_elem(3, 2) * E
-    rhs_ms = ph64[0]
-    rhs_tflops = flops_step / (rhs_ms * 1e-3) / 1e12
+    # dominant kernel: k_rhs3, launched once per parity colour (8 launches per step, each
+    # over E/8 elements); HIP events on the handle's stream bracket the 8 launches
+    launches = 8
+    launch_flops = f_elem(3, 2) * E / launches
+    launch_ms = ph64["rhs"] / launches
+    rhs_tflops = launch_flops / (launch_ms * 1e-3) / 1e12
     sweep_bytes = 2 * N * 8  # one directional sweep: read and write every coefficient once
-    sweep_ms = [ph64[2], ph64[3], ph64[4]]
+    sweep_ms = [ph64["sweep_x"], ph64["sweep_y"], ph64["sweep_z"]]
     sweep_gbs = [sweep_bytes / (m * 1e-3) / 1e9 for m in sweep_ms]
 
     result = {
@@ -142,18 +239,22 @@ def main():
             "dim": 3, "p": 2, "elements": ne, "dofs": N, "dt": dt,
         },
         "roofline": {
-            "kernel": "k_rhs3 (element-loop RHS assembly, 8 colour passes + zero fill per step)",
+            "kernel": "k_rhs3 (element-loop RHS assembly; one launch per parity colour)",
             "bound": "mfma",
             "achieved": rhs_tflops,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": rhs_tflops / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": launch_traffic(),
+            "launch_flops": launch_flops,
+            "launch_ms": launch_ms,
             "note": ("FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 matrix peak = "
-                     "78.6 TFLOP/s); achieved = 3051 flop/element x elements / RHS phase time"),
+                     "78.6 TFLOP/s); achieved = 3051 flop/element x E/8 elements per launch / "
+                     "average launch time (HIP events around the 8 launches on the handle's "
+                     "stream); traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE and "
+                     "WRITE_SIZE scaled by a copy-kernel calibration"),
         },
-        "phases_ms": {"rhs": ph64[0], "sweep_x": ph64[2], "sweep_y": ph64[3],
-                      "sweep_z": ph64[4], "step_eager": ph64[5]},
+        "phases_ms": ph64,
         "rhs_variant": var64,
         "solve_roofline": {
             "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
@@ -164,7 +265,7 @@ def main():
     }
 
     if not args.no_fp32:
-        t32, ph32, x32, var32 = run_form(ag, po, ag.FP32, ne, args.steps, args.warmup, u0, dt,
+        t32, ph32, x32, var32 = run_form(ag, ag.FP32, ne, args.steps, args.warmup, u0, dt,
                                          args.rhs_variant)
         err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
         result["forms"] = {
@@ -175,12 +276,18 @@ def main():
                 "speedup_vs_fp64": t64 / t32,
                 "rel_l2_vs_fp64_after_all_steps": err,
                 "rhs_variant": var32,
-                "phases_ms": {"rhs": ph32[0], "sweep_x": ph32[2], "sweep_y": ph32[3],
-                              "sweep_z": ph32[4], "step_eager": ph32[5]},
+                "phases_ms": ph32,
             },
         }
 
+    if not args.no_implicit:
+        result.setdefault("forms", {})["implicit_p3"] = implicit_block(
+            ag, np, args.implicit_elements, args.implicit_steps, 2)
+
     if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po  # the CPU baseline leg only
+
         threads = max(1, min(16, os.cpu_count() or 1))
         o = po.Sim(3, [2, 2, 2], [ne, ne, ne])
         o.prepare_matrices()

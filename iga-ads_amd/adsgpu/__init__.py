@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libadsgpu.so")
 
 OK, EINVAL, EHIP, ENOMEM, ESTATE, ENONFINITE = 0, -1, -2, -3, -4, -5
 FP64, FP32 = 0, 1
-U, U_PREV, BUFFER, KFIELD, SOURCE = 0, 1, 2, 3, 4
+U, U_PREV, BUFFER, KFIELD, SOURCE, GRAD_K = 0, 1, 2, 3, 4, 5
 STEP_DIRICHLET = 1
 STEP_REFINE = 2
 
@@ -61,7 +61,8 @@ EXPORTS = [
     "adsg_fix_dof", "adsg_set_matrix", "adsg_set_band_matrix", "adsg_prepare_matrices",
     "adsg_set_vector", "adsg_get_vector", "adsg_device_ptr", "adsg_swap", "adsg_compute_rhs",
     "adsg_zero_boundary", "adsg_ads_solve", "adsg_ads_solve_transposed", "adsg_step",
-    "adsg_step_delta", "adsg_adjoint_step_delta", "adsg_synchronize",
+    "adsg_step_delta", "adsg_adjoint_step", "adsg_adjoint_step_delta", "adsg_synchronize",
+    "adsg_set_vector_device", "adsg_get_vector_device",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
     "adsg_autotune",
 ]
@@ -100,12 +101,15 @@ def lib():
         L.adsg_ads_solve_transposed.argtypes = [C.c_void_p, _ip]
         L.adsg_step_delta.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), _ip, C.c_int]
         L.adsg_adjoint_step_delta.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), _ip, C.c_int]
+        L.adsg_adjoint_step.argtypes = [C.c_void_p, C.POINTER(RhsForm), C.c_int]
+        L.adsg_set_vector_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.adsg_get_vector_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_synchronize.argtypes = [C.c_void_p]
         L.adsg_band_factorize.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip]
-        L.adsg_time_phases.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), C.c_int, _dp]
+        L.adsg_time_phases.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), _ip, C.c_int, _dp]
         L.adsg_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.adsg_get_tuning.argtypes = [C.c_void_p, C.c_int]
-        L.adsg_autotune.argtypes = [C.c_void_p, C.POINTER(RhsForm), C.c_int]
+        L.adsg_autotune.argtypes = [C.c_void_p, C.POINTER(RhsForm), _ip, C.c_int]
         _lib = L
     return _lib
 
@@ -218,6 +222,18 @@ class Simulation:
         self._check(lib().adsg_step_delta(self.h, n, C.byref(form), self._slots(slots), flags),
                     "step_delta")
 
+    def adjoint_step(self, form, flags=0):
+        """One adjoint step of step(): u holds lam(n+1), u_prev the forward state u(n);
+        u := lam(n) and dL/dk is added into GRAD_K when the form uses the k field."""
+        self._check(lib().adsg_adjoint_step(self.h, C.byref(form), flags), "adjoint_step")
+
+    def set_device(self, which, ptr):
+        """Copy from FP64 device memory at ptr (e.g. a tensor's data_ptr())."""
+        self._check(lib().adsg_set_vector_device(self.h, which, C.c_void_p(ptr)), "set_vector_device")
+
+    def get_device(self, which, ptr):
+        self._check(lib().adsg_get_vector_device(self.h, which, C.c_void_p(ptr)), "get_vector_device")
+
     def adjoint_step_delta(self, n, form, slots, flags=0):
         """n adjoint steps of step_delta applied to u in place (u_prev, buffer clobbered)."""
         self._check(lib().adsg_adjoint_step_delta(self.h, n, C.byref(form), self._slots(slots),
@@ -226,12 +242,15 @@ class Simulation:
     def synchronize(self):
         self._check(lib().adsg_synchronize(self.h), "synchronize")
 
-    def time_phases(self, n, form, flags=0):
-        """Average device ms per step: rhs, boundary, sweep x, sweep y, sweep z, step."""
-        ms = np.zeros(6)
-        self._check(lib().adsg_time_phases(self.h, n, C.byref(form), flags, ms.ctypes.data_as(_dp)),
-                    "time_phases")
-        return ms
+    PHASES = ("rhs", "boundary", "sweep_x", "sweep_y", "sweep_z", "step", "fill")
+
+    def time_phases(self, n, form, slots=None, flags=0):
+        """Average device ms per phase of n eager steps (explicit steps, or delta steps with
+        slots): element kernels, boundary handling, x/y/z sweeps, whole step, zero fill."""
+        ms = np.zeros(7)
+        self._check(lib().adsg_time_phases(self.h, n, C.byref(form), self._slots(slots), flags,
+                                           ms.ctypes.data_as(_dp)), "time_phases")
+        return dict(zip(self.PHASES, ms.tolist()))
 
     def set_tuning(self, key, value):
         self._check(lib().adsg_set_tuning(self.h, key, value), "set_tuning")
@@ -239,9 +258,19 @@ class Simulation:
     def get_tuning(self, key):
         return lib().adsg_get_tuning(self.h, key)
 
-    def autotune(self, form, flags=0):
-        self._check(lib().adsg_autotune(self.h, C.byref(form), flags), "autotune")
+    def autotune(self, form, slots=None, flags=0):
+        self._check(lib().adsg_autotune(self.h, C.byref(form), self._slots(slots), flags),
+                    "autotune")
         return self.get_tuning(TUNE_RHS_VARIANT)
 
     def device_ptr(self, which):
         return lib().adsg_device_ptr(self.h, which)
+
+
+def heat_k(sim, k, u0, steps, every, dt, dirichlet=True):
+    """u(T) of `steps` explicit heat steps with diffusion field k, as a differentiable torch
+    operation (libtorch autograd Function; backward = hand-written adjoint, checkpoints
+    every `every` steps). k and u0: float64 GPU tensors of sim.size entries."""
+    from . import adsg_torch  # needs torch; built beside libadsgpu.so
+
+    return adsg_torch.heat_k(k, u0, sim.h.value, steps, every, dt, dirichlet)

@@ -61,7 +61,7 @@ struct adsg_sim {
     // with them) and a step is computed as an increment: compute_rhs assembles
     // rhs - M u_prev in FP32 into buffer, ads_solve turns it into M^-1 (rhs - M u_prev) with
     // FP32 sweeps on FP64-computed factors and adds it to u_prev in FP64.
-    void* vec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* vec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     void* shadow[2] = {nullptr, nullptr};
     bool pending = false;            // FP32: buffer holds an increment based on u_prev
     bool pending_dirichlet = false;  // FP32: boundary rows of the increment were fixed
@@ -69,6 +69,7 @@ struct adsg_sim {
     int rhs_variant = 0;    // element-kernel tuning choice (adsg_set_tuning)
     int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
     std::string err;
+    hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
     // captured step graphs keyed by (form, flags, u pointer, steps in graph)
     std::map<std::tuple<std::string, int, void*, int>, hipGraphExec_t> graphs;
 };
@@ -77,7 +78,7 @@ namespace {
 
 // bytes per coefficient of a vector
 size_t elt_of(const adsg_sim* h, int which) {
-    if (which == ADSG_U || which == ADSG_U_PREV) return sizeof(double);
+    if (which == ADSG_U || which == ADSG_U_PREV || which == ADSG_GRAD_K) return sizeof(double);
     return h->precision == ADSG_FP32 ? sizeof(float) : sizeof(double);
 }
 
@@ -95,6 +96,12 @@ int hip_fail(adsg_sim* h, hipError_t e, const char* what) {
         hipError_t e_ = (expr);                               \
         if (e_ != hipSuccess) return hip_fail((h), e_, #expr); \
     } while (0)
+
+// phase boundary i of a timed step: 1 after the zero fill, 2 after the element kernels,
+// 3 after the boundary handling, 4..6 after the x, y, z sweeps
+void mark(adsg_sim* h, int i) {
+    if (h->marks) (void)hipEventRecord(h->marks[i], h->stream);
+}
 
 template <typename T>
 int upload(adsg_sim* h, void** dst, const std::vector<double>& src) {
@@ -210,9 +217,12 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0) {
 // ads_solve: one sweep per direction in the order x, y, z (solver.hpp:35-41)
 template <typename T>
 int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0) {
-    for (int d = 0; d < h->dim; ++d) {
-        int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode);
-        if (rc) return rc;
+    for (int d = 0; d < 3; ++d) {
+        if (d < h->dim) {
+            int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode);
+            if (rc) return rc;
+        }
+        mark(h, 4 + d);
     }
     return ADSG_OK;
 }
@@ -257,8 +267,10 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
     }
     hipError_t e = adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
     if (e != hipSuccess) return hip_fail(h, e, "zero rhs");
+    mark(h, 1);
     e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->rhs_variant, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
+    mark(h, 2);
     if constexpr (sizeof(T) == sizeof(float)) {
         h->pending = !delta;
         h->pending_dirichlet = false;
@@ -338,6 +350,7 @@ int one_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
         rc = zero_boundary_impl(h);
         if (rc) return rc;
     }
+    mark(h, 3);
     return solve_impl(h, nullptr);
 }
 
@@ -354,6 +367,7 @@ int one_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int
         float* r = static_cast<float*>(h->vec[ADSG_BUFFER]);
         if (dir && (e = adsg::launch_zero_boundary<float>(h->dim, r, nx, ny, nz, h->stream)) != hipSuccess)
             return hip_fail(h, e, "zero boundary");
+        mark(h, 3);
         if ((rc = sweep_all<float>(h, r, slots, (flags & ADSG_STEP_REFINE) ? 2 : 0))) return rc;
         e = adsg::launch_apply_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
                                          static_cast<double*>(h->vec[ADSG_U]),
@@ -363,12 +377,77 @@ int one_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int
         double* r = static_cast<double*>(h->vec[ADSG_BUFFER]);
         if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, r, nx, ny, nz, h->stream)) != hipSuccess)
             return hip_fail(h, e, "zero boundary");
+        mark(h, 3);
         if ((rc = sweep_all<double>(h, r, slots))) return rc;
         e = adsg::launch_add_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
                                        static_cast<double*>(h->vec[ADSG_U]), h->dim, nx, ny, nz, dir,
                                        h->stream);
     }
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "apply increment");
+}
+
+void fp64_tables(const adsg_sim* h, adsg::dev_tables<double>* t) {
+    for (int d = 0; d < h->dim; ++d) {
+        t[d].val = static_cast<const double*>(h->dirs[d].val);
+        t[d].der = static_cast<const double*>(h->dirs[d].der);
+        t[d].wj = static_cast<const double*>(h->dirs[d].wj);
+        t[d].elements = h->dirs[d].tab.elements;
+        t[d].dofs = h->dirs[d].tab.dofs;
+    }
+}
+
+// R^T v = R v for the linear part of form (no source, no reaction), into out (zeroed first)
+int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, double* out) {
+    adsg::dev_tables<double> t[3];
+    fp64_tables(h, t);
+    adsg::rhs_params<double> prm;
+    prm.mass = form->mass;
+    prm.dt = form->dt;
+    prm.gx = form->grad[0];
+    prm.gy = form->grad[1];
+    prm.gz = form->grad[2];
+    prm.reaction = 0.0;
+    if (form->use_kfield) {
+        if (!h->vec[ADSG_KFIELD]) return fail(h, ADSG_ESTATE, "adjoint: k field not set");
+        prm.kfield = static_cast<const double*>(h->vec[ADSG_KFIELD]);
+    }
+    hipError_t e;
+    if ((e = adsg::launch_fill<double>(out, h->N, 0.0, h->stream)) != hipSuccess) return hip_fail(h, e, "zero");
+    mark(h, 1);
+    e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, h->rhs_variant, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
+    mark(h, 2);
+    return ADSG_OK;
+}
+
+// One step of the adjoint of one_step (FP64): see adsg_adjoint_step in adsgpu.h.
+int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
+    int nx, ny, nz;
+    extents(h, nx, ny, nz);
+    const bool dir = flags & ADSG_STEP_DIRICHLET;
+    double* lam = static_cast<double*>(h->vec[ADSG_U]);
+    double* w = static_cast<double*>(h->vec[ADSG_BUFFER]);
+    const double* un = static_cast<const double*>(h->vec[ADSG_U_PREV]);
+    HIP_TRY(h, hipMemcpyAsync(w, lam, sizeof(double) * h->N, hipMemcpyDeviceToDevice, h->stream));
+    int rc = sweep_all<double>(h, w, nullptr, 1);
+    if (rc) return rc;
+    hipError_t e;
+    if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, w, nx, ny, nz, h->stream)) != hipSuccess)
+        return hip_fail(h, e, "zero boundary");
+    mark(h, 3);
+    if (form->use_kfield) {
+        adsg::dev_tables<double> t[3];
+        fp64_tables(h, t);
+        adsg::rhs_params<double> prm;
+        prm.dt = form->dt;
+        prm.gx = form->grad[0];
+        prm.gy = form->grad[1];
+        prm.gz = form->grad[2];
+        e = adsg::launch_kgrad<double>(h->dim, h->dirs[0].tab.p, un, w,
+                                       static_cast<double*>(h->vec[ADSG_GRAD_K]), t, prm, h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "k-field gradient");
+    }
+    return apply_linear_form(h, form, w, lam);
 }
 
 // One step of the adjoint of one_delta_step (FP64) for a linear form r(u) = R u + c:
@@ -387,34 +466,12 @@ int one_adjoint_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* sl
     if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, lam, nx, ny, nz, h->stream)) != hipSuccess)
         return hip_fail(h, e, "zero boundary");
     HIP_TRY(h, hipMemcpyAsync(v, lam, sizeof(double) * h->N, hipMemcpyDeviceToDevice, h->stream));
+    mark(h, 3);
     int rc = sweep_all<double>(h, v, slots, 1);
     if (rc) return rc;
     if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, v, nx, ny, nz, h->stream)) != hipSuccess)
         return hip_fail(h, e, "zero boundary");
-    adsg_rhs_form lin = *form;
-    lin.use_source = 0;
-    adsg::dev_tables<double> t[3];
-    for (int d = 0; d < h->dim; ++d) {
-        t[d].val = static_cast<const double*>(h->dirs[d].val);
-        t[d].der = static_cast<const double*>(h->dirs[d].der);
-        t[d].wj = static_cast<const double*>(h->dirs[d].wj);
-        t[d].elements = h->dirs[d].tab.elements;
-        t[d].dofs = h->dirs[d].tab.dofs;
-    }
-    adsg::rhs_params<double> prm;
-    prm.mass = lin.mass;
-    prm.dt = lin.dt;
-    prm.gx = lin.grad[0];
-    prm.gy = lin.grad[1];
-    prm.gz = lin.grad[2];
-    prm.reaction = 0.0;
-    if (lin.use_kfield) {
-        if (!h->vec[ADSG_KFIELD]) return fail(h, ADSG_ESTATE, "adjoint: k field not set");
-        prm.kfield = static_cast<const double*>(h->vec[ADSG_KFIELD]);
-    }
-    if ((e = adsg::launch_fill<double>(w, h->N, 0.0, h->stream)) != hipSuccess) return hip_fail(h, e, "zero");
-    e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, w, t, prm, h->rhs_variant, h->stream);
-    if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
+    if ((rc = apply_linear_form(h, form, v, w))) return rc;
     e = adsg::launch_add_increment(lam, w, lam, h->dim, nx, ny, nz, false, h->stream);
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "adjoint update");
 }
@@ -730,7 +787,7 @@ int adsg_prepare_matrices(adsg_sim* h) {
 }
 
 int adsg_set_vector(adsg_sim* h, int which, const double* host) {
-    if (!h || !host || which < ADSG_U || which > ADSG_SOURCE) return fail(h, ADSG_EINVAL, "set_vector: bad argument");
+    if (!h || !host || which < ADSG_U || which > ADSG_GRAD_K) return fail(h, ADSG_EINVAL, "set_vector: bad argument");
     const size_t es = elt_of(h, which);
     if (!h->vec[which]) {
         if (hipMalloc(&h->vec[which], h->N * es) != hipSuccess)
@@ -755,7 +812,7 @@ int adsg_set_vector(adsg_sim* h, int which, const double* host) {
 }
 
 int adsg_get_vector(adsg_sim* h, int which, double* host) {
-    if (!h || !host || which < ADSG_U || which > ADSG_SOURCE || !h->vec[which])
+    if (!h || !host || which < ADSG_U || which > ADSG_GRAD_K || !h->vec[which])
         return fail(h, ADSG_EINVAL, "get_vector: bad argument");
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     if (elt_of(h, which) == sizeof(float)) {
@@ -769,8 +826,38 @@ int adsg_get_vector(adsg_sim* h, int which, double* host) {
 }
 
 void* adsg_device_ptr(adsg_sim* h, int which) {
-    if (!h || which < ADSG_U || which > ADSG_SOURCE) return nullptr;
+    if (!h || which < ADSG_U || which > ADSG_GRAD_K) return nullptr;
     return h->vec[which];
+}
+
+int adsg_set_vector_device(adsg_sim* h, int which, const double* dev) {
+    if (!h || !dev || which < ADSG_U || which > ADSG_GRAD_K)
+        return fail(h, ADSG_EINVAL, "set_vector_device: bad argument");
+    if (elt_of(h, which) != sizeof(double))
+        return fail(h, ADSG_EINVAL, "set_vector_device: FP32 vectors are set from the host");
+    if (!h->vec[which]) {
+        if (hipMalloc(&h->vec[which], h->N * sizeof(double)) != hipSuccess)
+            return fail(h, ADSG_ENOMEM, "set_vector_device: device allocation failed");
+        drop_graphs(h);
+    }
+    HIP_TRY(h, hipMemcpyAsync(h->vec[which], dev, h->N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (h->precision == ADSG_FP32 && (which == ADSG_U || which == ADSG_U_PREV)) {
+        hipError_t e = adsg::launch_convert_d2f(static_cast<const double*>(h->vec[which]),
+                                                static_cast<float*>(h->shadow[which]), h->N, h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "set_vector_device: shadow");
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return ADSG_OK;
+}
+
+int adsg_get_vector_device(adsg_sim* h, int which, double* dev) {
+    if (!h || !dev || which < ADSG_U || which > ADSG_GRAD_K || !h->vec[which])
+        return fail(h, ADSG_EINVAL, "get_vector_device: bad argument");
+    if (elt_of(h, which) != sizeof(double))
+        return fail(h, ADSG_EINVAL, "get_vector_device: FP32 vectors are read to the host");
+    HIP_TRY(h, hipMemcpyAsync(dev, h->vec[which], h->N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return ADSG_OK;
 }
 
 int adsg_swap(adsg_sim* h) {
@@ -810,6 +897,17 @@ int adsg_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const in
     return run_graphs(h, DELTA_STEP, nsteps, form, flags, slots);
 }
 
+int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
+    if (!h || !form) return fail(h, ADSG_EINVAL, "adjoint_step: bad argument");
+    if (!h->prepared) return fail(h, ADSG_ESTATE, "adjoint_step: prepare_matrices first");
+    if (h->precision != ADSG_FP64) return fail(h, ADSG_EINVAL, "adjoint_step: FP64 handles only");
+    if (form->reaction != 0.0)
+        return fail(h, ADSG_EINVAL, "adjoint_step: the form must be linear in u (reaction = 0)");
+    if (form->use_kfield && !h->vec[ADSG_GRAD_K])
+        return fail(h, ADSG_ESTATE, "adjoint_step: set ADSG_GRAD_K (the accumulator) first");
+    return one_adjoint_step(h, form, flags);
+}
+
 int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots,
                             int flags) {
     if (!h || !form || !slots || nsteps < 0) return fail(h, ADSG_EINVAL, "adjoint_step_delta: bad argument");
@@ -834,50 +932,37 @@ int adsg_synchronize(adsg_sim* h) {
     return ADSG_OK;
 }
 
-int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags, double* ms) {
+int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots, int flags,
+                     double* ms) {
     if (!h || !form || !ms || nsteps <= 0) return fail(h, ADSG_EINVAL, "time_phases: bad argument");
     if (!h->prepared) return fail(h, ADSG_ESTATE, "time_phases: prepare_matrices first");
-    hipEvent_t ev[7];
+    const step_kind kind = slots ? DELTA_STEP : EXPLICIT_STEP;
+    hipEvent_t ev[8];
     for (auto& e : ev) HIP_TRY(h, hipEventCreate(&e));
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     int rc = ADSG_OK;
+    h->marks = ev;
     for (int s = 0; s < nsteps && rc == ADSG_OK; ++s) {
-        swap_state(h);
         (void)hipEventRecord(ev[0], h->stream);
-        rc = rhs_dispatch(h, form);
-        (void)hipEventRecord(ev[1], h->stream);
-        if (!rc && (flags & ADSG_STEP_DIRICHLET)) rc = zero_boundary_impl(h);
-        (void)hipEventRecord(ev[2], h->stream);
-        for (int d = 0; d < 3; ++d) {
-            if (!rc && d < h->dim) {
-                rc = h->precision == ADSG_FP32
-                         ? sweep_dir<float>(h, static_cast<float*>(h->vec[ADSG_BUFFER]), d, 0)
-                         : sweep_dir<double>(h, static_cast<double*>(h->vec[ADSG_U]), d, 0);
-            }
-            (void)hipEventRecord(ev[3 + d], h->stream);
+        rc = step_of(h, kind, form, slots, flags);
+        (void)hipEventRecord(ev[7], h->stream);
+        if (rc) break;
+        hipError_t e = hipEventSynchronize(ev[7]);
+        if (e != hipSuccess) {
+            rc = hip_fail(h, e, "hipEventSynchronize");
+            break;
         }
-        if (!rc && h->precision == ADSG_FP32) {
-            int nx, ny, nz;
-            extents(h, nx, ny, nz);
-            hipError_t e = adsg::launch_apply_increment(
-                static_cast<const double*>(h->vec[ADSG_U_PREV]), static_cast<const float*>(h->vec[ADSG_BUFFER]),
-                static_cast<double*>(h->vec[ADSG_U]), static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz,
-                h->pending_dirichlet, h->stream);
-            if (e != hipSuccess) rc = hip_fail(h, e, "apply increment");
-            h->pending = h->pending_dirichlet = false;
-        }
-        (void)hipEventRecord(ev[6], h->stream);
-        HIP_TRY(h, hipEventSynchronize(ev[6]));
         float t;
-        const int from[6] = {0, 1, 2, 3, 4, 0};
-        const int to[6] = {1, 2, 3, 4, 5, 6};
-        for (int i = 0; i < 6; ++i) {
+        const int from[7] = {1, 2, 3, 4, 5, 0, 0};
+        const int to[7] = {2, 3, 4, 5, 6, 7, 1};
+        for (int i = 0; i < 7; ++i) {
             (void)hipEventElapsedTime(&t, ev[from[i]], ev[to[i]]);
             acc[i] += t;
         }
     }
+    h->marks = nullptr;
     for (auto& e : ev) (void)hipEventDestroy(e);
-    for (int i = 0; i < 6; ++i) ms[i] = acc[i] / nsteps;
+    for (int i = 0; i < 7; ++i) ms[i] = acc[i] / nsteps;
     return rc;
 }
 
@@ -899,7 +984,7 @@ int adsg_get_tuning(const adsg_sim* h, int key) {
     return ADSG_EINVAL;
 }
 
-int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, int flags) {
+int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags) {
     if (!h || !form) return fail(h, ADSG_EINVAL, "autotune: bad argument");
     if (!h->prepared) return fail(h, ADSG_ESTATE, "autotune: prepare_matrices first");
     // keep the caller's state: copies of u, u_prev (and their FP32 shadows)
@@ -926,9 +1011,9 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, int flags) {
     for (int v = 0; v <= 1 && rc == ADSG_OK; ++v) {
         h->rhs_variant = v;
         h->sweep_variant = v;
-        double ms[6];
-        rc = adsg_time_phases(h, 1, form, flags, ms);  // warm-up (code load, caches)
-        if (!rc) rc = adsg_time_phases(h, 3, form, flags, ms);
+        double ms[7];
+        rc = adsg_time_phases(h, 1, form, slots, flags, ms);  // warm-up (code load, caches)
+        if (!rc) rc = adsg_time_phases(h, 3, form, slots, flags, ms);
         if (!rc && ms[0] < best_ms) {
             best_ms = ms[0];
             best = v;

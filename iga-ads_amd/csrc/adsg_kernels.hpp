@@ -76,6 +76,12 @@ template <typename T>
 hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
                       const rhs_params<T>& prm, int variant, hipStream_t s);
 
+// d/dk of w^T rhs(u; k) for the gradient term of prm, added into out:
+//   out_b += -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d (d_d u)(d_d w)
+template <typename T>
+hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,
+                        const rhs_params<T>& prm, hipStream_t s);
+
 template <typename T>
 hipError_t launch_fill(T* p, long n, T v, hipStream_t s);
 

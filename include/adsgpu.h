@@ -43,6 +43,7 @@ extern "C" {
 #define ADSG_BUFFER 2 /* scratch of the same shape (simulation_2d.cpp:19 buffer) */
 #define ADSG_KFIELD 3 /* coefficients of a diffusion field k(x) on the same basis */
 #define ADSG_SOURCE 4 /* coefficients of a source f(x) on the same basis */
+#define ADSG_GRAD_K 5 /* accumulated dL/dk of adsg_adjoint_step (FP64 in both precisions) */
 
 typedef struct adsg_sim adsg_sim;
 
@@ -114,6 +115,10 @@ int adsg_set_vector(adsg_sim* h, int which, const double* host);
 int adsg_get_vector(adsg_sim* h, int which, double* host);
 /* device pointer of a vector (double* or float* by precision), for zero-copy interop */
 void* adsg_device_ptr(adsg_sim* h, int which);
+/* the same copies from/to FP64 device memory of the caller (e.g. a framework's tensors);
+ * they complete before returning. */
+int adsg_set_vector_device(adsg_sim* h, int which, const double* dev);
+int adsg_get_vector_device(adsg_sim* h, int which, double* dev);
 
 /* before_step's swap(u, u_prev) (validation.hpp:73-76). */
 int adsg_swap(adsg_sim* h);
@@ -154,6 +159,16 @@ int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags);
 #define ADSG_STEP_REFINE 2
 int adsg_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots, int flags);
 
+/* One step of the adjoint of adsg_step (explicit step, FP64 handles) for a form linear in u
+ * (reaction = 0): with u holding lam(n+1) and u_prev holding the forward state u(n) of that
+ * step, for u(n+1) = S r(u(n)), S = M^-1 Z (slot-0 solve, Z the boundary zeroing of
+ * ADSG_STEP_DIRICHLET) and r(u) = R(k) u + c:
+ *   w = Z M^-T lam(n+1);  lam(n) = R(k) w  (R symmetric);
+ *   if use_kfield: grad_k += d/dk [w^T R(k) u(n)] = -dt sum_e sum_q w_q J_e B_b sum_d grad_d
+ *   (d_d u)(d_d w)  (added into ADSG_GRAD_K, which the caller sets, e.g. to zeros, first).
+ * buffer is overwritten. No counterpart in the reference (it has no adjoint code). */
+int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags);
+
 /* nsteps of the adjoint of adsg_step_delta with the same form, slots and flags, applied to
  * u in place (no swap; u_prev and buffer are overwritten): for the step
  * u' = Z (u + A^-1 Z r(u)) with r(u) = R u + c linear (reaction = 0) and Z the boundary
@@ -166,12 +181,14 @@ int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, 
 
 int adsg_synchronize(adsg_sim* h);
 
-/* Device time per phase of nsteps eagerly launched steps (same sequence as adsg_step),
- * measured with HIP events on the handle's stream -- the counterpart of the reference's
- * StatTimer around compute_rhs (examples/scalability/test3d.hpp:67-101). Averages per step
- * in milliseconds: ms[0] rhs assembly (zero fill + colour passes), ms[1] boundary zeroing,
- * ms[2], ms[3], ms[4] line sweeps in x, y, z, ms[5] the whole step. */
-int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags, double* ms);
+/* Device time per phase of nsteps eagerly launched steps -- adsg_step's sequence when
+ * slots is NULL, adsg_step_delta's with these slots otherwise -- measured with HIP events on
+ * the handle's stream: the counterpart of the reference's StatTimer around compute_rhs
+ * (examples/scalability/test3d.hpp:67-101). Averages per step in milliseconds: ms[0] the
+ * element-loop kernels (2^dim colour launches), ms[1] boundary handling, ms[2], ms[3],
+ * ms[4] line sweeps in x, y, z, ms[5] the whole step, ms[6] the zero fill of the rhs. */
+int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots, int flags,
+                     double* ms);
 
 /* Kernel tuning (no counterpart in the reference, whose CPU loops have no launch shape):
  * ADSG_TUNE_RHS_VARIANT selects the element-kernel variant (0: element coefficients held in
@@ -179,13 +196,14 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int fla
  * ADSG_TUNE_SWEEP_VARIANT selects the line sweep (0: lines cut into segments solved by
  * separate waves and joined through FP64-computed influence coefficients, where the factors
  * are unpivoted and the lines are 2..16 segments long; 1: one lane per whole line).
- * adsg_autotune times every variant on the handle's current state with the given form and
- * keeps the fastest of each; the state vectors are restored afterwards. */
+ * adsg_autotune times every variant on the handle's current state with the given form
+ * (explicit steps, or delta steps with slots) and keeps the fastest of each; the state
+ * vectors are restored afterwards. */
 #define ADSG_TUNE_RHS_VARIANT 0
 #define ADSG_TUNE_SWEEP_VARIANT 1
 int adsg_set_tuning(adsg_sim* h, int key, int value);
 int adsg_get_tuning(const adsg_sim* h, int key);
-int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, int flags);
+int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags);
 
 /* lin::factorize + lin::solve_with_factorized (band_solve.hpp:16-31) on the host in
  * FP64, for callers that hold their own band matrices: factor ab (n x n, kl, ku, full

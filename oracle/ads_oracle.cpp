@@ -889,6 +889,65 @@ double orc_sim_l2norm_sq(const orc_sim* s, const double* u) {
     return E;
 }
 
+// Derivative of w^T rhs(u; k) with respect to the coefficients k_b of a B-spline k field,
+// for the form's gradient term -dt * k * sum_d grad_d u_d v_d (validation.hpp:110-115 with
+// k as in flow.hpp:73-100): out_b += -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d
+// (d_d u)(x_q) (d_d w)(x_q). No counterpart in the reference (it has no adjoint code).
+void orc_sim_kgrad(const orc_sim* s, const orc_rhs_form* f, const double* u, const double* w,
+                   double* out) {
+    const long n = element_count(s);
+    int e[3];
+    const orc_dim& X = s->d[0];
+    const orc_dim& Y = s->d[1];
+    const orc_dim& Z = s->d[2];
+    const int dim = s->dim;
+    const int mx = X.p + 1, my = Y.p + 1, mz = dim == 3 ? Z.p + 1 : 1;
+    for (long l = 0; l < n; ++l) {
+        element_index(s, l, e);
+        double J = X.J[e[0]] * Y.J[e[1]];
+        if (dim == 3) J = J * Z.J[e[2]];
+        const int qz_n = dim == 3 ? Z.q : 1;
+        auto gidx = [&](int bx, int by, int bz) {
+            return (X.first[e[0]] + bx) +
+                   X.n * ((Y.first[e[1]] + by) +
+                          static_cast<long>(Y.n) * ((dim == 3 ? Z.first[e[2]] : 0) + bz));
+        };
+        for (int qx = 0; qx < X.q; ++qx)
+            for (int qy = 0; qy < Y.q; ++qy)
+                for (int qz = 0; qz < qz_n; ++qz) {
+                    double wq = X.w[qx] * Y.w[qy];
+                    if (dim == 3) wq = wq * Z.w[qz];
+                    const double* tx = tab(X, e[0], qx);
+                    const double* ty = tab(Y, e[1], qy);
+                    const double* tz = dim == 3 ? tab(Z, e[2], qz) : nullptr;
+                    fval gu, gw;
+                    for (int bx = 0; bx < mx; ++bx)
+                        for (int by = 0; by < my; ++by)
+                            for (int bz = 0; bz < mz; ++bz) {
+                                const long gi = gidx(bx, by, bz);
+                                fval B = dim == 3 ? eval_basis3(tx, ty, tz, mx, my, mz, bx, by, bz)
+                                                  : eval_basis2(tx, ty, mx, my, bx, by);
+                                gu.dx += u[gi] * B.dx;
+                                gu.dy += u[gi] * B.dy;
+                                gu.dz += u[gi] * B.dz;
+                                gw.dx += w[gi] * B.dx;
+                                gw.dy += w[gi] * B.dy;
+                                gw.dz += w[gi] * B.dz;
+                            }
+                    double g = f->grad[0] * gu.dx * gw.dx + f->grad[1] * gu.dy * gw.dy;
+                    if (dim == 3) g += f->grad[2] * gu.dz * gw.dz;
+                    g *= -f->dt * wq * J;
+                    for (int bx = 0; bx < mx; ++bx)
+                        for (int by = 0; by < my; ++by)
+                            for (int bz = 0; bz < mz; ++bz) {
+                                fval B = dim == 3 ? eval_basis3(tx, ty, tz, mx, my, mz, bx, by, bz)
+                                                  : eval_basis2(tx, ty, mx, my, bx, by);
+                                out[gidx(bx, by, bz)] += g * B.val;
+                            }
+                }
+    }
+}
+
 // Greville abscissae xi_i = (t_{i+1} + ... + t_{i+p}) / p
 void orc_greville(const double* knot, int knot_size, int p, double* out) {
     int n = knot_size - p - 1;

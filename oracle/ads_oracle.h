@@ -119,6 +119,10 @@ void orc_sim_compute_rhs_threaded(const orc_sim* s, const orc_rhs_form* f, const
                                   double* rhs, int threads);
 
 /* ---- initial states of the benchmark problems ---- */
+/* d/dk_b of w^T rhs(u; k) for the gradient term of form (-dt k grad u . grad v), added
+ * into out (no counterpart in the reference; the k-field adjoint). */
+void orc_sim_kgrad(const orc_sim* s, const orc_rhs_form* f, const double* u, const double* w,
+                   double* out);
 void orc_greville(const double* knot, int knot_size, int p, double* out);
 void orc_init_gaussians(const orc_sim* s, unsigned long long seed, int count, double sigma_lo,
                         double sigma_hi, double centre_lo, double centre_hi, double noise,

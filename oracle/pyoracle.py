@@ -58,6 +58,7 @@ def lib():
         L.orc_sim_solve.argtypes = [C.c_void_p, _dp, _ip]
         L.orc_sim_solve_transposed.argtypes = [C.c_void_p, _dp, _ip]
         L.orc_sim_tables.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _ip]
+        L.orc_sim_kgrad.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp]
         L.orc_sim_l2norm_sq.argtypes = [C.c_void_p, _dp]
         L.orc_sim_l2norm_sq.restype = C.c_double
         L.orc_init_gaussians.argtypes = [C.c_void_p, C.c_ulonglong, C.c_int, C.c_double,
@@ -200,6 +201,14 @@ class Sim:
         else:
             lib().orc_sim_compute_rhs(self.h, C.byref(form), _d(u_prev), _d(rhs))
         return rhs
+
+    def kgrad(self, u, w, form):
+        """d/dk of w^T rhs(u; k) for the gradient term of form (k-field adjoint)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        out = self.zeros()
+        lib().orc_sim_kgrad(self.h, C.byref(form), _d(u), _d(w), _d(out))
+        return out
 
     def zero_boundary(self, v):
         lib().orc_sim_zero_boundary(self.h, _d(v))

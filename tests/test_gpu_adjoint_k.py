@@ -1,0 +1,163 @@
+"""GPU parity of the explicit-step adjoint with respect to a B-spline diffusion field k
+(u_t = div(k grad u), explicit ADS step with Dirichlet boundary as in validation.hpp),
+through the C ABI against the oracle's discrete adjoint, and the full gradient of
+L = ||u(T) - u_target||^2 against central finite differences (h = 1e-6).
+
+Bars: one adjoint step <= 1e-12 relative L2 (lam and dL/dk); the gradient through all
+steps <= 1e-10 against the oracle's discrete adjoint; directional derivatives <= 1e-5
+against central differences.
+"""
+import numpy as np
+import pytest
+
+import adsgpu as ag
+import pyoracle as po
+from adsgpu import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _kfield(p, ne, seed):
+    """k = exp(g), g a smooth seeded field scaled so that k lies in [0.5, 2]."""
+    g = wl.sine_source([p] * 2, list(ne), seed=seed)
+    g = g / max(np.abs(g).max(), 1e-300) * np.log(2.0)
+    return np.exp(g)
+
+
+def _pair(p, ne, dirichlet):
+    g = ag.Simulation(2, [p, p], list(ne))
+    o = po.Sim(2, [p, p], list(ne))
+    if dirichlet:
+        g.fix_all_boundaries()
+        o.fix_all_boundaries()
+    assert g.prepare_matrices() == 0 and o.prepare_matrices() == 0
+    return g, o
+
+
+def _oracle_forward(o, u, k, dt, steps, dirichlet):
+    traj = [u]
+    form = po.heat_form(dt, 2, kfield=k)
+    for _ in range(steps):
+        r = o.compute_rhs(traj[-1], form)
+        if dirichlet:
+            o.zero_boundary(r)
+        traj.append(o.solve(r))
+    return traj
+
+
+def _oracle_adjoint(o, traj, lam, k, dt, dirichlet):
+    """dL/dk and dL/du0 by the discrete adjoint of _oracle_forward, given lam = dL/du(T)."""
+    form = po.heat_form(dt, 2, kfield=k)
+    dk = np.zeros_like(lam)
+    for n in range(len(traj) - 2, -1, -1):
+        w = lam.copy()
+        o.solve(w, transposed=True)
+        if dirichlet:
+            o.zero_boundary(w)
+        dk += o.kgrad(traj[n], w, form)
+        lam = o.compute_rhs(w, form)
+    return dk, lam
+
+
+@pytest.mark.parametrize("dirichlet", [True, False])
+def test_adjoint_step_matches_oracle(dirichlet):
+    p, ne, dt = 2, (40, 36), 1e-5
+    _log(f"case: explicit adjoint step 2D p=2 {ne} dirichlet={dirichlet}")
+    g, o = _pair(p, ne, dirichlet)
+    k = _kfield(p, ne, seed=31)
+    rng = np.random.default_rng(5)
+    un = rng.standard_normal(g.size)
+    lam = rng.standard_normal(g.size)
+    g.set(ag.KFIELD, k)
+    g.set(ag.GRAD_K, np.zeros(g.size))
+    g.set(ag.U, lam)
+    g.set(ag.U_PREV, un)
+    flags = ag.STEP_DIRICHLET if dirichlet else 0
+    g.adjoint_step(ag.rhs_form(dt, kfield=True), flags)
+    dk, lam0 = _oracle_adjoint(o, [un, None], lam, k, dt, dirichlet)
+    e_lam = po.rel_l2(g.get(ag.U), lam0)
+    e_dk = po.rel_l2(g.get(ag.GRAD_K), dk)
+    _log(f"  lam {e_lam:.3e}  dk {e_dk:.3e}")
+    assert e_lam <= 1e-12 and e_dk <= 1e-12
+
+
+def _gpu_forward(g, k, u0, dt, steps, flags):
+    g.set(ag.KFIELD, k)
+    g.set(ag.U, u0)
+    g.step(steps, ag.rhs_form(dt, kfield=True), flags)
+    return g.get(ag.U)
+
+
+def _gpu_gradient(g, k, u0, ut, dt, steps, every, flags):
+    """dL/dk, dL/du0 with checkpoints every `every` steps (states recomputed per segment)."""
+    form = ag.rhs_form(dt, kfield=True)
+    g.set(ag.KFIELD, k)
+    g.set(ag.U, u0)
+    checkpoints = [u0.copy()]
+    for s in range(steps):
+        g.step(1, form, flags)
+        if (s + 1) % every == 0 and s + 1 < steps:
+            checkpoints.append(g.get(ag.U))
+    uN = g.get(ag.U)
+    lam = 2.0 * (uN - ut)
+    g.set(ag.GRAD_K, np.zeros(g.size))
+    for c in range(len(checkpoints) - 1, -1, -1):
+        start = c * every
+        stop = min(steps, start + every)
+        seg = [checkpoints[c]]
+        g.set(ag.U, checkpoints[c])
+        for _ in range(start, stop - 1):
+            g.step(1, form, flags)
+            seg.append(g.get(ag.U))
+        for n in range(stop - 1, start - 1, -1):
+            g.set(ag.U, lam)
+            g.set(ag.U_PREV, seg[n - start])
+            g.adjoint_step(form, flags)
+            lam = g.get(ag.U)
+    return g.get(ag.GRAD_K), lam, uN
+
+
+def test_gradient_through_steps_matches_oracle_and_finite_differences():
+    p, ne, dt, steps, every = 2, (40, 36), 1e-5, 30, 7
+    _log(f"case: dL/dk through {steps} steps, checkpoints every {every}")
+    g, o = _pair(p, ne, True)
+    flags = ag.STEP_DIRICHLET
+    k = _kfield(p, ne, seed=41)
+    kt = _kfield(p, ne, seed=42)
+    u0 = wl.gaussians([p, p], list(ne), seed=43, count=2, sigma=(0.1, 0.15), noise=0.0)
+    o.zero_boundary(u0)
+    ut = _gpu_forward(g, kt, u0, dt, steps, flags)
+    dk, du0, uN = _gpu_gradient(g, k, u0, ut, dt, steps, every, flags)
+    traj = _oracle_forward(o, u0, k, dt, steps, True)
+    assert po.rel_l2(uN, traj[-1]) <= 1e-12
+    dk_o, du0_o = _oracle_adjoint(o, traj, 2.0 * (traj[-1] - ut), k, dt, True)
+    _log(f"  dk vs oracle {po.rel_l2(dk, dk_o):.3e}  du0 vs oracle {po.rel_l2(du0, du0_o):.3e}")
+    assert po.rel_l2(dk, dk_o) <= 1e-10 and po.rel_l2(du0, du0_o) <= 1e-10
+
+    def loss(kk):
+        v = _gpu_forward(g, kk, u0, dt, steps, flags)
+        return float((v - ut) @ (v - ut))
+
+    rng = np.random.default_rng(44)
+    h = 1e-6
+    for j in range(4):
+        d = rng.standard_normal(g.size)
+        d /= np.linalg.norm(d)
+        fd = (loss(k + h * d) - loss(k - h * d)) / (2 * h)
+        ad = float(dk @ d)
+        rel = abs(fd - ad) / abs(ad)
+        _log(f"  direction {j}: adjoint {ad:.9e} fd {fd:.9e} rel {rel:.2e}")
+        assert rel <= 1e-5
+
+
+def test_adjoint_step_rejects_bad_state():
+    g, _ = _pair(2, (32, 32), False)
+    g.set(ag.U_PREV, np.zeros(g.size))
+    with pytest.raises(ag.AdsgError):  # GRAD_K not set
+        g.adjoint_step(ag.rhs_form(1e-5, kfield=True))
+    with pytest.raises(ag.AdsgError):  # nonlinear form
+        g.adjoint_step(ag.rhs_form(1e-5, reaction=1.0))

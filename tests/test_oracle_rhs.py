@@ -199,3 +199,23 @@ def test_heat_amplification_of_a_mode():
     rhs = sim.compute_rhs(c, po.heat_form(dt, 2))
     sim.solve(rhs)
     assert po.rel_l2(rhs, (1 - 2 * dt * lam[5]) * c) < 1e-12
+
+
+@pytest.mark.parametrize("dim,p,ne", [(2, 2, (3, 4)), (3, 2, (2, 3, 2)), (2, 3, (3, 2))])
+def test_kgrad_equals_rhs_with_unit_k_fields(dim, p, ne):
+    # w^T rhs(u; k) is linear in k, so d/dk_b = w^T rhs(u; k = e_b) with the mass term off
+    o = po.Sim(dim, [p] * dim, list(ne))
+    rng = np.random.default_rng(40 + dim)
+    u = rng.standard_normal(o.size)
+    w = rng.standard_normal(o.size)
+    grad = (1.0, 0.7, 1.3)
+    g = o.kgrad(u, w, po.heat_form(0.25, dim, grad=grad))
+    ref = np.zeros(o.size)
+    for b in range(o.size):
+        e = np.zeros(o.size)
+        e[b] = 1.0
+        f = po.heat_form(0.25, dim, grad=grad, kfield=e)
+        f.mass = 0.0
+        ref[b] = w @ o.compute_rhs(u, f)
+    assert np.allclose(g, ref, rtol=1e-12, atol=1e-14)
+    assert np.abs(g).max() > 1e-3
