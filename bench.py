@@ -141,6 +141,50 @@ def implicit_block(ag, np, ne, steps, warmup):
     return res
 
 
+def ensemble_block(ag, np, count, steps, dt):
+    """Batched 2D reaction-diffusion ensemble: `count` problems of uneven size and degree,
+    `steps` explicit steps from one captured graph; FP32 (increment scheme) and FP64."""
+    from adsgpu import workloads as wl
+
+    probs = wl.ensemble(seed=2028, count=count)
+    shape = [(q["p"], q["ne"]) for q in probs]
+    forms = [ag.rhs_form(dt, grad=(q["eps"], q["eps"], 0.0), reaction=1.0) for q in probs]
+    u0 = [wl.smooth_unit(q["p"], q["ne"], q["seed"]) for q in probs]
+    dofs = sum(u.size for u in u0)
+    elems = sum(q["ne"] ** 2 for q in probs)
+    flops = sum(f_elem(2, q["p"]) * q["ne"] ** 2 for q in probs)
+    out, res = {}, {}
+    for name, prec in (("fp32", ag.FP32), ("fp64", ag.FP64)):
+        b = ag.Batch(shape, precision=prec)
+        b.prepare()
+        b.set_forms(forms)
+        for i, u in enumerate(u0):
+            b.set(i, u)
+        tile, tile_ms = b.autotune()
+        b.step(steps)  # capture, instantiate and warm the graph of `steps` steps
+        b.synchronize()
+        for i, u in enumerate(u0):
+            b.set(i, u)
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.synchronize()
+        t = time.perf_counter() - t0
+        out[name] = [b.get(i) for i in range(len(probs))]
+        b.close()
+        res[name] = {"value": dofs * steps / t, "ms_per_step": 1e3 * t / steps,
+                     "tile_shape": ["16x16", "8x8"][tile],
+                     "autotune_ms_per_step": list(map(float, tile_ms)),
+                     "element_tflops": flops * steps / t / 1e12}
+    err = max(float(np.linalg.norm(a - c) / np.linalg.norm(c)) for a, c in zip(out["fp32"], out["fp64"]))
+    res["fp32"]["speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"]
+    res["fp32"][f"max_rel_l2_vs_fp64_after_{steps}_steps"] = err
+    res["workload"] = (f"{count} 2D reaction-diffusion problems u_t = eps Lap u + u(1-u), "
+                       f"elements per side in [32, 256], p in {{1,2,3}}, eps log-uniform in "
+                       f"[1e-4, 1e-2], dt={dt}, {steps} steps in one graph launch, natural BC; "
+                       f"{dofs} DOFs, {elems} elements in total")
+    return res
+
+
 def f_elem_source(d, p):
     """Flops per element of the delta-form integrand dt (f v - grad u . grad v), 3D: the
     forward tree needs 8 one-dimensional contractions for grad u (value and derivative in x;
@@ -182,6 +226,9 @@ def main():
                     help="skip the 3D p=3 implicit / adjoint block")
     ap.add_argument("--implicit-elements", type=int, default=192)
     ap.add_argument("--implicit-steps", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble block")
+    ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain D2D copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -283,6 +330,10 @@ def main():
     if not args.no_implicit:
         result.setdefault("forms", {})["implicit_p3"] = implicit_block(
             ag, np, args.implicit_elements, args.implicit_steps, 2)
+
+    if not args.no_batch:
+        result.setdefault("forms", {})["batched_ensemble"] = ensemble_block(
+            ag, np, args.batch_problems, args.batch_steps, 2e-5)
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

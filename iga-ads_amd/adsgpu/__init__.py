@@ -63,11 +63,16 @@ EXPORTS = [
     "adsg_zero_boundary", "adsg_ads_solve", "adsg_ads_solve_transposed", "adsg_step",
     "adsg_step_delta", "adsg_adjoint_step", "adsg_adjoint_step_delta", "adsg_synchronize",
     "adsg_set_vector_device", "adsg_get_vector_device",
+    "adsg_batch_create", "adsg_batch_destroy", "adsg_batch_last_error", "adsg_batch_count",
+    "adsg_batch_size", "adsg_batch_dofs", "adsg_batch_prepare", "adsg_batch_set_forms",
+    "adsg_batch_set_vector", "adsg_batch_get_vector", "adsg_batch_step", "adsg_batch_synchronize",
+    "adsg_batch_set_tuning", "adsg_batch_get_tuning", "adsg_batch_autotune",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
     "adsg_autotune",
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
+TUNE_BATCH_TILE = 2
 
 
 def lib():
@@ -102,6 +107,24 @@ def lib():
         L.adsg_step_delta.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), _ip, C.c_int]
         L.adsg_adjoint_step_delta.argtypes = [C.c_void_p, C.c_int, C.POINTER(RhsForm), _ip, C.c_int]
         L.adsg_adjoint_step.argtypes = [C.c_void_p, C.POINTER(RhsForm), C.c_int]
+        L.adsg_batch_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]
+        L.adsg_batch_destroy.argtypes = [C.c_void_p]
+        L.adsg_batch_destroy.restype = None
+        L.adsg_batch_last_error.argtypes = [C.c_void_p]
+        L.adsg_batch_last_error.restype = C.c_char_p
+        L.adsg_batch_count.argtypes = [C.c_void_p]
+        L.adsg_batch_size.argtypes = [C.c_void_p, C.c_int]
+        L.adsg_batch_size.restype = C.c_long
+        L.adsg_batch_dofs.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.adsg_batch_prepare.argtypes = [C.c_void_p]
+        L.adsg_batch_set_forms.argtypes = [C.c_void_p, C.POINTER(RhsForm)]
+        L.adsg_batch_set_vector.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.adsg_batch_get_vector.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.adsg_batch_step.argtypes = [C.c_void_p, C.c_int]
+        L.adsg_batch_synchronize.argtypes = [C.c_void_p]
+        L.adsg_batch_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.adsg_batch_get_tuning.argtypes = [C.c_void_p, C.c_int]
+        L.adsg_batch_autotune.argtypes = [C.c_void_p, _dp]
         L.adsg_set_vector_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_get_vector_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_synchronize.argtypes = [C.c_void_p]
@@ -274,3 +297,76 @@ def heat_k(sim, k, u0, steps, every, dt, dirichlet=True):
     from . import adsg_torch  # needs torch; built beside libadsgpu.so
 
     return adsg_torch.heat_k(k, u0, sim.h.value, steps, every, dt, dirichlet)
+
+
+class Batch:
+    """Many independent 2D problems (degree 1..3, natural boundaries) advanced together with
+    grouped launches and one captured graph per step count (adsg_batch_*)."""
+
+    def __init__(self, problems, precision=FP32):
+        """problems: list of (p, elements) pairs, square meshes on [0,1]^2."""
+        L = lib()
+        cfgs = (Config * len(problems))()
+        for c, (p, ne) in zip(cfgs, problems):
+            c.dim = 2
+            for dc in (c.x, c.y):
+                dc.p = p
+                dc.elements = ne
+                dc.a, dc.b = 0.0, 1.0
+                dc.quad_order = 0
+                dc.repeated_nodes = 0
+            c.derivatives = 1
+            c.precision = precision
+        h = C.c_void_p()
+        rc = L.adsg_batch_create(cfgs, len(problems), C.byref(h))
+        if rc != OK:
+            raise AdsgError(f"adsg_batch_create: {rc}: {L.adsg_batch_last_error(None).decode()}")
+        self.h = h
+        self.count = len(problems)
+        self.sizes = [int(L.adsg_batch_size(h, i)) for i in range(self.count)]
+
+    def _check(self, rc, what):
+        if rc != OK:
+            raise AdsgError(f"{what}: {rc}: {lib().adsg_batch_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().adsg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def prepare(self):
+        self._check(lib().adsg_batch_prepare(self.h), "batch_prepare")
+
+    def set_forms(self, forms):
+        arr = (RhsForm * self.count)(*forms)
+        self._check(lib().adsg_batch_set_forms(self.h, arr), "batch_set_forms")
+
+    def set(self, i, v, which=U):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == self.sizes[i]
+        self._check(lib().adsg_batch_set_vector(self.h, i, which, v.ctypes.data_as(_dp)), "batch_set_vector")
+
+    def get(self, i, which=U):
+        v = np.empty(self.sizes[i])
+        self._check(lib().adsg_batch_get_vector(self.h, i, which, v.ctypes.data_as(_dp)), "batch_get_vector")
+        return v
+
+    def step(self, n):
+        self._check(lib().adsg_batch_step(self.h, n), "batch_step")
+
+    def synchronize(self):
+        self._check(lib().adsg_batch_synchronize(self.h), "batch_synchronize")
+
+    def set_tuning(self, value):
+        self._check(lib().adsg_batch_set_tuning(self.h, TUNE_BATCH_TILE, value), "batch_set_tuning")
+
+    def get_tuning(self):
+        return lib().adsg_batch_get_tuning(self.h, TUNE_BATCH_TILE)
+
+    def autotune(self):
+        ms = np.zeros(2)
+        self._check(lib().adsg_batch_autotune(self.h, ms.ctypes.data_as(_dp)), "batch_autotune")
+        return self.get_tuning(), ms

@@ -104,3 +104,28 @@ def sine_source(p, elements, seed, terms=4):
             prod = prod * np.sin(k * np.pi * g[d]).reshape(axis)
         f += prod
     return f.reshape(-1)
+
+
+def ensemble(seed, count, ne_range=(32, 256), degrees=(1, 2, 3), eps_range=(1e-4, 1e-2)):
+    """Problems of the batched reaction-diffusion ensemble u_t = eps Lap u + u (1 - u):
+    per problem (draw order) elements per side uniform in ne_range, degree uniform in
+    degrees, eps log-uniform in eps_range, and a seed for its initial state. Returns a list
+    of dicts {p, ne, eps, seed}."""
+    st = Stream(seed)
+    out = []
+    for _ in range(count):
+        ne = ne_range[0] + min(int((ne_range[1] - ne_range[0] + 1) * st.uniform()), ne_range[1] - ne_range[0])
+        p = degrees[min(int(len(degrees) * st.uniform()), len(degrees) - 1)]
+        lo, hi = np.log(eps_range[0]), np.log(eps_range[1])
+        eps = float(np.exp(lo + (hi - lo) * st.uniform()))
+        s = int(st.uniform() * 2**53)
+        out.append({"p": p, "ne": ne, "eps": eps, "seed": s})
+    return out
+
+
+def smooth_unit(p, ne, seed):
+    """Smooth initial data in [0, 1]: three seeded Gaussians (sigma in [0.1, 0.3]) rescaled
+    to span [0, 1]."""
+    u = gaussians([p, p], [ne, ne], seed, 3, sigma=(0.1, 0.3), centre=(0.2, 0.8), noise=0.0)
+    lo, hi = u.min(), u.max()
+    return (u - lo) / (hi - lo) if hi > lo else np.zeros_like(u)

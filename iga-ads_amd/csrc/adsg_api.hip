@@ -9,70 +9,16 @@
 #include <tuple>
 #include <vector>
 
-#include "../../include/adsgpu.h"
-#include "adsg_kernels.hpp"
-#include "adsg_setup.hpp"
+#include "adsg_sim.hpp"
 
 namespace {
 
 thread_local std::string g_create_error;
 
-struct dev_slot {
-    bool set = false;
-    bool factored = false;
-    int kl = 0, ku = 0;
-    std::vector<double> ab;  // unfactored band, column size 2kl+ku+1
-    adsg::line_factors lf;
-    // device factors in the precision of the sweeps
-    void* lower = nullptr;
-    void* upper = nullptr;
-    void* rdiag = nullptr;
-    int* poff = nullptr;
-    // segmented sweep (unpivoted factors only)
-    adsg::line_segments seg;
-    void* gf = nullptr;
-    void* hb = nullptr;
-    void* gft = nullptr;  // transposed solve
-    void* hbt = nullptr;
-    double* arow = nullptr;  // FP64 rows of the unfactored matrix, for refinement
-};
-
-struct dev_dir {
-    adsg::direction tab;
-    std::vector<dev_slot> slots;
-    void* val = nullptr;  // [elements][nq][p+1] in the step precision
-    void* der = nullptr;
-    void* wj = nullptr;
-    void* rows = nullptr;  // FP32 mode: slot-0 matrix as n rows of 2p+1 entries
-};
-
 }  // namespace
 
-struct adsg_sim {
-    int dim = 0;
-    int precision = ADSG_FP64;
-    int step_count = 0;
-    double dt = 0;
-    dev_dir dirs[3];
-    long N = 0;
-    hipStream_t stream = nullptr;
-    // u and u_prev are FP64 in both precisions; buffer, kfield and source are in the step
-    // precision. In FP32 mode shadow[0]/shadow[1] are FP32 copies of u/u_prev (they swap
-    // with them) and a step is computed as an increment: compute_rhs assembles
-    // rhs - M u_prev in FP32 into buffer, ads_solve turns it into M^-1 (rhs - M u_prev) with
-    // FP32 sweeps on FP64-computed factors and adds it to u_prev in FP64.
-    void* vec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    void* shadow[2] = {nullptr, nullptr};
-    bool pending = false;            // FP32: buffer holds an increment based on u_prev
-    bool pending_dirichlet = false;  // FP32: boundary rows of the increment were fixed
-    bool prepared = false;
-    int rhs_variant = 0;    // element-kernel tuning choice (adsg_set_tuning)
-    int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
-    std::string err;
-    hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
-    // captured step graphs keyed by (form, flags, u pointer, steps in graph)
-    std::map<std::tuple<std::string, int, void*, int>, hipGraphExec_t> graphs;
-};
+using adsg::dev_dir;
+using adsg::dev_slot;
 
 namespace {
 
@@ -556,11 +502,9 @@ int run_graphs(adsg_sim* h, step_kind kind, int nsteps, const adsg_rhs_form* for
 
 }  // namespace
 
-extern "C" {
-
-const char* adsg_last_error(const adsg_sim* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
-
-int adsg_create(const adsg_config* cfg, adsg_sim** out) {
+// the handle of adsg_create; with a shared stream (not owned) and without state vectors for
+// the problems of a batch, whose vectors live in the batch's arenas
+int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, adsg_sim** out) {
     if (!cfg || !out) {
         g_create_error = "adsg_create: null argument";
         return ADSG_EINVAL;
@@ -601,7 +545,10 @@ int adsg_create(const adsg_config* cfg, adsg_sim** out) {
         s0.ab = adsg::band_form(h->dirs[d].tab, 1.0, 0.0);  // gram_matrix_1d (dimension.cpp:17)
         h->dirs[d].slots.push_back(std::move(s0));
     }
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (shared) {
+        h->stream = shared;
+        h->owns_stream = false;
+    } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "adsg_create: hipStreamCreate failed (no HIP device?)";
         delete h;
         return ADSG_EHIP;
@@ -613,7 +560,7 @@ int adsg_create(const adsg_config* cfg, adsg_sim** out) {
             return ADSG_EHIP;
         }
     }
-    for (int v = ADSG_U; v <= ADSG_BUFFER; ++v) {
+    for (int v = ADSG_U; v <= ADSG_BUFFER && vectors; ++v) {
         if (hipMalloc(&h->vec[v], h->N * elt_of(h, v)) != hipSuccess) {
             g_create_error = "adsg_create: device allocation of the state vectors failed";
             adsg_destroy(h);
@@ -625,7 +572,7 @@ int adsg_create(const adsg_config* cfg, adsg_sim** out) {
             return ADSG_EHIP;
         }
     }
-    if (h->precision == ADSG_FP32) {
+    if (h->precision == ADSG_FP32 && vectors) {
         for (auto& sp : h->shadow) {
             if (hipMalloc(&sp, h->N * sizeof(float)) != hipSuccess ||
                 hipMemsetAsync(sp, 0, h->N * sizeof(float), h->stream) != hipSuccess) {
@@ -640,6 +587,12 @@ int adsg_create(const adsg_config* cfg, adsg_sim** out) {
     return ADSG_OK;
 }
 
+extern "C" {
+
+const char* adsg_last_error(const adsg_sim* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int adsg_create(const adsg_config* cfg, adsg_sim** out) { return adsg::create_sim(cfg, nullptr, true, out); }
+
 void adsg_destroy(adsg_sim* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -653,7 +606,7 @@ void adsg_destroy(adsg_sim* h) {
         if (v) (void)hipFree(v);
     for (auto& v : h->shadow)
         if (v) (void)hipFree(v);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
 

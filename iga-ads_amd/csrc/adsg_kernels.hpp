@@ -76,6 +76,36 @@ template <typename T>
 hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
                       const rhs_params<T>& prm, int variant, hipStream_t s);
 
+// Batched 2D element kernels: one item per problem (degree p, no k field, no source);
+// ntx2[c] tiles per tile row and first[c] the first block of the item in colour c's launch.
+template <typename T>
+struct rhs2_item {
+    const T* up = nullptr;
+    T* rhs = nullptr;
+    dev_tables<T> X, Y;
+    rhs_params<T> prm;
+    int ntx2[4] = {0, 0, 0, 0};
+    int first[4] = {0, 0, 0, 0};
+};
+// tiles of each colour for tile shape 0 (16 x 16 elements) or 1 (8 x 8)
+void rhs2_batch_tiles(int shape, int elements_x, int elements_y, int ntx2[4], int count[4]);
+template <typename T>
+hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int nitems, int colour,
+                             int blocks, hipStream_t s);
+
+// Batched line sweeps of unpivoted factors: one item per (problem, direction); 64 lines
+// per block (x lines staged through LDS, y lines one per lane), first = first block.
+template <typename T>
+struct sweep_item {
+    T* u = nullptr;
+    line_geometry g;
+    dev_factors<T> f;
+    int first = 0;
+};
+template <typename T>
+hipError_t launch_sweep_batch(int kl, int ku, const sweep_item<T>* items, int nitems, int blocks,
+                              hipStream_t s);
+
 // d/dk of w^T rhs(u; k) for the gradient term of prm, added into out:
 //   out_b += -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d (d_d u)(d_d w)
 template <typename T>

@@ -277,10 +277,12 @@ __global__ __launch_bounds__(TX* TY* TZ, CREG ? 1 : 2) void k_rhs3(
 // ------------------------------------------------------------------------------------
 // element-loop RHS, 2D
 // ------------------------------------------------------------------------------------
+// One tile of TX x TY elements (tile t of the given parity colour; ntx2 tiles of that
+// colour per tile row), added into rhs.
 template <typename T, int P, int TX, int TY, bool KF, bool SRC>
-__global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __restrict__ rhs,
-                                                 dev_tables<T> X, dev_tables<T> Y,
-                                                 rhs_params<T> prm, int colour, int ntx2) {
+__device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restrict__ rhs,
+                                          const dev_tables<T>& X, const dev_tables<T>& Y,
+                                          const rhs_params<T>& prm, int colour, int t, int ntx2) {
     constexpr int M = P + 1, NT = TX * TY;
     constexpr int BX = TX + P, BY = TY + P, NB = BX * BY;
     constexpr int WY = 64 / TX;  // element rows per wave
@@ -295,7 +297,6 @@ __global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __
     __shared__ double acc[NW * WR * BX];  // wave w owns DOF rows w*WY .. w*WY+WR-1
 
     const int tid = threadIdx.x;
-    const int t = blockIdx.x;
     const int tx = 2 * (t % ntx2) + (colour & 1);
     const int ty = 2 * (t / ntx2) + ((colour >> 1) & 1);
     const int ex0 = tx * TX, ey0 = ty * TY;
@@ -440,6 +441,32 @@ __global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __
     }
 }
 
+template <typename T, int P, int TX, int TY, bool KF, bool SRC>
+__global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __restrict__ rhs,
+                                                 dev_tables<T> X, dev_tables<T> Y,
+                                                 rhs_params<T> prm, int colour, int ntx2) {
+    rhs2_tile<T, P, TX, TY, KF, SRC>(up, rhs, X, Y, prm, colour, blockIdx.x, ntx2);
+}
+
+// Many 2D problems of one degree in one launch: block b belongs to the last item whose
+// first[colour] <= b (items are in launch order), and is tile b - first[colour] of it.
+template <typename T, int P, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_rhs2_batch(const rhs2_item<T>* __restrict__ items,
+                                                       int nitems, int colour) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (items[mid].first[colour] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const rhs2_item<T>& it = items[lo];
+    rhs2_tile<T, P, TX, TY, false, false>(it.up, it.rhs, it.X, it.Y, it.prm, colour,
+                                          b - it.first[colour], it.ntx2[colour]);
+}
+
 // tile shapes (elements) per dimension and degree: one lane per element
 template <int D, int P>
 struct tile;
@@ -541,6 +568,40 @@ hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<
     }
 }
 
+void rhs2_batch_tiles(int shape, int elements_x, int elements_y, int ntx2[4], int count[4]) {
+    const int TX = shape == 0 ? 16 : 8, TY = TX;
+    const int ntx = (elements_x + TX - 1) / TX, nty = (elements_y + TY - 1) / TY;
+    for (int c = 0; c < 4; ++c) {
+        const int cx = c & 1, cy = (c >> 1) & 1;
+        ntx2[c] = (ntx - cx + 1) / 2;
+        count[c] = ntx2[c] * ((nty - cy + 1) / 2);
+    }
+}
+
+template <typename T>
+hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int nitems, int colour,
+                             int blocks, hipStream_t s) {
+    if (blocks <= 0 || nitems <= 0) return hipSuccess;
+#define ADSG_RB(P_, TX_)                                                                       \
+    if (p == P_ && TX_ == (shape == 0 ? 16 : 8)) {                                             \
+        hipLaunchKernelGGL((k_rhs2_batch<T, P_, TX_, TX_>), dim3(blocks), dim3(TX_ * TX_), 0, s, \
+                           items, nitems, colour);                                              \
+        return hipGetLastError();                                                               \
+    }
+    ADSG_RB(1, 16)
+    ADSG_RB(2, 16)
+    ADSG_RB(3, 16)
+    ADSG_RB(1, 8)
+    ADSG_RB(2, 8)
+    ADSG_RB(3, 8)
+#undef ADSG_RB
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_rhs2_batch<double>(int, int, const rhs2_item<double>*, int, int, int,
+                                              hipStream_t);
+template hipError_t launch_rhs2_batch<float>(int, int, const rhs2_item<float>*, int, int, int,
+                                             hipStream_t);
 template hipError_t launch_rhs<double>(int, int, const double*, double*, const dev_tables<double>*,
                                        const rhs_params<double>&, int, hipStream_t);
 template hipError_t launch_rhs<float>(int, int, const float*, float*, const dev_tables<float>*,

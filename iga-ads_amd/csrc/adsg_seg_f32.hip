@@ -1,0 +1,7 @@
+// Segmented line sweeps, FP32 instantiation.
+#include "adsg_seg.inc"
+
+namespace adsg {
+template hipError_t launch_line_sweep_seg<float, float>(float*, const line_geometry&,
+                                                        const seg_factors<float>&, hipStream_t);
+}  // namespace adsg

@@ -1,0 +1,7 @@
+// Segmented line sweeps, FP64 instantiation.
+#include "adsg_seg.inc"
+
+namespace adsg {
+template hipError_t launch_line_sweep_seg<double, double>(double*, const line_geometry&,
+                                                          const seg_factors<double>&, hipStream_t);
+}  // namespace adsg

@@ -1,0 +1,80 @@
+// Internal state of an adsg_sim handle, shared by the C ABI and the batch code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/adsgpu.h"
+#include "adsg_kernels.hpp"
+#include "adsg_setup.hpp"
+
+namespace adsg {
+
+struct dev_slot {
+    bool set = false;
+    bool factored = false;
+    int kl = 0, ku = 0;
+    std::vector<double> ab;  // unfactored band, column size 2kl+ku+1
+    adsg::line_factors lf;
+    // device factors in the precision of the sweeps
+    void* lower = nullptr;
+    void* upper = nullptr;
+    void* rdiag = nullptr;
+    int* poff = nullptr;
+    // segmented sweep (unpivoted factors only)
+    adsg::line_segments seg;
+    void* gf = nullptr;
+    void* hb = nullptr;
+    void* gft = nullptr;  // transposed solve
+    void* hbt = nullptr;
+    double* arow = nullptr;  // FP64 rows of the unfactored matrix, for refinement
+};
+
+struct dev_dir {
+    adsg::direction tab;
+    std::vector<dev_slot> slots;
+    void* val = nullptr;  // [elements][nq][p+1] in the step precision
+    void* der = nullptr;
+    void* wj = nullptr;
+    void* rows = nullptr;  // FP32 mode: slot-0 matrix as n rows of 2p+1 entries
+};
+
+}  // namespace adsg
+
+struct adsg_sim {
+    int dim = 0;
+    int precision = ADSG_FP64;
+    int step_count = 0;
+    double dt = 0;
+    adsg::dev_dir dirs[3];
+    long N = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = true;
+    // u and u_prev are FP64 in both precisions; buffer, kfield and source are in the step
+    // precision. In FP32 mode shadow[0]/shadow[1] are FP32 copies of u/u_prev (they swap
+    // with them) and a step is computed as an increment: compute_rhs assembles
+    // rhs - M u_prev in FP32 into buffer, ads_solve turns it into M^-1 (rhs - M u_prev) with
+    // FP32 sweeps on FP64-computed factors and adds it to u_prev in FP64.
+    void* vec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* shadow[2] = {nullptr, nullptr};
+    bool pending = false;            // FP32: buffer holds an increment based on u_prev
+    bool pending_dirichlet = false;  // FP32: boundary rows of the increment were fixed
+    bool prepared = false;
+    int rhs_variant = 0;    // element-kernel tuning choice (adsg_set_tuning)
+    int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
+    std::string err;
+    hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
+    // captured step graphs keyed by (form, flags, u pointer, steps in graph)
+    std::map<std::tuple<std::string, int, void*, int>, hipGraphExec_t> graphs;
+};
+
+
+namespace adsg {
+// adsg_create with a shared stream (not owned by the handle) and, without `vectors`, no
+// state vectors (the problems of a batch keep theirs in the batch's arenas)
+int create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, adsg_sim** out);
+}  // namespace adsg

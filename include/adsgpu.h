@@ -205,6 +205,39 @@ int adsg_set_tuning(adsg_sim* h, int key, int value);
 int adsg_get_tuning(const adsg_sim* h, int key);
 int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags);
 
+/* ---- batches: many independent 2D problems advanced together --------------------------
+ * No counterpart in the reference, which runs one simulation per process; each problem is
+ * the explicit step of adsg_step (swap; compute_rhs(form); ads_solve with the Gram matrices,
+ * validation.hpp:73-81) with natural boundaries, its own mesh size and degree p = 1..3 (the
+ * same in x and y), and a form without k field or source (e.g. the reaction-diffusion form
+ * {mass 1, dt, grad eps, reaction 1}). The state of all problems lives in shared arenas;
+ * one step is a fixed sequence of grouped launches over all problems (element kernels per
+ * degree and colour, x and y sweeps per degree, one update), and nsteps steps are captured
+ * once into a HIP graph and replayed. FP32 batches use the increment scheme of ADSG_FP32. */
+typedef struct adsg_batch adsg_batch;
+int adsg_batch_create(const adsg_config* cfgs, int count, adsg_batch** out);
+void adsg_batch_destroy(adsg_batch* b);
+const char* adsg_batch_last_error(const adsg_batch* b); /* NULL: last create error */
+int adsg_batch_count(const adsg_batch* b);
+long adsg_batch_size(const adsg_batch* b, int problem);
+int adsg_batch_dofs(const adsg_batch* b, int problem, int dir);
+/* factor every problem's Gram matrices (FP64) and build the launch lists */
+int adsg_batch_prepare(adsg_batch* b);
+/* one form per problem (default: explicit heat, dt = 0) */
+int adsg_batch_set_forms(adsg_batch* b, const adsg_rhs_form* forms);
+/* ADSG_U or ADSG_U_PREV of one problem, FP64 host data */
+int adsg_batch_set_vector(adsg_batch* b, int problem, int which, const double* host);
+int adsg_batch_get_vector(adsg_batch* b, int problem, int which, double* host);
+int adsg_batch_step(adsg_batch* b, int nsteps);
+int adsg_batch_synchronize(adsg_batch* b);
+/* ADSG_TUNE_BATCH_TILE: element tile of the batched kernels, 0 = 16 x 16, 1 = 8 x 8.
+ * adsg_batch_autotune times one step with each (ms_out[0..1], may be NULL), keeps the
+ * faster and restores the state. */
+#define ADSG_TUNE_BATCH_TILE 2
+int adsg_batch_set_tuning(adsg_batch* b, int key, int value);
+int adsg_batch_get_tuning(const adsg_batch* b, int key);
+int adsg_batch_autotune(adsg_batch* b, double* ms_out);
+
 /* lin::factorize + lin::solve_with_factorized (band_solve.hpp:16-31) on the host in
  * FP64, for callers that hold their own band matrices: factor ab (n x n, kl, ku, full
  * buffer as above) in place, pivots 1-based like dgbtrf. Returns LAPACK info. */

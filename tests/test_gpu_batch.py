@@ -1,0 +1,116 @@
+"""GPU parity of batched 2D problems (grouped launches over many problems of uneven mesh
+size and degree, one captured graph per step count) against the oracle per problem and
+against the same problems run one by one.
+
+Workload shape: the reaction-diffusion ensemble u_t = eps Lap u + u (1 - u) with natural
+boundaries, explicit ADS steps (rhs = M u - dt eps K u + dt u (1 - u), then M^-1).
+Bars: FP64 batch <= 1e-12 relative L2 per problem against the oracle; FP32 batch <= 1e-6
+against the FP64 oracle; batch equals the single-problem path bitwise.
+"""
+import numpy as np
+import pytest
+
+import adsgpu as ag
+import pyoracle as po
+from adsgpu import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+DT = 2e-5
+PROBLEMS = [(1, 32), (2, 40), (3, 33), (2, 64), (1, 50), (3, 48), (2, 35), (1, 64), (3, 40),
+            (2, 32), (1, 37), (3, 36)]
+EPS = [1e-4, 3e-3, 1e-2, 5e-4, 2e-3, 1e-2, 7e-3, 1e-3, 4e-3, 2e-4, 6e-3, 8e-3]
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _forms():
+    return [ag.rhs_form(DT, grad=(e, e, 0.0), reaction=1.0) for e in EPS]
+
+
+def _u0(i):
+    p, ne = PROBLEMS[i]
+    return wl.smooth_unit(p, ne, seed=100 + i)
+
+
+def _batch(precision, shape=0):
+    b = ag.Batch(PROBLEMS, precision=precision)
+    b.prepare()
+    b.set_forms(_forms())
+    b.set_tuning(shape)
+    for i in range(len(PROBLEMS)):
+        b.set(i, _u0(i))
+    return b
+
+
+def _oracle(i, steps):
+    p, ne = PROBLEMS[i]
+    o = po.Sim(2, [p, p], [ne, ne])
+    assert o.prepare_matrices() == 0
+    f = po.heat_form(DT, 2, grad=(EPS[i], EPS[i], 0.0), reaction=1.0)
+    u = _u0(i)
+    for _ in range(steps):
+        u = o.solve(o.compute_rhs(u, f))
+    return u
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+def test_fp64_batch_matches_oracle_per_problem(shape):
+    _log(f"case: FP64 batch of {len(PROBLEMS)} problems, tile shape {shape}, 10 steps")
+    b = _batch(ag.FP64, shape)
+    b.step(10)
+    worst = 0.0
+    for i in range(len(PROBLEMS)):
+        worst = max(worst, po.rel_l2(b.get(i), _oracle(i, 10)))
+    _log(f"  worst rel L2 {worst:.3e}")
+    assert worst <= 1e-12
+    b.close()
+
+
+def test_fp32_batch_against_fp64_oracle():
+    _log("case: FP32 batch, 20 steps, against the FP64 oracle")
+    b = _batch(ag.FP32)
+    b.step(20)
+    worst = 0.0
+    for i in range(len(PROBLEMS)):
+        worst = max(worst, po.rel_l2(b.get(i), _oracle(i, 20)))
+    _log(f"  worst rel L2 {worst:.3e}")
+    assert worst <= 1e-6
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [ag.FP64, ag.FP32])
+def test_batch_equals_single_problem_path_bitwise(precision):
+    _log(f"case: batch vs one-by-one, precision {precision}")
+    b = _batch(precision, 0)
+    b.step(6)
+    for i in (0, 2, 5, 11):
+        p, ne = PROBLEMS[i]
+        g = ag.Simulation(2, [p, p], [ne, ne], precision=precision)
+        g.prepare_matrices()
+        g.set_tuning(ag.TUNE_SWEEP_VARIANT, 1)  # one lane per line, as the batch
+        g.set(ag.U, _u0(i))
+        g.step(6, _forms()[i])
+        assert np.array_equal(g.get(ag.U), b.get(i)), i
+        g.close()
+    b.close()
+
+
+def test_batch_graph_replay_and_autotune_keep_results():
+    _log("case: batch graph chunks and autotune")
+    runs = []
+    for chunks, tune in (([7], False), ([1, 2, 4], False), ([3, 4], True)):
+        b = _batch(ag.FP32, 0)
+        if tune:
+            shape, ms = b.autotune()
+            _log(f"  autotune: shape {shape}, ms per step {ms}")
+            b.set_tuning(0)
+        for n in chunks:
+            b.step(n)
+        runs.append([b.get(i) for i in range(len(PROBLEMS))])
+        b.close()
+    for r in runs[1:]:
+        for a, c in zip(runs[0], r):
+            assert np.array_equal(a, c)
