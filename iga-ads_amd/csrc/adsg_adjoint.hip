@@ -1,5 +1,6 @@
-// HIP kernels of the adjoint step (gfx950): the derivative of w^T rhs(u; k) with respect to
-// the B-spline coefficients of the diffusion field k. The reference has no adjoint code;
+// HIP kernels of the adjoint step and of the norm functionals (gfx950). The adjoint needs the
+// derivative of w^T rhs(u; k) with respect to the B-spline coefficients of the diffusion
+// field k. The reference has no adjoint code;
 // the form follows its variable-coefficient integrand (flow.hpp:73-100, validation.hpp:
 // 110-115):  d/dk_b = -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d (d_d u)(d_d w).
 //
@@ -119,7 +120,121 @@ hipError_t kgrad_dim(int dim, const T* u, const T* w, T* out, const dev_tables<T
     return hipErrorInvalidValue;
 }
 
+// Squared norm functionals (basic_simulation_2d.hpp:42-52, :291-316): per element and Gauss
+// point w J phi(u) with phi = u^2 (kind 0), |grad u|^2 (1) or both (2). One lane per element
+// (grid-stride), FP64 sums reduced in a fixed order: lanes -> block partial -> one total.
+template <typename TT, int DIM, int P>
+__global__ __launch_bounds__(256) void k_norm(const double* __restrict__ u, dev_tables<TT> tx,
+                                              dev_tables<TT> ty, dev_tables<TT> tz, int kind,
+                                              double* __restrict__ partial) {
+    constexpr int n = P + 1;
+    constexpr int NZ = DIM == 3 ? n : 1;
+    __shared__ double red[256];
+    const long E = static_cast<long>(tx.elements) * ty.elements * (DIM == 3 ? tz.elements : 1);
+    const long nx = tx.dofs, ny = ty.dofs;
+    double acc = 0;
+    for (long l = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; l < E;
+         l += static_cast<long>(gridDim.x) * blockDim.x) {
+        const int ex = static_cast<int>(l % tx.elements);
+        const long r = l / tx.elements;
+        const int ey = static_cast<int>(r % ty.elements), ez = static_cast<int>(r / ty.elements);
+        double U[NZ][n][n];
+        for (int c = 0; c < NZ; ++c)
+            for (int b = 0; b < n; ++b)
+                for (int a = 0; a < n; ++a)
+                    U[c][b][a] = u[(ex + a) + nx * ((ey + b) + ny * static_cast<long>(ez + c))];
+        const TT* vx = tx.val + static_cast<long>(ex) * n * n;
+        const TT* dx = tx.der + static_cast<long>(ex) * n * n;
+        const TT* vy = ty.val + static_cast<long>(ey) * n * n;
+        const TT* dy = ty.der + static_cast<long>(ey) * n * n;
+        const TT* vz = DIM == 3 ? tz.val + static_cast<long>(ez) * n * n : nullptr;
+        const TT* dz = DIM == 3 ? tz.der + static_cast<long>(ez) * n * n : nullptr;
+        for (int qz = 0; qz < NZ; ++qz)
+            for (int qy = 0; qy < n; ++qy)
+                for (int qx = 0; qx < n; ++qx) {
+                    double v = 0, gx = 0, gy = 0, gz = 0;
+                    for (int c = 0; c < NZ; ++c) {
+                        const double bz = DIM == 3 ? double(vz[qz * n + c]) : 1.0;
+                        const double bdz = DIM == 3 ? double(dz[qz * n + c]) : 0.0;
+                        for (int b = 0; b < n; ++b) {
+                            const double by = vy[qy * n + b], bdy = dy[qy * n + b];
+                            for (int a = 0; a < n; ++a) {
+                                const double bx = vx[qx * n + a], bdx = dx[qx * n + a];
+                                const double cu = U[c][b][a];
+                                v += cu * (bx * by * bz);
+                                gx += cu * (bdx * by * bz);
+                                gy += cu * (bx * bdy * bz);
+                                if (DIM == 3) gz += cu * (bx * by * bdz);
+                            }
+                        }
+                    }
+                    double w = double(tx.wj[ex * n + qx]) * double(ty.wj[ey * n + qy]);
+                    if (DIM == 3) w *= double(tz.wj[ez * n + qz]);
+                    const double g2 = gx * gx + gy * gy + gz * gz;
+                    acc += w * (kind == 0 ? v * v : kind == 1 ? g2 : v * v + g2);
+                }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partial, int count,
+                                                      double* __restrict__ out) {
+    __shared__ double red[256];
+    double acc = 0;
+    for (int i = threadIdx.x; i < count; i += 256) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+template <typename TT, int P>
+hipError_t norm_dim(int dim, const double* u, const dev_tables<TT>* d, int kind, double* partial,
+                    int blocks, double* out, hipStream_t s) {
+    const dev_tables<TT> none{};
+    if (dim == 2)
+        hipLaunchKernelGGL((k_norm<TT, 2, P>), dim3(blocks), dim3(256), 0, s, u, d[0], d[1], none,
+                           kind, partial);
+    else
+        hipLaunchKernelGGL((k_norm<TT, 3, P>), dim3(blocks), dim3(256), 0, s, u, d[0], d[1], d[2],
+                           kind, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, blocks, out);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+template <typename TT>
+hipError_t launch_norm(int dim, int p, const double* u, const dev_tables<TT>* dirs, int kind,
+                       double* partial, int blocks, double* out, hipStream_t s) {
+    if (blocks <= 0 || (dim != 2 && dim != 3)) return hipErrorInvalidValue;
+    switch (p) {
+        case 1:
+            return norm_dim<TT, 1>(dim, u, dirs, kind, partial, blocks, out, s);
+        case 2:
+            return norm_dim<TT, 2>(dim, u, dirs, kind, partial, blocks, out, s);
+        case 3:
+            return norm_dim<TT, 3>(dim, u, dirs, kind, partial, blocks, out, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_norm<double>(int, int, const double*, const dev_tables<double>*, int,
+                                        double*, int, double*, hipStream_t);
+template hipError_t launch_norm<float>(int, int, const double*, const dev_tables<float>*, int,
+                                       double*, int, double*, hipStream_t);
 
 template <typename T>
 hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,

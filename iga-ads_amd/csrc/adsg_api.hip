@@ -1,6 +1,7 @@
 // C ABI of the MI355X ADS step (declared in include/adsgpu.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -606,6 +607,7 @@ void adsg_destroy(adsg_sim* h) {
         if (v) (void)hipFree(v);
     for (auto& v : h->shadow)
         if (v) (void)hipFree(v);
+    if (h->norm_scratch) (void)hipFree(h->norm_scratch);
     if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -871,6 +873,43 @@ int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, 
     if (!h->vec[ADSG_BUFFER] || !h->vec[ADSG_U_PREV]) return fail(h, ADSG_ESTATE, "adjoint_step_delta: vectors not allocated");
     if (nsteps == 0) return ADSG_OK;
     return run_graphs(h, ADJOINT_DELTA_STEP, nsteps, form, flags, slots);
+}
+
+int adsg_norm(adsg_sim* h, int which, int kind, double* out) {
+    if (!h || !out || (which != ADSG_U && which != ADSG_U_PREV) || kind < 0 || kind > 2)
+        return fail(h, ADSG_EINVAL, "norm: bad argument (U or U_PREV; kind 0, 1 or 2)");
+    constexpr int kBlocks = 1024;
+    if (!h->norm_scratch &&
+        hipMalloc(reinterpret_cast<void**>(&h->norm_scratch), sizeof(double) * (kBlocks + 1)) != hipSuccess)
+        return fail(h, ADSG_ENOMEM, "norm: no memory for partial sums");
+    long E = 1;
+    for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
+    const int blocks = static_cast<int>(std::min<long>(kBlocks, (E + 255) / 256));
+    const double* u = static_cast<const double*>(h->vec[which]);
+    hipError_t e;
+    if (h->precision == ADSG_FP32) {
+        adsg::dev_tables<float> t[3];
+        for (int d = 0; d < h->dim; ++d) {
+            t[d].val = static_cast<const float*>(h->dirs[d].val);
+            t[d].der = static_cast<const float*>(h->dirs[d].der);
+            t[d].wj = static_cast<const float*>(h->dirs[d].wj);
+            t[d].elements = h->dirs[d].tab.elements;
+            t[d].dofs = h->dirs[d].tab.dofs;
+        }
+        e = adsg::launch_norm<float>(h->dim, h->dirs[0].tab.p, u, t, kind, h->norm_scratch, blocks,
+                                     h->norm_scratch + kBlocks, h->stream);
+    } else {
+        adsg::dev_tables<double> t[3];
+        fp64_tables(h, t);
+        e = adsg::launch_norm<double>(h->dim, h->dirs[0].tab.p, u, t, kind, h->norm_scratch, blocks,
+                                      h->norm_scratch + kBlocks, h->stream);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "norm");
+    double sq = 0;
+    HIP_TRY(h, hipMemcpyAsync(&sq, h->norm_scratch + kBlocks, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    *out = std::sqrt(sq);
+    return ADSG_OK;
 }
 
 int adsg_ads_solve_transposed(adsg_sim* h, const int* slots) {

@@ -112,6 +112,12 @@ template <typename T>
 hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,
                         const rhs_params<T>& prm, hipStream_t s);
 
+// Squared norm of the FP64 coefficient vector u: sum_e sum_q w J phi(u), phi = u^2 (kind 0),
+// |grad u|^2 (1), u^2 + |grad u|^2 (2); `blocks` partial sums in partial[], total in out[0].
+template <typename TT>
+hipError_t launch_norm(int dim, int p, const double* u, const dev_tables<TT>* dirs, int kind,
+                       double* partial, int blocks, double* out, hipStream_t s);
+
 template <typename T>
 hipError_t launch_fill(T* p, long n, T v, hipStream_t s);
 

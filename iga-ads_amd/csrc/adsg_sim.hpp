@@ -68,6 +68,7 @@ struct adsg_sim {
     int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
     std::string err;
     hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
+    double* norm_scratch = nullptr;  // adsg_norm: block partial sums and the total
     // captured step graphs keyed by (form, flags, u pointer, steps in graph)
     std::map<std::tuple<std::string, int, void*, int>, hipGraphExec_t> graphs;
 };

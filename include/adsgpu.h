@@ -134,6 +134,16 @@ int adsg_zero_boundary(adsg_sim* h);
  * simulation_2d.hpp:41). */
 int adsg_ads_solve(adsg_sim* h, const int* slots);
 
+/* normL2 / normH1 of basic_simulation_2d.hpp:291-316 (basic_simulation_3d.hpp likewise) on
+ * the device: sqrt(sum_e sum_q w J phi(u)) of u (which = ADSG_U or ADSG_U_PREV) with
+ * phi = u^2 (ADSG_NORM_L2), |grad u|^2 (ADSG_NORM_H10, the H10 struct of :46-48) or
+ * u^2 + |grad u|^2 (ADSG_NORM_H1). Summed in FP64 in a fixed order (deterministic); FP32
+ * handles evaluate with their FP32 basis tables. */
+#define ADSG_NORM_L2 0
+#define ADSG_NORM_H10 1
+#define ADSG_NORM_H1 2
+int adsg_norm(adsg_sim* h, int which, int kind, double* out);
+
 /* The transpose of ads_solve: u := (A_x (x) A_y [(x) A_z])^-T u, direction by direction
  * with dgbtrs 'T' semantics on the same FP64 factors (band_solve.hpp:20-31 with
  * trans = 'T'; the reference calls dgbtrs only with 'N'). Needed by adjoint steps. FP64 handles; every

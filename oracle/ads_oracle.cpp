@@ -948,6 +948,53 @@ void orc_sim_kgrad(const orc_sim* s, const orc_rhs_form* f, const double* u, con
     }
 }
 
+// basic_simulation_2d.hpp:42-52, :291-316 (3D: basic_simulation_3d.hpp): the squared norm
+// sum_e sum_q w J phi(u) with phi = u^2 (kind 0, L2), |grad u|^2 (kind 1, H1_0) or
+// u^2 + |grad u|^2 (kind 2, H1); the reference's norm() returns its square root.
+double orc_sim_norm_sq(const orc_sim* s, const double* u, int kind) {
+    const long n = element_count(s);
+    int e[3];
+    const orc_dim& X = s->d[0];
+    const orc_dim& Y = s->d[1];
+    const orc_dim& Z = s->d[2];
+    const int dim = s->dim;
+    const int mx = X.p + 1, my = Y.p + 1, mz = dim == 3 ? Z.p + 1 : 1;
+    double val = 0;
+    for (long l = 0; l < n; ++l) {
+        element_index(s, l, e);
+        double J = X.J[e[0]] * Y.J[e[1]];
+        if (dim == 3) J = J * Z.J[e[2]];
+        const int qz_n = dim == 3 ? Z.q : 1;
+        for (int qx = 0; qx < X.q; ++qx)
+            for (int qy = 0; qy < Y.q; ++qy)
+                for (int qz = 0; qz < qz_n; ++qz) {
+                    double w = X.w[qx] * Y.w[qy];
+                    if (dim == 3) w = w * Z.w[qz];
+                    const double* tx = tab(X, e[0], qx);
+                    const double* ty = tab(Y, e[1], qy);
+                    const double* tz = dim == 3 ? tab(Z, e[2], qz) : nullptr;
+                    fval v;
+                    for (int bx = 0; bx < mx; ++bx)
+                        for (int by = 0; by < my; ++by)
+                            for (int bz = 0; bz < mz; ++bz) {
+                                const long gi = (X.first[e[0]] + bx) +
+                                                X.n * ((Y.first[e[1]] + by) +
+                                                       static_cast<long>(Y.n) * ((dim == 3 ? Z.first[e[2]] : 0) + bz));
+                                fval B = dim == 3 ? eval_basis3(tx, ty, tz, mx, my, mz, bx, by, bz)
+                                                  : eval_basis2(tx, ty, mx, my, bx, by);
+                                v.val += u[gi] * B.val;
+                                v.dx += u[gi] * B.dx;
+                                v.dy += u[gi] * B.dy;
+                                v.dz += u[gi] * B.dz;
+                            }
+                    const double g2 = v.dx * v.dx + v.dy * v.dy + (dim == 3 ? v.dz * v.dz : 0.0);
+                    const double phi = kind == 0 ? v.val * v.val : kind == 1 ? g2 : v.val * v.val + g2;
+                    val += phi * w * J;
+                }
+    }
+    return val;
+}
+
 // Greville abscissae xi_i = (t_{i+1} + ... + t_{i+p}) / p
 void orc_greville(const double* knot, int knot_size, int p, double* out) {
     int n = knot_size - p - 1;

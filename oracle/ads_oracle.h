@@ -112,6 +112,8 @@ int orc_sim_tables(const orc_sim* s, int dir, double* b, double* x, double* J, d
 /* L2-projection right-hand side of a coefficient field (projection.hpp:11-153 pattern) and
  * the weighted norm sum_e sum_q w J u(x_q)^2 (implicit.hpp L2norm) */
 double orc_sim_l2norm_sq(const orc_sim* s, const double* u);
+/* squared L2 (kind 0), H1_0 seminorm (1) or H1 norm (2) of u (basic_simulation_2d.hpp:42-52) */
+double orc_sim_norm_sq(const orc_sim* s, const double* u, int kind);
 
 /* ---- threaded element loop: executor contract for_each + synchronized
  * (executor/sequential.hpp:12-26, executor/galois.hpp:27-44) with std::thread workers ---- */

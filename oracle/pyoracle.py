@@ -61,6 +61,8 @@ def lib():
         L.orc_sim_kgrad.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp]
         L.orc_sim_l2norm_sq.argtypes = [C.c_void_p, _dp]
         L.orc_sim_l2norm_sq.restype = C.c_double
+        L.orc_sim_norm_sq.argtypes = [C.c_void_p, _dp, C.c_int]
+        L.orc_sim_norm_sq.restype = C.c_double
         L.orc_init_gaussians.argtypes = [C.c_void_p, C.c_ulonglong, C.c_int, C.c_double,
                                          C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         L.orc_find_span.argtypes = [C.c_double, _dp, C.c_int, C.c_int]
@@ -237,6 +239,11 @@ class Sim:
     def l2norm_sq(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         return lib().orc_sim_l2norm_sq(self.h, _d(u))
+
+    def norm_sq(self, u, kind):
+        """Squared L2 (0), H1_0 (1) or H1 (2) norm of the coefficient vector u."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().orc_sim_norm_sq(self.h, _d(u), kind)
 
     def init_gaussians(self, seed, count, sigma=(0.1, 0.1), centre=(0.3, 0.7), noise=1e-3):
         u = self.zeros()

@@ -342,3 +342,20 @@ def test_autotune_keeps_state_and_picks_a_variant():
     ref = o.solve(o.compute_rhs(u, po.heat_form(1e-4, 3), threads=8))
     ref = o.solve(o.compute_rhs(ref, po.heat_form(1e-4, 3), threads=8))
     assert po.rel_l2(g.get(ag.U), ref) <= FP64_STEP_TOL
+
+
+@pytest.mark.parametrize("dim,p,ne,prec", [(2, 2, (40, 36), ag.FP64), (3, 3, (12, 10, 9), ag.FP64),
+                                           (3, 2, (16, 14, 12), ag.FP32)])
+def test_device_norms_match_oracle(dim, p, ne, prec):
+    """normL2 / H1_0 / normH1 on the device (basic_simulation_2d.hpp:42-52) against the
+    oracle's functionals (pinned by the Kronecker quadratic forms); FP32 handles evaluate
+    with FP32 basis tables."""
+    _log(f"case: device norms dim={dim} p={p} {ne}")
+    g = ag.Simulation(dim, [p] * dim, list(ne), precision=prec)
+    o = po.Sim(dim, [p] * dim, list(ne))
+    u = np.random.default_rng(90 + dim).standard_normal(g.size)
+    g.set(ag.U, u)
+    tol = 1e-13 if prec == ag.FP64 else 1e-6
+    for kind in range(3):
+        ref = np.sqrt(o.norm_sq(u, kind))
+        assert abs(g.norm(kind) - ref) <= tol * ref, kind

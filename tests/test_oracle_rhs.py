@@ -219,3 +219,31 @@ def test_kgrad_equals_rhs_with_unit_k_fields(dim, p, ne):
         ref[b] = w @ o.compute_rhs(u, f)
     assert np.allclose(g, ref, rtol=1e-12, atol=1e-14)
     assert np.abs(g).max() > 1e-3
+
+
+@pytest.mark.parametrize("dim,p,ne", [(2, 2, (5, 4)), (3, 1, (3, 2, 4)), (2, 3, (4, 3))])
+def test_norms_equal_quadratic_forms_of_the_1d_matrices(dim, p, ne):
+    # exact quadrature: ||u||_L2^2 = u^T (M_z (x) M_y (x) M_x) u and
+    # |u|_H1^2 = u^T (sum_d K_d (x) M_others) u, from the pinned 1D matrices
+    o = po.Sim(dim, [p] * dim, list(ne))
+    M, K = [], []
+    for d in range(dim):
+        n = o.shape[d]
+        M.append(_dense(o.matrix(d, 0)[0], n, p))
+        o.set_matrix(d, 1, 0.0, 1.0)
+        K.append(_dense(o.matrix(d, 1)[0], n, p))
+
+    def kron(mats):  # first index fastest: kron(A_z, kron(A_y, A_x))
+        out = mats[0]
+        for m in mats[1:]:
+            out = np.kron(m, out)
+        return out
+
+    Mt = kron(M)
+    Kt = sum(kron([K[d] if d == e else M[d] for d in range(dim)]) for e in range(dim))
+    u = np.random.default_rng(70 + dim).standard_normal(o.size)
+    l2, h10, h1 = (o.norm_sq(u, k) for k in range(3))
+    assert abs(l2 - u @ Mt @ u) <= 1e-12 * abs(l2)
+    assert abs(h10 - u @ Kt @ u) <= 1e-12 * abs(h10)
+    assert abs(h1 - (l2 + h10)) <= 1e-12 * h1
+    assert abs(l2 - o.l2norm_sq(u)) <= 1e-13 * l2
