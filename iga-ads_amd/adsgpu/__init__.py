@@ -62,7 +62,7 @@ EXPORTS = [
     "adsg_set_vector", "adsg_get_vector", "adsg_device_ptr", "adsg_swap", "adsg_compute_rhs",
     "adsg_zero_boundary", "adsg_ads_solve", "adsg_ads_solve_transposed", "adsg_step",
     "adsg_step_delta", "adsg_adjoint_step", "adsg_adjoint_step_delta", "adsg_synchronize",
-    "adsg_set_vector_device", "adsg_get_vector_device", "adsg_norm",
+    "adsg_set_vector_device", "adsg_get_vector_device", "adsg_norm", "adsg_set_line_matrices",
     "adsg_batch_create", "adsg_batch_destroy", "adsg_batch_last_error", "adsg_batch_count",
     "adsg_batch_size", "adsg_batch_dofs", "adsg_batch_prepare", "adsg_batch_set_forms",
     "adsg_batch_set_vector", "adsg_batch_get_vector", "adsg_batch_step", "adsg_batch_synchronize",
@@ -127,6 +127,7 @@ def lib():
         L.adsg_batch_autotune.argtypes = [C.c_void_p, _dp]
         L.adsg_set_vector_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_norm.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.adsg_set_line_matrices.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.adsg_get_vector_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_synchronize.argtypes = [C.c_void_p]
         L.adsg_band_factorize.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip]
@@ -250,6 +251,13 @@ class Simulation:
         """One adjoint step of step(): u holds lam(n+1), u_prev the forward state u(n);
         u := lam(n) and dL/dk is added into GRAD_K when the form uses the k field."""
         self._check(lib().adsg_adjoint_step(self.h, C.byref(form), flags), "adjoint_step")
+
+    def set_line_matrices(self, d, slot, kl, ku, abs_):
+        """One LAPACK band matrix per line of direction d (generalized ADS); abs_ has shape
+        (lines, n, 2kl+ku+1) in the reference's cyclic line order."""
+        a = np.ascontiguousarray(abs_, dtype=np.float64)
+        self._check(lib().adsg_set_line_matrices(self.h, d, slot, kl, ku, a.ctypes.data_as(_dp)),
+                    "set_line_matrices")
 
     def norm(self, kind=0, which=U):
         """normL2 (kind 0), H1_0 seminorm (1) or normH1 (2) of a state vector, on the device."""

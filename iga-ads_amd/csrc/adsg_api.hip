@@ -123,6 +123,20 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0) {
     if (sl < 0 || sl >= static_cast<int>(h->dirs[d].slots.size()) || !h->dirs[d].slots[sl].factored)
         return fail(h, ADSG_ESTATE, "ads_solve: matrix slot not set or not factored");
     const dev_slot& s = h->dirs[d].slots[sl];
+    if (s.lines > 0) {
+        if (mode != 0) return fail(h, ADSG_EINVAL, "ads_solve: per-line matrices support the plain solve only");
+        adsg::own_factors<T> of;
+        of.lower = static_cast<const T*>(s.lower);
+        of.upper = static_cast<const T*>(s.upper);
+        of.rdiag = static_cast<const T*>(s.rdiag);
+        of.poff = s.poff;
+        of.n = s.lf.n;
+        of.kl = s.lf.kl;
+        of.ku = s.lf.ku_eff;
+        of.pivoted = s.lf.pivoted;
+        hipError_t e = adsg::launch_line_sweep_own<T>(u, geometry(h, d), of, h->stream);
+        return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "line sweep (per-line matrices)");
+    }
     adsg::dev_factors<T> f;
     f.lower = static_cast<const T*>(s.lower);
     f.upper = static_cast<const T*>(s.upper);
@@ -162,14 +176,26 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0) {
 }
 
 // ads_solve: one sweep per direction in the order x, y, z (solver.hpp:35-41)
+// The generalized ADS (solver.hpp:43-195) solves the direction with per-line matrices first
+// and the others after it in cyclic order; the standard directions commute.
 template <typename T>
 int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0) {
-    for (int d = 0; d < 3; ++d) {
-        if (d < h->dim) {
+    int special = -1;
+    for (int d = 0; d < h->dim; ++d) {
+        const int sl = slots ? slots[d] : 0;
+        if (sl >= 0 && sl < static_cast<int>(h->dirs[d].slots.size()) && h->dirs[d].slots[sl].lines > 0) {
+            if (special >= 0) return fail(h, ADSG_EINVAL, "ads_solve: at most one direction with per-line matrices");
+            special = d;
+        }
+    }
+    const int start = special < 0 ? 0 : special;
+    for (int i = 0; i < 3; ++i) {
+        if (i < h->dim) {
+            const int d = (start + i) % h->dim;
             int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode);
             if (rc) return rc;
         }
-        mark(h, 4 + d);
+        mark(h, 4 + i);
     }
     return ADSG_OK;
 }
@@ -430,6 +456,69 @@ std::string form_key(const adsg_rhs_form* f) {
     return buf;
 }
 
+// Factor every line's matrix of a per-line slot (FP64, dgbtrf) and upload the factors
+// interleaved by line in the order of geometry(h, dir). The reference numbers the lines of
+// direction d by the remaining indices in cyclic order (d+1 fastest, solver.hpp:100-150),
+// which differs from geometry's order only for y lines in 3D (z fastest there, x here).
+int prepare_line_matrices(adsg_sim* h, int dir, dev_slot& s) {
+    const int n = h->dirs[dir].tab.dofs, kl = s.kl, ku = s.ku, ld = 2 * kl + ku + 1;
+    const long L = s.lines;
+    std::vector<adsg::line_factors> lf(L);
+    int first_info = 0;
+    bool pivoted = false;
+    int kue = ku;
+    for (long r = 0; r < L; ++r) {
+        std::vector<double> ab(s.ab.begin() + r * ld * n, s.ab.begin() + (r + 1) * ld * n);
+        int info = adsg::factor_for_lines(ab, n, kl, ku, lf[r]);
+        if (info && !first_info) first_info = info;
+        pivoted = pivoted || lf[r].pivoted;
+    }
+    if (first_info) return first_info;
+    if (pivoted) kue = std::min(kl + ku, std::max(n - 1, 0));
+    const long nx = h->dirs[0].tab.dofs, nz = h->dim == 3 ? h->dirs[2].tab.dofs : 1;
+    auto ref_line = [&](long l) -> long {
+        if (h->dim == 3 && dir == 1) return (l / nx) + nz * (l % nx);  // l = x + nx*z -> z + nz*x
+        return l;
+    };
+    std::vector<double> lower(static_cast<size_t>(n) * std::max(kl, 1) * L, 0.0);
+    std::vector<double> upper(static_cast<size_t>(n) * std::max(kue, 1) * L, 0.0);
+    std::vector<double> rdiag(static_cast<size_t>(n) * L, 0.0);
+    std::vector<int> poff(static_cast<size_t>(n) * L, 0);
+    for (long l = 0; l < L; ++l) {
+        const adsg::line_factors& f = lf[ref_line(l)];
+        for (int j = 0; j < n; ++j) {
+            for (int t = 0; t < kl; ++t) lower[(static_cast<size_t>(j) * kl + t) * L + l] = f.lower[static_cast<size_t>(j) * kl + t];
+            for (int t = 0; t < f.ku_eff; ++t)
+                upper[(static_cast<size_t>(j) * kue + t) * L + l] = f.upper[static_cast<size_t>(j) * f.ku_eff + t];
+            rdiag[static_cast<size_t>(j) * L + l] = f.rdiag[j];
+            poff[static_cast<size_t>(j) * L + l] = f.poff[j];
+        }
+    }
+    free_slot(s);
+    int rc;
+    if (h->precision == ADSG_FP32) {
+        if ((rc = upload<float>(h, &s.lower, lower))) return rc;
+        if ((rc = upload<float>(h, &s.upper, upper))) return rc;
+        if ((rc = upload<float>(h, &s.rdiag, rdiag))) return rc;
+    } else {
+        if ((rc = upload<double>(h, &s.lower, lower))) return rc;
+        if ((rc = upload<double>(h, &s.upper, upper))) return rc;
+        if ((rc = upload<double>(h, &s.rdiag, rdiag))) return rc;
+    }
+    if (hipMalloc(reinterpret_cast<void**>(&s.poff), sizeof(int) * poff.size()) != hipSuccess)
+        return fail(h, ADSG_ENOMEM, "hipMalloc failed for pivots");
+    HIP_TRY(h, hipMemcpy(s.poff, poff.data(), sizeof(int) * poff.size(), hipMemcpyHostToDevice));
+    s.lf = adsg::line_factors{};
+    s.lf.n = n;
+    s.lf.kl = kl;
+    s.lf.ku = ku;
+    s.lf.ku_eff = kue;
+    s.lf.pivoted = pivoted;
+    s.seg = adsg::line_segments{};
+    s.factored = true;
+    return ADSG_OK;
+}
+
 enum step_kind { EXPLICIT_STEP, DELTA_STEP, ADJOINT_DELTA_STEP };
 
 int step_of(adsg_sim* h, step_kind kind, const adsg_rhs_form* form, const int* slots, int flags) {
@@ -666,6 +755,27 @@ int adsg_set_band_matrix(adsg_sim* h, int dir, int slot, int kl, int ku, const d
     return ADSG_OK;
 }
 
+int adsg_set_line_matrices(adsg_sim* h, int dir, int slot, int kl, int ku, const double* ab) {
+    if (!valid_dir(h, dir) || !ab) return fail(h, ADSG_EINVAL, "set_line_matrices: bad argument");
+    if (slot < 1 || slot > 64) return fail(h, ADSG_EINVAL, "set_line_matrices: slot must be in 1..64");
+    if (kl < 0 || ku < 0 || kl > 8 || ku > 8)
+        return fail(h, ADSG_EINVAL, "set_line_matrices: kl and ku must be in 0..8");
+    dev_dir& d = h->dirs[dir];
+    if (static_cast<int>(d.slots.size()) <= slot) d.slots.resize(slot + 1);
+    dev_slot& s = d.slots[slot];
+    free_slot(s);
+    s = dev_slot{};
+    s.set = true;
+    s.kl = kl;
+    s.ku = ku;
+    s.lines = h->N / d.tab.dofs;
+    const size_t total = static_cast<size_t>(2 * kl + ku + 1) * d.tab.dofs * s.lines;
+    s.ab.assign(ab, ab + total);
+    h->prepared = false;
+    drop_graphs(h);
+    return ADSG_OK;
+}
+
 int adsg_prepare_matrices(adsg_sim* h) {
     if (!h) return ADSG_EINVAL;
     int first_info = 0;
@@ -673,6 +783,15 @@ int adsg_prepare_matrices(adsg_sim* h) {
         dev_dir& d = h->dirs[dd];
         for (auto& s : d.slots) {
             if (!s.set || s.factored) continue;
+            if (s.lines > 0) {
+                int rc = prepare_line_matrices(h, dd, s);
+                if (rc > 0) {
+                    if (!first_info) first_info = rc;
+                    continue;
+                }
+                if (rc) return rc;
+                continue;
+            }
             int info = adsg::factor_for_lines(s.ab, d.tab.dofs, s.kl, s.ku, s.lf);
             if (info != 0) {
                 if (!first_info) first_info = info;

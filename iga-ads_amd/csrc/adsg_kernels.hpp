@@ -50,6 +50,22 @@ template <typename T, typename F>
 hipError_t launch_line_sweep_seg(T* u, const line_geometry& g, const seg_factors<F>& f,
                                  hipStream_t s);
 
+// Per-line factors (one band matrix per line, lines of geometry g in its order): entry k
+// of line l at k * n_lines + l, widths kl and ku (ku = kl + ku of the matrix when pivoted).
+template <typename T>
+struct own_factors {
+    const T* lower = nullptr;  // [n*kl][L]
+    const T* upper = nullptr;  // [n*ku][L]
+    const T* rdiag = nullptr;  // [n][L]
+    const int* poff = nullptr; // [n][L]
+    int n = 0, kl = 0, ku = 0;
+    bool pivoted = false;
+};
+
+template <typename T>
+hipError_t launch_line_sweep_own(T* u, const line_geometry& g, const own_factors<T>& f,
+                                 hipStream_t s);
+
 // Tables of one direction on the device: val/der [elements][p+1][p+1] (element, Gauss
 // point, local function) and wj [elements][p+1].
 template <typename T>

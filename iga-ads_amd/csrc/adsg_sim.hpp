@@ -32,6 +32,9 @@ struct dev_slot {
     void* gft = nullptr;  // transposed solve
     void* hbt = nullptr;
     double* arow = nullptr;  // FP64 rows of the unfactored matrix, for refinement
+    // lines > 0: one matrix per line (generalized ADS), ab holds them all in the
+    // reference's line order; the device factors are interleaved by line (own_factors)
+    long lines = 0;
 };
 
 struct dev_dir {

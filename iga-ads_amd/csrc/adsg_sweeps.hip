@@ -217,6 +217,42 @@ __global__ __launch_bounds__(64) void k_sweep_contig(T* __restrict__ u, long n_l
     contig_lines<T, F, KL, KU, PIV>(u, static_cast<long>(blockIdx.x) * 64, n_lines, f, tile);
 }
 
+// Lines with their own factors (the generalized ADS of solver.hpp:43-195, one matrix per
+// line of the special direction): lane = line, factors interleaved by line (entry k of line
+// l at k*L + l) so that the lanes of a wave read consecutive addresses. dgbtrs 'N' order:
+// interchange, then column updates of L; back substitution by columns of U.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_own(T* __restrict__ u, long n_lines, long A, long B,
+                                                   long S, int n, int kl, int ku,
+                                                   const T* __restrict__ lower,
+                                                   const T* __restrict__ upper,
+                                                   const T* __restrict__ rdiag,
+                                                   const int* __restrict__ poff, int pivoted) {
+    const long l = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (l >= n_lines) return;
+    T* p = S == 1 ? u + l * n : u + (l % A) + (l / A) * B;
+    const long L = n_lines;
+    for (int j = 0; j < n; ++j) {
+        if (pivoted) {
+            const int o = poff[j * L + l];
+            if (o != 0) {
+                const T t = p[j * S];
+                p[j * S] = p[(j + o) * S];
+                p[(j + o) * S] = t;
+            }
+        }
+        const T bj = p[j * S];
+        for (int t = 1; t <= kl && j + t < n; ++t)
+            p[(j + t) * S] -= lower[(static_cast<long>(j) * kl + t - 1) * L + l] * bj;
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        const T xj = p[j * S] * rdiag[static_cast<long>(j) * L + l];
+        p[j * S] = xj;
+        for (int t = 1; t <= ku && j - t >= 0; ++t)
+            p[(j - t) * S] -= upper[(static_cast<long>(j) * ku + t - 1) * L + l] * xj;
+    }
+}
+
 // Many problems in one launch (unpivoted factors): block b belongs to the last item whose
 // first <= b; x lines (S == 1) are staged through LDS, other lines are one per lane.
 template <typename T, int KL, int KU>
@@ -304,6 +340,20 @@ hipError_t launch_sweep_batch(int kl, int ku, const sweep_item<T>* items, int ni
     return hipErrorInvalidValue;
 }
 
+template <typename T>
+hipError_t launch_line_sweep_own(T* u, const line_geometry& g, const own_factors<T>& f,
+                                 hipStream_t s) {
+    if (g.n_lines <= 0 || f.n <= 0) return hipSuccess;
+    const unsigned blocks = static_cast<unsigned>((g.n_lines + 255) / 256);
+    hipLaunchKernelGGL(k_sweep_own<T>, dim3(blocks), dim3(256), 0, s, u, g.n_lines, g.A, g.B, g.S,
+                       f.n, f.kl, f.ku, f.lower, f.upper, f.rdiag, f.poff, f.pivoted ? 1 : 0);
+    return hipGetLastError();
+}
+
+template hipError_t launch_line_sweep_own<double>(double*, const line_geometry&,
+                                                  const own_factors<double>&, hipStream_t);
+template hipError_t launch_line_sweep_own<float>(float*, const line_geometry&,
+                                                 const own_factors<float>&, hipStream_t);
 template hipError_t launch_sweep_batch<double>(int, int, const sweep_item<double>*, int, int,
                                                hipStream_t);
 template hipError_t launch_sweep_batch<float>(int, int, const sweep_item<float>*, int, int,

@@ -105,6 +105,14 @@ int adsg_set_matrix(adsg_sim* h, int dir, int slot, double mass, double stiff);
  * (band_matrix.hpp:31-32, column size 2kl+ku+1), n = adsg_dofs(h, dir). */
 int adsg_set_band_matrix(adsg_sim* h, int dir, int slot, int kl, int ku, const double* ab);
 
+/* The generalized ADS (solver.hpp:43-195, tests solver_test.cpp:257-534): direction dir gets
+ * one band matrix per line in slot `slot` (>= 1). ab holds the lines' LAPACK full buffers
+ * ((2kl+ku+1) x n each) one after another, lines numbered by the remaining indices in the
+ * reference's cyclic order (dir+1 fastest: x lines by (y, z), y lines by (z, x), z lines by
+ * (x, y)). ads_solve with that slot solves this direction first and the others after it in
+ * cyclic order, as the reference does with a callable in that position. */
+int adsg_set_line_matrices(adsg_sim* h, int dir, int slot, int kl, int ku, const double* ab);
+
 /* prepare_matrices (simulation_2d.hpp:56-59, simulation_3d.hpp:58-62): factor every slot
  * in FP64 (lin::factorize -> dgbtrf, band_solve.hpp:16-18) and upload the factors.
  * Returns the first nonzero LAPACK info (zero pivot) if any. */
