@@ -196,6 +196,26 @@ def f_elem_source(d, p):
     return 2 * (8 + 3 + 9) * n ** (d + 1) + (d + 2) * n ** d
 
 
+def copy_calibration(mib, reps):
+    """A plain device-to-device copy of known size through the HIP runtime (its copy
+    kernel reads and writes mib MiB per launch): the reference point for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of the step's kernels."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_mib": mib, "launches": reps}), flush=True)
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_rhs3_heat3d_p2_128.md")
 
 

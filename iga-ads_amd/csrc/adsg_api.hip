@@ -1082,7 +1082,8 @@ int adsg_set_tuning(adsg_sim* h, int key, int value) {
     int* slot = key == ADSG_TUNE_RHS_VARIANT ? &h->rhs_variant
                 : key == ADSG_TUNE_SWEEP_VARIANT ? &h->sweep_variant
                                                  : nullptr;
-    if (!slot || value < 0 || value > 1) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
+    const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 2 : 1;
+    if (!slot || value < 0 || value > max_value) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
     if (*slot != value) drop_graphs(h);
     *slot = value;
     return ADSG_OK;
@@ -1119,9 +1120,9 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
     int best = h->rhs_variant, best_sweep = h->sweep_variant;
     double best_ms = 1e30, best_sweep_ms = 1e30;
     int rc = ADSG_OK;
-    for (int v = 0; v <= 1 && rc == ADSG_OK; ++v) {
+    for (int v = 0; v <= 2 && rc == ADSG_OK; ++v) {
         h->rhs_variant = v;
-        h->sweep_variant = v;
+        h->sweep_variant = std::min(v, 1);
         double ms[7];
         rc = adsg_time_phases(h, 1, form, slots, flags, ms);  // warm-up (code load, caches)
         if (!rc) rc = adsg_time_phases(h, 3, form, slots, flags, ms);
@@ -1130,7 +1131,7 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
             best = v;
         }
         const double sweeps = ms[2] + ms[3] + ms[4];
-        if (!rc && sweeps < best_sweep_ms) {
+        if (!rc && v <= 1 && sweeps < best_sweep_ms) {
             best_sweep_ms = sweeps;
             best_sweep = v;
         }

@@ -13,8 +13,10 @@ namespace {
 // ------------------------------------------------------------------------------------
 // element-loop RHS, 3D
 // ------------------------------------------------------------------------------------
-// CREG: hold the element's coefficient block in registers (fewer LDS reads) or re-read it
-// from LDS for every x Gauss point (fewer registers: two waves per SIMD).
+// VAR 0: hold the element's coefficient block in registers (fewer LDS reads); 1: re-read it
+// from LDS for every x Gauss point (fewer registers: two waves per SIMD); 2: as 1, and the
+// element's contribution is added to the LDS planes after every x Gauss point instead of
+// being accumulated in registers (fewer registers again; FP32 runs four waves per SIMD).
 //
 // One lane per element; the x/y tables of the tile's elements are staged in LDS, the
 // forward ones as tabulated and the backward ones with the quadrature weight (and, for the
@@ -23,12 +25,19 @@ namespace {
 // planes with LDS atomic adds: a wave's adds are executed in instruction order, so the sums
 // are deterministic, and waves never share a plane. One barrier later the planes of the
 // (P+1) waves that touch a DOF plane are added and the tile's block is added into the rhs.
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool CREG>
-__global__ __launch_bounds__(TX* TY* TZ, CREG ? 1 : 2) void k_rhs3(
+template <typename T, int P, int VAR>
+constexpr int rhs3_waves() {  // minimum waves per SIMD the register allocation must allow
+    return VAR == 0 ? 1 : VAR == 1 ? 2 : (sizeof(T) == sizeof(float) && P < 3 ? 4 : 2);
+}
+
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int VAR>
+__global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
     dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
     constexpr int M = P + 1, NT = TX * TY * TZ;
     constexpr int BX = TX + P, BY = TY + P, BZ = TZ + P, NB = BX * BY * BZ, PL = BX * BY;
+    constexpr bool CREG = VAR == 0;   // coefficient block held in registers
+    constexpr bool FLUSH = VAR == 2;  // element contribution added to the planes per x point
     static_assert(TX * TY == 64, "a wave must cover exactly one z-layer of the tile");
     __shared__ T blk[NB];
     __shared__ T kb[KF ? NB : 1];
@@ -88,14 +97,16 @@ __global__ __launch_bounds__(TX* TY* TZ, CREG ? 1 : 2) void k_rhs3(
     const int lx = tid % TX, ly = (tid / TX) % TY;
     const int lz = __builtin_amdgcn_readfirstlane(tid / (TX * TY));
     const int ez = ez0 + lz;
+    // the wave's planes: in one add instruction the 64 lanes hit 64 distinct entries
+    double* mine = acc + lz * M * PL;
 
-    T R[M][M][M];  // the element's contribution, (a, b, c) = local (x, y, z) function
+    T R[FLUSH ? 1 : M][FLUSH ? 1 : M][FLUSH ? 1 : M];  // the element's contribution, (a, b, c)
 #pragma unroll
-    for (int a = 0; a < M; ++a)
+    for (int a = 0; a < (FLUSH ? 1 : M); ++a)
 #pragma unroll
-        for (int b = 0; b < M; ++b)
+        for (int b = 0; b < (FLUSH ? 1 : M); ++b)
 #pragma unroll
-            for (int c = 0; c < M; ++c) R[a][b][c] = T(0);
+            for (int c = 0; c < (FLUSH ? 1 : M); ++c) R[a][b][c] = T(0);
 
     if (ez < Z.elements) {  // wave-uniform
         // z tables of the wave's element layer: uniform addresses in uniform control flow,
@@ -241,22 +252,29 @@ __global__ __launch_bounds__(TX* TY* TZ, CREG ? 1 : 2) void k_rhs3(
 #pragma unroll
                     for (int b = 0; b < M; ++b)
 #pragma unroll
-                        for (int c = 0; c < M; ++c) R[a][b][c] += wb * K0[b][c] + wd * Kx[b][c];
+                        for (int c = 0; c < M; ++c) {
+                            if constexpr (FLUSH)
+                                __hip_atomic_fetch_add(mine + c * PL + (lx + a) + BX * (ly + b),
+                                                       static_cast<double>(wb * K0[b][c] + wd * Kx[b][c]),
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                            else
+                                R[FLUSH ? 0 : a][FLUSH ? 0 : b][FLUSH ? 0 : c] += wb * K0[b][c] + wd * Kx[b][c];
+                        }
                 }
             }
         }
     }
-    // sum into the wave's planes: in one instruction the 64 lanes hit 64 distinct entries
-    double* mine = acc + lz * M * PL;
+    if constexpr (!FLUSH) {
 #pragma unroll
-    for (int c = 0; c < M; ++c)
+        for (int c = 0; c < M; ++c)
 #pragma unroll
-        for (int b = 0; b < M; ++b)
+            for (int b = 0; b < M; ++b)
 #pragma unroll
-            for (int a = 0; a < M; ++a)
-                __hip_atomic_fetch_add(mine + c * PL + (lx + a) + BX * (ly + b),
-                                       static_cast<double>(R[a][b][c]), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+                for (int a = 0; a < M; ++a)
+                    __hip_atomic_fetch_add(mine + c * PL + (lx + a) + BX * (ly + b),
+                                           static_cast<double>(R[FLUSH ? 0 : a][FLUSH ? 0 : b][FLUSH ? 0 : c]),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
     __syncthreads();
 
     // DOF plane k of the block collects plane k - lz of the waves lz = k-P .. k
@@ -487,7 +505,7 @@ struct tile<3, 3> {
     static constexpr int X = 8, Y = 8, Z = 3;
 };
 
-template <typename T, int P, bool KF, bool SRC, bool CREG>
+template <typename T, int P, bool KF, bool SRC, int VAR>
 hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                         hipStream_t s) {
     using TL = tile<3, P>;
@@ -499,7 +517,7 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
         const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2, nz2 = (ntz - cz + 1) / 2;
         const long blocks = static_cast<long>(nx2) * ny2 * nz2;
         if (blocks == 0) continue;
-        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, CREG>), dim3(blocks),
+        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, VAR>), dim3(blocks),
                            dim3(TL::X * TL::Y * TL::Z), 0, s, up, rhs, d[0], d[1], d[2], prm, c,
                            nx2, ny2);
         hipError_t e = hipGetLastError();
@@ -527,14 +545,14 @@ hipError_t rhs2_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
     return hipSuccess;
 }
 
-template <typename T, int P, bool CREG>
+template <typename T, int P, int VAR>
 hipError_t rhs3_fields(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                        hipStream_t s) {
     const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
-    if (kf && src) return rhs3_colours<T, P, true, true, CREG>(up, rhs, d, prm, s);
-    if (kf) return rhs3_colours<T, P, true, false, CREG>(up, rhs, d, prm, s);
-    if (src) return rhs3_colours<T, P, false, true, CREG>(up, rhs, d, prm, s);
-    return rhs3_colours<T, P, false, false, CREG>(up, rhs, d, prm, s);
+    if (kf && src) return rhs3_colours<T, P, true, true, VAR>(up, rhs, d, prm, s);
+    if (kf) return rhs3_colours<T, P, true, false, VAR>(up, rhs, d, prm, s);
+    if (src) return rhs3_colours<T, P, false, true, VAR>(up, rhs, d, prm, s);
+    return rhs3_colours<T, P, false, false, VAR>(up, rhs, d, prm, s);
 }
 
 template <typename T, int P>
@@ -542,8 +560,9 @@ hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
                       const rhs_params<T>& prm, int variant, hipStream_t s) {
     const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
     if (dim == 3) {
-        if (variant == 1) return rhs3_fields<T, P, false>(up, rhs, d, prm, s);
-        return rhs3_fields<T, P, true>(up, rhs, d, prm, s);
+        if (variant == 1) return rhs3_fields<T, P, 1>(up, rhs, d, prm, s);
+        if (variant == 2) return rhs3_fields<T, P, 2>(up, rhs, d, prm, s);
+        return rhs3_fields<T, P, 0>(up, rhs, d, prm, s);
     }
     if (kf && src) return rhs2_colours<T, P, true, true>(up, rhs, d, prm, s);
     if (kf) return rhs2_colours<T, P, true, false>(up, rhs, d, prm, s);

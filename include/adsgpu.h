@@ -209,8 +209,9 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
                      double* ms);
 
 /* Kernel tuning (no counterpart in the reference, whose CPU loops have no launch shape):
- * ADSG_TUNE_RHS_VARIANT selects the element-kernel variant (0: element coefficients held in
- * registers, 1: re-read from LDS); both compute the same sums in the same order.
+ * ADSG_TUNE_RHS_VARIANT selects the 3D element-kernel variant (0: element coefficients held
+ * in registers, 1: re-read from LDS, 2: as 1 with the element's contribution added to LDS
+ * per x Gauss point); they compute the same sums (variant 2 adds them in another order).
  * ADSG_TUNE_SWEEP_VARIANT selects the line sweep (0: lines cut into segments solved by
  * separate waves and joined through FP64-computed influence coefficients, where the factors
  * are unpivoted and the lines are 2..16 segments long; 1: one lane per whole line).

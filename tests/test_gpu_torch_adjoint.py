@@ -1,91 +1,24 @@
-"""The libtorch autograd Function over the ADS heat step with a k field: the 2D inverse
-problem u_t = div(k grad u), p=2, 128^2 elements, 200 steps of dt = 1e-6, Dirichlet
-boundary, k = exp(g) in [0.5, 2], L = ||u(T) - u_target||^2 with u_target from another
-seeded k. Gradients from loss.backward() (checkpoints every 20 steps) against the oracle's
-FP64 discrete adjoint (<= 1e-10) and central finite differences, h = 1e-6, on 32 seeded
-directions (<= 1e-5).
+"""The libtorch autograd Function over the ADS heat step with a k field (c4 shape: 2D,
+p=2, 128^2, 200 steps of dt = 1e-6, checkpoints every 20 steps), run in its own process by
+torch_adjoint_check.py: torch must bring up the GPU runtime before libadsgpu.so is loaded,
+which other tests in this process have already done the other way round. Bars: gradients
+against the oracle's FP64 discrete adjoint <= 1e-10, 32 central differences (h = 1e-6)
+<= 1e-5.
 """
-import numpy as np
-import pytest
+import os
+import subprocess
+import sys
 
-import adsgpu as ag
-import pyoracle as po
-from adsgpu import workloads as wl
+import pytest
 
 pytestmark = pytest.mark.gpu
 
-P, NE, DT, STEPS, EVERY = 2, (128, 128), 1e-6, 200, 20
-
-
-def _log(msg):
-    print(msg, flush=True)
-
-
-def _kfield(seed):
-    g = wl.sine_source([P, P], list(NE), seed=seed)
-    g = g / np.abs(g).max() * np.log(2.0)
-    return np.exp(g)
-
 
 def test_torch_function_gradient_matches_oracle_adjoint_and_fd():
-    import torch
-
-    _log("case: torch heat_k 2D p=2 128^2 200 steps, checkpoints every 20")
-    dev = torch.device("cuda:0")
-    sim = ag.Simulation(2, [P, P], list(NE))
-    sim.fix_all_boundaries()
-    assert sim.prepare_matrices() == 0
-    o = po.Sim(2, [P, P], list(NE))
-    o.fix_all_boundaries()
-    assert o.prepare_matrices() == 0
-    u0 = wl.gaussians([P, P], list(NE), seed=2024, count=1, sigma=(0.1, 0.1), noise=1e-3)
-    o.zero_boundary(u0)
-    k = _kfield(61)
-    kt = _kfield(62)
-    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
-    with torch.no_grad():
-        ut = ag.heat_k(sim, t(kt), t(u0), STEPS, EVERY, DT)
-    kk = t(k).requires_grad_(True)
-    uu = t(u0).requires_grad_(True)
-    uT = ag.heat_k(sim, kk, uu, STEPS, EVERY, DT)
-    loss = ((uT - ut) ** 2).sum()
-    loss.backward()
-    dk = kk.grad.cpu().numpy()
-    du0 = uu.grad.cpu().numpy()
-
-    # oracle: forward trajectory and discrete adjoint in FP64 on the CPU
-    form = po.heat_form(DT, 2, kfield=k)
-    traj = [u0]
-    for _ in range(STEPS):
-        r = o.compute_rhs(traj[-1], form, threads=8)
-        o.zero_boundary(r)
-        traj.append(o.solve(r))
-    assert po.rel_l2(uT.detach().cpu().numpy(), traj[-1]) <= 1e-12
-    lam = 2.0 * (traj[-1] - ut.cpu().numpy())
-    dk_o = np.zeros_like(k)
-    for n in range(STEPS - 1, -1, -1):
-        w = o.solve(lam.copy(), transposed=True)
-        o.zero_boundary(w)
-        dk_o += o.kgrad(traj[n], w, form)
-        lam = o.compute_rhs(w, form, threads=8)
-    e_k, e_u = po.rel_l2(dk, dk_o), po.rel_l2(du0, lam)
-    _log(f"  dL/dk vs oracle {e_k:.3e}  dL/du0 vs oracle {e_u:.3e}")
-    assert e_k <= 1e-10 and e_u <= 1e-10
-
-    def L(kv):
-        with torch.no_grad():
-            v = ag.heat_k(sim, t(kv), t(u0), STEPS, EVERY, DT)
-            return float(((v - ut) ** 2).sum())
-
-    rng = np.random.default_rng(63)
-    h = 1e-6
-    worst = 0.0
-    for j in range(32):
-        d = rng.standard_normal(k.size)
-        d /= np.linalg.norm(d)
-        fd = (L(k + h * d) - L(k - h * d)) / (2 * h)
-        ad = float(dk @ d)
-        worst = max(worst, abs(fd - ad) / abs(ad))
-    _log(f"  32 directions: worst relative FD disagreement {worst:.2e}")
-    assert worst <= 1e-5
-    sim.close()
+    print("case: torch heat_k (separate process)", flush=True)
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([sys.executable, "-u", os.path.join(here, "torch_adjoint_check.py")],
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout[-3000:], r.stderr[-3000:], flush=True)
+    assert r.returncode == 0
+    assert "torch adjoint check: ok" in r.stdout
