@@ -27,10 +27,10 @@ namespace {
 // (P+1) waves that touch a DOF plane are added and the tile's block is added into the rhs.
 template <typename T, int P, int VAR>
 constexpr int rhs3_waves() {  // minimum waves per SIMD the register allocation must allow
-    return VAR == 0 ? 1 : VAR == 1 ? 2 : (sizeof(T) == sizeof(float) && P < 3 ? 4 : 2);
+    return VAR == 0 ? 1 : VAR == 1 ? 2 : P == 3 ? 2 : sizeof(T) == sizeof(float) ? 4 : 3;
 }
 
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int VAR>
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX, int VAR>
 __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
     dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
@@ -123,7 +123,16 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                 Dz[q][c] = zd[q * M + c];
             }
         }
-        const T mass = prm.mass, cz = -prm.dt * prm.gz;
+        const T cz = -prm.dt * prm.gz;
+        // per z point constants with the weight folded in (wave-uniform)
+        T mwz[M], cwz[M], swz[M], rwz[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            mwz[q] = prm.mass * wz[q];
+            cwz[q] = cz * wz[q];
+            swz[q] = prm.dt * wz[q];
+            rwz[q] = prm.dt * prm.reaction * wz[q];
+        }
         if (ex0 + lx < X.elements && ey0 + ly < Y.elements) {
             T cf[CREG ? M : 1][CREG ? M : 1][CREG ? M : 1];
             T kc[KF && CREG ? M : 1][KF && CREG ? M : 1][KF && CREG ? M : 1];
@@ -220,16 +229,24 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                             if constexpr (KF) kv += Bz[qz][c] * ka[c];
                             if constexpr (SRC) fv += Bz[qz][c] * sa[c];
                         }
-                        T g0 = mass * u;
-                        if (prm.reaction != T(0)) g0 += prm.dt * prm.reaction * u * (T(1) - u);
-                        if constexpr (SRC) g0 += prm.dt * fv;
-                        T gw = wz[qz];
-                        if constexpr (KF) gw *= kv;
-                        g0 *= wz[qz];
-                        const T gx = gw * ux, gy = gw * uy, gz = (cz * gw) * uz;
+                        T g0 = mwz[qz] * u;
+                        if constexpr (RX) g0 += rwz[qz] * u * (T(1) - u);
+                        if constexpr (SRC) g0 += swz[qz] * fv;
+                        T gx, gy, gz;
+                        if constexpr (KF) {
+                            const T kw = wz[qz] * kv;
+                            gx = kw * ux;
+                            gy = kw * uy;
+                            gz = (cz * kw) * uz;
+                        } else {
+                            gx = wz[qz] * ux;
+                            gy = wz[qz] * uy;
+                            gz = cwz[qz] * uz;
+                        }
 #pragma unroll
-                        for (int c = 0; c < M; ++c) {
-                            H0[c] += Bz[qz][c] * g0 + Dz[qz][c] * gz;
+                        for (int c = 0; c < M; ++c) {  // separate updates: two FMAs, no add
+                            H0[c] += Bz[qz][c] * g0;
+                            H0[c] += Dz[qz][c] * gz;
                             Hx[c] += Bz[qz][c] * gx;
                             Hy[c] += Bz[qz][c] * gy;
                         }
@@ -240,7 +257,8 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                         const T wb = bvy[(ly * M + qy) * M + b], wd = bdy[(ly * M + qy) * M + b];
 #pragma unroll
                         for (int c = 0; c < M; ++c) {
-                            K0[b][c] += wb * H0[c] + wd * Hy[c];
+                            K0[b][c] += wb * H0[c];
+                            K0[b][c] += wd * Hy[c];
                             Kx[b][c] += wb * Hx[c];
                         }
                     }
@@ -257,8 +275,10 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                                 __hip_atomic_fetch_add(mine + c * PL + (lx + a) + BX * (ly + b),
                                                        static_cast<double>(wb * K0[b][c] + wd * Kx[b][c]),
                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-                            else
-                                R[FLUSH ? 0 : a][FLUSH ? 0 : b][FLUSH ? 0 : c] += wb * K0[b][c] + wd * Kx[b][c];
+                            else {
+                                R[FLUSH ? 0 : a][FLUSH ? 0 : b][FLUSH ? 0 : c] += wb * K0[b][c];
+                                R[FLUSH ? 0 : a][FLUSH ? 0 : b][FLUSH ? 0 : c] += wd * Kx[b][c];
+                            }
                         }
                 }
             }
@@ -425,7 +445,8 @@ __device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restric
 #pragma unroll
                 for (int b = 0; b < M; ++b) {
                     const T wb = bvy[(ly * M + qy) * M + b], wd = bdy[(ly * M + qy) * M + b];
-                    K0[b] += wb * g0 + wd * gy;
+                    K0[b] += wb * g0;
+                    K0[b] += wd * gy;
                     Kx[b] += wb * gx;
                 }
             }
@@ -433,7 +454,10 @@ __device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restric
             for (int a = 0; a < M; ++a) {
                 const T wb = bvx[(lx * M + qx) * M + a], wd = bdx[(lx * M + qx) * M + a];
 #pragma unroll
-                for (int b = 0; b < M; ++b) R[a][b] += wb * K0[b] + wd * Kx[b];
+                for (int b = 0; b < M; ++b) {
+                    R[a][b] += wb * K0[b];
+                    R[a][b] += wd * Kx[b];
+                }
             }
         }
     }
@@ -505,7 +529,7 @@ struct tile<3, 3> {
     static constexpr int X = 8, Y = 8, Z = 3;
 };
 
-template <typename T, int P, bool KF, bool SRC, int VAR>
+template <typename T, int P, bool KF, bool SRC, bool RX, int VAR>
 hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                         hipStream_t s) {
     using TL = tile<3, P>;
@@ -517,7 +541,7 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
         const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2, nz2 = (ntz - cz + 1) / 2;
         const long blocks = static_cast<long>(nx2) * ny2 * nz2;
         if (blocks == 0) continue;
-        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, VAR>), dim3(blocks),
+        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, RX, VAR>), dim3(blocks),
                            dim3(TL::X * TL::Y * TL::Z), 0, s, up, rhs, d[0], d[1], d[2], prm, c,
                            nx2, ny2);
         hipError_t e = hipGetLastError();
@@ -545,14 +569,21 @@ hipError_t rhs2_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
     return hipSuccess;
 }
 
+template <typename T, int P, bool RX, int VAR>
+hipError_t rhs3_forms(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
+                      hipStream_t s) {
+    const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
+    if (kf && src) return rhs3_colours<T, P, true, true, RX, VAR>(up, rhs, d, prm, s);
+    if (kf) return rhs3_colours<T, P, true, false, RX, VAR>(up, rhs, d, prm, s);
+    if (src) return rhs3_colours<T, P, false, true, RX, VAR>(up, rhs, d, prm, s);
+    return rhs3_colours<T, P, false, false, RX, VAR>(up, rhs, d, prm, s);
+}
+
 template <typename T, int P, int VAR>
 hipError_t rhs3_fields(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                        hipStream_t s) {
-    const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
-    if (kf && src) return rhs3_colours<T, P, true, true, VAR>(up, rhs, d, prm, s);
-    if (kf) return rhs3_colours<T, P, true, false, VAR>(up, rhs, d, prm, s);
-    if (src) return rhs3_colours<T, P, false, true, VAR>(up, rhs, d, prm, s);
-    return rhs3_colours<T, P, false, false, VAR>(up, rhs, d, prm, s);
+    if (prm.reaction != T(0)) return rhs3_forms<T, P, true, VAR>(up, rhs, d, prm, s);
+    return rhs3_forms<T, P, false, VAR>(up, rhs, d, prm, s);
 }
 
 template <typename T, int P>
