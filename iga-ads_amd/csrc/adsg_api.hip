@@ -79,7 +79,7 @@ int upload_tables(adsg_sim* h, dev_dir& d) {
 }
 
 void free_slot(dev_slot& s) {
-    for (void** p : {&s.lower, &s.upper, &s.rdiag, &s.gf, &s.hb, &s.gft, &s.hbt}) {
+    for (void** p : {&s.lower, &s.upper, &s.rdiag, &s.pack, &s.packt}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
@@ -155,10 +155,8 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0) {
     if (seg_ok && (mode != 0 || h->sweep_variant == 0)) {
         adsg::seg_factors<T> sf;
         sf.f = f;
-        sf.gf = static_cast<const T*>(s.gf);
-        sf.hb = static_cast<const T*>(s.hb);
-        sf.gft = static_cast<const T*>(s.gft);
-        sf.hbt = static_cast<const T*>(s.hbt);
+        sf.pack = static_cast<const T*>(s.pack);
+        sf.packt = static_cast<const T*>(s.packt);
         sf.arow = s.arow;
         sf.m = s.seg.m;
         sf.nseg = s.seg.nseg;
@@ -811,15 +809,11 @@ int adsg_prepare_matrices(adsg_sim* h) {
             s.seg = adsg::line_segments{};
             if (adsg::segment_lines(s.lf, s.seg)) {
                 if (h->precision == ADSG_FP32) {
-                    if ((rc = upload<float>(h, &s.gf, s.seg.gf))) return rc;
-                    if ((rc = upload<float>(h, &s.hb, s.seg.hb))) return rc;
-                    if ((rc = upload<float>(h, &s.gft, s.seg.gft))) return rc;
-                    if ((rc = upload<float>(h, &s.hbt, s.seg.hbt))) return rc;
+                    if ((rc = upload<float>(h, &s.pack, s.seg.pack))) return rc;
+                    if ((rc = upload<float>(h, &s.packt, s.seg.packt))) return rc;
                 } else {
-                    if ((rc = upload<double>(h, &s.gf, s.seg.gf))) return rc;
-                    if ((rc = upload<double>(h, &s.hb, s.seg.hb))) return rc;
-                    if ((rc = upload<double>(h, &s.gft, s.seg.gft))) return rc;
-                    if ((rc = upload<double>(h, &s.hbt, s.seg.hbt))) return rc;
+                    if ((rc = upload<double>(h, &s.pack, s.seg.pack))) return rc;
+                    if ((rc = upload<double>(h, &s.packt, s.seg.packt))) return rc;
                 }
                 // rows of the unfactored matrix: arow[j*w + t + kl] = A(j, j+t), t in [-kl, ku]
                 const int n = s.lf.n, w = s.kl + s.ku + 1, ld = 2 * s.kl + s.ku + 1;

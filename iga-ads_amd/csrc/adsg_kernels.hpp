@@ -28,19 +28,18 @@ hipError_t launch_line_sweep(T* u, const line_geometry& g, const dev_factors<F>&
                              hipStream_t s);
 
 // Segmented sweep of unpivoted factors: every line is cut into nseg segments of length m
-// (the last one shorter) solved concurrently; gf[j*kl + t] is the influence of the t-th
-// value of the previous segment's tail (positions j0-kl .. j0-1) on y_j, hb[j*ku + t] the
-// influence of the t-th value of the next segment's head (positions j1 .. j1+ku-1) on x_j.
-// For the transposed solve (A^T = U^T L^T) gft [n*ku] and hbt [n*kl] play the same roles.
-// refine: one refinement of every line solve against the FP64 residual w - A z, with arow
-// the unfactored matrix as rows of kl+ku+1 entries (columns j-kl .. j+ku).
+// (the last one shorter) solved concurrently. pack holds, per row j of the line padded to
+// nseg*m rows, the factor entries row j uses and its influence coefficients: G (the
+// influence of the previous segment's last kl values on y_j) and H (of the next segment's
+// first ku values on x_j); packt the same for the transposed solve (layout in
+// adsg_setup.hpp). refine: one refinement of every line solve against the FP64 residual
+// w - A z, with arow the unfactored matrix as rows of kl+ku+1 entries (columns j-kl ..
+// j+ku).
 template <typename F>
 struct seg_factors {
     dev_factors<F> f;
-    const F* gf = nullptr;   // [n*kl]
-    const F* hb = nullptr;   // [n*ku]
-    const F* gft = nullptr;  // [n*ku]
-    const F* hbt = nullptr;  // [n*kl]
+    const F* pack = nullptr;   // [nseg*m][2(kl+ku)+1]
+    const F* packt = nullptr;  // [nseg*m][2(kl+ku)+1]
     const double* arow = nullptr;  // [n*(kl+ku+1)]
     int m = 0, nseg = 0, mseg = 0;  // mseg: register capacity per lane (16 or 32)
     bool transposed = false, refine = false;

@@ -373,6 +373,32 @@ bool segment_lines(const line_factors& f, line_segments& out) {
             }
         }
     }
+    // packed rows for the sweep kernels (see adsg_setup.hpp), padded to nseg*m rows with
+    // identity rows so that every segment runs m uniform steps
+    const int np = nseg * m, w = 2 * (kl + ku) + 1;
+    out.pack.assign(static_cast<size_t>(np) * w, 0.0);
+    out.packt.assign(static_cast<size_t>(np) * w, 0.0);
+    for (int j = 0; j < np; ++j) {
+        double* r = &out.pack[static_cast<size_t>(j) * w];
+        double* rt = &out.packt[static_cast<size_t>(j) * w];
+        if (j >= n) {
+            r[kl + ku] = rt[ku + kl] = 1.0;
+            continue;
+        }
+        for (int t = 1; t <= kl; ++t) {
+            if (j - t >= 0) r[t - 1] = L(j, t);                                            // L(j, j-t)
+            if (j + t < n) rt[ku + t - 1] = f.lower[static_cast<size_t>(j) * kl + t - 1];  // L(j+t, j)
+        }
+        for (int t = 1; t <= ku; ++t) {
+            if (j + t < n) r[kl + t - 1] = U(j, t);                                        // U(j, j+t)
+            if (j - t >= 0) rt[t - 1] = f.upper[static_cast<size_t>(j) * ku + t - 1];      // U(j-t, j)
+        }
+        r[kl + ku] = rt[ku + kl] = f.rdiag[j];
+        for (int q = 0; q < kl; ++q) r[kl + ku + 1 + q] = out.gf[static_cast<size_t>(j) * kl + q];
+        for (int q = 0; q < ku; ++q) r[2 * kl + ku + 1 + q] = out.hb[static_cast<size_t>(j) * ku + q];
+        for (int q = 0; q < ku; ++q) rt[ku + kl + 1 + q] = out.gft[static_cast<size_t>(j) * ku + q];
+        for (int q = 0; q < kl; ++q) rt[2 * ku + kl + 1 + q] = out.hbt[static_cast<size_t>(j) * kl + q];
+    }
     return true;
 }
 

@@ -68,6 +68,13 @@ struct line_segments {
     int m = 0, nseg = 0, mseg = 0;
     std::vector<double> gf, hb;    // A = L U
     std::vector<double> gft, hbt;  // A^T = U^T L^T: gft [n*ku], hbt [n*kl]
+    // Everything row j of a sweep reads, packed per row (w = 2(kl+ku)+1 values) over
+    // nseg*m rows (rows >= n are identity rows): for A = L U
+    //   [L(j,j-t), t=1..kl | U(j,j+t), t=1..ku | 1/U(j,j) | gf[j][0..kl) | hb[j][0..ku)]
+    // and for A^T (packt)
+    //   [U(j-t,j), t=1..ku | L(j+t,j), t=1..kl | 1/U(j,j) | gft[j][0..ku) | hbt[j][0..kl)]
+    // with zeros where the band leaves the matrix.
+    std::vector<double> pack, packt;
 };
 bool segment_lines(const line_factors& f, line_segments& out);
 

@@ -27,10 +27,8 @@ struct dev_slot {
     int* poff = nullptr;
     // segmented sweep (unpivoted factors only)
     adsg::line_segments seg;
-    void* gf = nullptr;
-    void* hb = nullptr;
-    void* gft = nullptr;  // transposed solve
-    void* hbt = nullptr;
+    void* pack = nullptr;   // packed rows of the segmented sweep
+    void* packt = nullptr;  // the same for the transposed solve
     double* arow = nullptr;  // FP64 rows of the unfactored matrix, for refinement
     // lines > 0: one matrix per line (generalized ADS), ab holds them all in the
     // reference's line order; the device factors are interleaved by line (own_factors)
