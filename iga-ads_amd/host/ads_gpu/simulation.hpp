@@ -177,6 +177,31 @@ protected:
     void advance(int nsteps, const rhs_form& f, bool dirichlet = false) {
         check(adsg_step(h_, nsteps, &f, dirichlet ? ADSG_STEP_DIRICHLET : 0), "step");
     }
+    // nsteps of the implicit step u += A^-1 r(u), A_d = the matrix in slots[d]
+    void advance_delta(int nsteps, const rhs_form& f, const std::array<int, 3>& slots,
+                       bool refine = false, bool dirichlet = false) {
+        int flags = (dirichlet ? ADSG_STEP_DIRICHLET : 0) | (refine ? ADSG_STEP_REFINE : 0);
+        check(adsg_step_delta(h_, nsteps, &f, slots.data(), flags), "step_delta");
+    }
+    // dgbtrs 'T' per direction (adjoint steps)
+    void ads_solve_transposed(const std::array<int, 3>& slots) {
+        check(adsg_ads_solve_transposed(h_, slots.data()), "ads_solve_transposed");
+    }
+    // generalized ADS: one band matrix per line of direction dir (solver.hpp:43-195)
+    void set_line_matrices(int dir, int slot, int kl, int ku, const std::vector<double>& ab) {
+        check(adsg_set_line_matrices(h_, dir, slot, kl, ku, ab.data()), "set_line_matrices");
+    }
+    // normL2 / normH1 (basic_simulation_2d.hpp:291-316) of u, on the device
+    double normL2() {
+        double v = 0;
+        check(adsg_norm(h_, ADSG_U, ADSG_NORM_L2, &v), "norm");
+        return v;
+    }
+    double normH1() {
+        double v = 0;
+        check(adsg_norm(h_, ADSG_U, ADSG_NORM_H1, &v), "norm");
+        return v;
+    }
 
     void set_vector(int which, const std::vector<double>& v) {
         if (static_cast<long>(v.size()) != size()) throw error(ADSG_EINVAL, "set_vector: wrong size");

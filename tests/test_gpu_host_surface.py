@@ -60,3 +60,33 @@ def test_example_program_reports_bad_configuration(tmp_path):
     r = subprocess.run([EXE, "3", "7", "4", "1", "1e-4", "0", "fp64", str(tmp_path / "in.bin"),
                         str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "degree" in r.stderr
+
+
+def test_implicit_example_program_matches_oracle(tmp_path):
+    """implicit_gpu: the implicit heat step with source through the C++ surface
+    (advance_delta), FP64 against the oracle's delta steps, FP32 + refinement within 1e-6."""
+    from adsgpu import workloads as wl
+
+    exe = os.path.join(ROOT, "iga-ads_amd", "examples", "bin", "implicit_gpu")
+    p, ne, steps, dt = 3, 16, 4, 1e-4
+    u0 = wl.gaussians([p] * 3, [ne] * 3, seed=12, count=3, noise=1e-3)
+    f = wl.sine_source([p] * 3, [ne] * 3, seed=13)
+    u0.astype("<f8").tofile(tmp_path / "u0.bin")
+    f.astype("<f8").tofile(tmp_path / "f.bin")
+    o = po.Sim(3, [p] * 3, [ne] * 3)
+    for d in range(3):
+        o.set_matrix(d, 1, 1.0, 0.5 * dt)
+    o.prepare_matrices()
+    form = po.heat_form(dt, 3, source=f)
+    form.mass = 0.0
+    ref = u0.copy()
+    for _ in range(steps):
+        ref = ref + o.solve(o.compute_rhs(ref, form, threads=8), slots=[1, 1, 1])
+    for prec, refine, tol in (("fp64", 0, 1e-12), ("fp32", 1, 1e-6)):
+        print(f"case: implicit_gpu p={p} ne={ne} {prec} refine={refine}", flush=True)
+        r = subprocess.run([exe, str(p), str(ne), str(steps), repr(dt), prec, str(refine),
+                            str(tmp_path / "u0.bin"), str(tmp_path / "f.bin"), str(tmp_path / "out.bin")],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        got = np.fromfile(tmp_path / "out.bin", dtype="<f8")
+        assert po.rel_l2(got, ref) <= tol, prec
