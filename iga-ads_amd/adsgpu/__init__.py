@@ -375,11 +375,13 @@ class Batch:
     def synchronize(self):
         self._check(lib().adsg_batch_synchronize(self.h), "batch_synchronize")
 
-    def set_tuning(self, value):
-        self._check(lib().adsg_batch_set_tuning(self.h, TUNE_BATCH_TILE, value), "batch_set_tuning")
+    def set_tuning(self, value, key=TUNE_BATCH_TILE):
+        """TUNE_BATCH_TILE: element tile (0 = 16x16, 1 = 8x8); TUNE_SWEEP_VARIANT: 0 =
+        segmented line sweeps, 1 = one lane per line."""
+        self._check(lib().adsg_batch_set_tuning(self.h, key, value), "batch_set_tuning")
 
-    def get_tuning(self):
-        return lib().adsg_batch_get_tuning(self.h, TUNE_BATCH_TILE)
+    def get_tuning(self, key=TUNE_BATCH_TILE):
+        return lib().adsg_batch_get_tuning(self.h, key)
 
     def autotune(self):
         ms = np.zeros(2)

@@ -32,6 +32,12 @@ struct batch_group {
     int sx_blocks = 0;
     void* sy_items = nullptr;  // y lines
     int sy_blocks = 0;
+    // segmented sweeps (when every problem's lines split into segments)
+    bool seg = false;
+    void* gx_items = nullptr;  // seg_item<T>[nitems], x lines
+    int gx_blocks = 0;
+    void* gy_items = nullptr;
+    int gy_blocks = 0;
 };
 
 }  // namespace
@@ -51,7 +57,9 @@ struct adsg_batch {
     std::vector<adsg_rhs_form> forms;
     bool prepared = false;
     int shape = 0;  // element tile shape: 0 = 16 x 16, 1 = 8 x 8
+    int sweep = 0;  // 0: segmented line sweeps where every problem allows, 1: one lane per line
     std::vector<batch_group> groups[2];
+    void* packs = nullptr;  // packed sweep rows of every problem and direction (step precision)
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (nsteps, phase at start)
     std::string err;
 };
@@ -70,10 +78,12 @@ int bhip(adsg_batch* b, hipError_t e, const char* what) {
 void free_groups(adsg_batch* b) {
     for (auto& gs : b->groups) {
         for (auto& g : gs)
-            for (void* p : {g.rhs_items, g.sx_items, g.sy_items})
+            for (void* p : {g.rhs_items, g.sx_items, g.sy_items, g.gx_items, g.gy_items})
                 if (p) (void)hipFree(p);
         gs.clear();
     }
+    if (b->packs) (void)hipFree(b->packs);
+    b->packs = nullptr;
 }
 
 void drop_batch_graphs(adsg_batch* b) {
@@ -121,10 +131,36 @@ template <typename T>
 int build_groups(adsg_batch* b) {
     free_groups(b);
     drop_batch_graphs(b);
+    // batched segmentation of every problem's two Gram matrices, packed rows in one arena
+    const size_t P = b->sims.size();
+    std::vector<adsg::line_segments> segs(2 * P);
+    std::vector<size_t> pack_off(2 * P, 0);
+    bool all_seg = true;
+    size_t total_pack = 0;
+    for (size_t i = 0; i < P && all_seg; ++i)
+        for (int d = 0; d < 2; ++d) {
+            auto& sg = segs[2 * i + d];
+            if (!adsg::segment_lines(b->sims[i]->dirs[d].slots[0].lf, sg, adsg::kBatchSegments)) {
+                all_seg = false;
+                break;
+            }
+            pack_off[2 * i + d] = total_pack;
+            total_pack += sg.pack.size();
+        }
+    if (all_seg) {
+        std::vector<T> arena(std::max<size_t>(total_pack, 1));
+        for (size_t k = 0; k < 2 * P; ++k)
+            for (size_t j = 0; j < segs[k].pack.size(); ++j) arena[pack_off[k] + j] = static_cast<T>(segs[k].pack[j]);
+        if (hipMalloc(&b->packs, sizeof(T) * arena.size()) != hipSuccess)
+            return bfail(b, ADSG_ENOMEM, "batch: no memory for the sweep rows");
+        hipError_t e = hipMemcpy(b->packs, arena.data(), sizeof(T) * arena.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bhip(b, e, "batch: uploading sweep rows");
+    }
     for (int ph = 0; ph < 2; ++ph) {
         for (int p = 1; p <= 3; ++p) {
             std::vector<adsg::rhs2_item<T>> ri;
             std::vector<adsg::sweep_item<T>> sx, sy;
+            std::vector<adsg::seg_item<T>> gx, gy;
             batch_group g;
             g.p = p;
             for (size_t i = 0; i < b->sims.size(); ++i) {
@@ -175,6 +211,31 @@ int build_groups(adsg_batch* b) {
                 g.sy_blocks += static_cast<int>((nx + 63) / 64);
                 sx.push_back(xi);
                 sy.push_back(yi);
+                if (all_seg) {
+                    adsg::seg_item<T> a, c;
+                    const auto& sgx = segs[2 * i];
+                    const auto& sgy = segs[2 * i + 1];
+                    a.u = c.u = target;
+                    a.g.n_lines = ny;  // x line l starts at l * nx
+                    a.g.A = 1;
+                    a.g.B = nx;
+                    a.g.S = 1;
+                    a.pack = static_cast<const T*>(b->packs) + pack_off[2 * i];
+                    a.n = static_cast<int>(nx);
+                    a.m = sgx.m;
+                    a.nseg = sgx.nseg;
+                    a.first = g.gx_blocks;
+                    g.gx_blocks += static_cast<int>((ny + 63) / 64);
+                    c.g = yi.g;
+                    c.pack = static_cast<const T*>(b->packs) + pack_off[2 * i + 1];
+                    c.n = static_cast<int>(ny);
+                    c.m = sgy.m;
+                    c.nseg = sgy.nseg;
+                    c.first = g.gy_blocks;
+                    g.gy_blocks += static_cast<int>((nx + 63) / 64);
+                    gx.push_back(a);
+                    gy.push_back(c);
+                }
             }
             if (ri.empty()) continue;
             g.nitems = static_cast<int>(ri.size());
@@ -184,6 +245,11 @@ int build_groups(adsg_batch* b) {
             if ((rc = upload_items(b, &G.rhs_items, ri))) return rc;
             if ((rc = upload_items(b, &G.sx_items, sx))) return rc;
             if ((rc = upload_items(b, &G.sy_items, sy))) return rc;
+            G.seg = all_seg;
+            if (all_seg) {
+                if ((rc = upload_items(b, &G.gx_items, gx))) return rc;
+                if ((rc = upload_items(b, &G.gy_items, gy))) return rc;
+            }
         }
     }
     return ADSG_OK;
@@ -211,13 +277,21 @@ int batch_step_impl(adsg_batch* b) {
             if (e != hipSuccess) return bhip(b, e, "batch: element kernels");
         }
     for (const batch_group& g : b->groups[ph]) {
-        e = adsg::launch_sweep_batch<T>(g.p, g.p, static_cast<const adsg::sweep_item<T>*>(g.sx_items), g.nitems,
-                                        g.sx_blocks, b->stream);
+        if (g.seg && b->sweep == 0)
+            e = adsg::launch_sweep_seg_batch<T>(g.p, g.p, static_cast<const adsg::seg_item<T>*>(g.gx_items),
+                                                g.nitems, g.gx_blocks, b->stream);
+        else
+            e = adsg::launch_sweep_batch<T>(g.p, g.p, static_cast<const adsg::sweep_item<T>*>(g.sx_items),
+                                            g.nitems, g.sx_blocks, b->stream);
         if (e != hipSuccess) return bhip(b, e, "batch: x sweeps");
     }
     for (const batch_group& g : b->groups[ph]) {
-        e = adsg::launch_sweep_batch<T>(g.p, g.p, static_cast<const adsg::sweep_item<T>*>(g.sy_items), g.nitems,
-                                        g.sy_blocks, b->stream);
+        if (g.seg && b->sweep == 0)
+            e = adsg::launch_sweep_seg_batch<T>(g.p, g.p, static_cast<const adsg::seg_item<T>*>(g.gy_items),
+                                                g.nitems, g.gy_blocks, b->stream);
+        else
+            e = adsg::launch_sweep_batch<T>(g.p, g.p, static_cast<const adsg::sweep_item<T>*>(g.sy_items),
+                                            g.nitems, g.sy_blocks, b->stream);
         if (e != hipSuccess) return bhip(b, e, "batch: y sweeps");
     }
     if constexpr (sizeof(T) == sizeof(float)) {
@@ -422,16 +496,23 @@ int adsg_batch_synchronize(adsg_batch* b) {
 }
 
 int adsg_batch_set_tuning(adsg_batch* b, int key, int value) {
-    if (!b || key != ADSG_TUNE_BATCH_TILE || value < 0 || value > 1)
+    if (!b || (key != ADSG_TUNE_BATCH_TILE && key != ADSG_TUNE_SWEEP_VARIANT) || value < 0 || value > 1)
         return bfail(b, ADSG_EINVAL, "batch_set_tuning: unknown key or value");
+    if (key == ADSG_TUNE_SWEEP_VARIANT) {
+        if (b->sweep != value) drop_batch_graphs(b);
+        b->sweep = value;
+        return ADSG_OK;
+    }
     if (b->shape == value) return ADSG_OK;
     b->shape = value;
     return b->prepared ? build(b) : ADSG_OK;
 }
 
 int adsg_batch_get_tuning(const adsg_batch* b, int key) {
-    if (!b || key != ADSG_TUNE_BATCH_TILE) return ADSG_EINVAL;
-    return b->shape;
+    if (!b) return ADSG_EINVAL;
+    if (key == ADSG_TUNE_BATCH_TILE) return b->shape;
+    if (key == ADSG_TUNE_SWEEP_VARIANT) return b->sweep;
+    return ADSG_EINVAL;
 }
 
 int adsg_batch_autotune(adsg_batch* b, double* ms_out) {

@@ -121,6 +121,22 @@ template <typename T>
 hipError_t launch_sweep_batch(int kl, int ku, const sweep_item<T>* items, int nitems, int blocks,
                               hipStream_t s);
 
+// Batched segmented sweeps (unpivoted factors): one item per (problem, direction) with its
+// packed rows (line_segments::pack, segmented for at most kBatchSegments segments), 64
+// lines per block of kBatchSegments waves; first = first block of the item.
+constexpr int kBatchSegments = 9;
+template <typename T>
+struct seg_item {
+    T* u = nullptr;
+    line_geometry g;
+    const T* pack = nullptr;
+    int n = 0, m = 0, nseg = 0;
+    int first = 0;
+};
+template <typename T>
+hipError_t launch_sweep_seg_batch(int kl, int ku, const seg_item<T>* items, int nitems, int blocks,
+                                  hipStream_t s);
+
 // d/dk of w^T rhs(u; k) for the gradient term of prm, added into out:
 //   out_b += -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d (d_d u)(d_d w)
 template <typename T>

@@ -286,20 +286,32 @@ int factor_for_lines(const std::vector<double>& ab_in, int n, int kl, int ku, li
     return info;
 }
 
-bool segment_lines(const line_factors& f, line_segments& out) {
+bool segment_lines(const line_factors& f, line_segments& out, int max_nseg) {
     const int n = f.n, kl = f.kl, ku = f.ku_eff;
     const int km = std::max(kl, ku);
     if (f.pivoted || kl < 1 || ku < 1 || kl != ku) return false;
     int mseg = 0, nseg = 0, m = 0;
-    for (int cap : {16, 32}) {
-        int s = (n + cap - 1) / cap;
-        if (s < 2 || s > 16) continue;
-        int mm = (n + s - 1) / s;  // balanced segments
-        if (mm > cap || n - (s - 1) * mm < km || mm < km) continue;
-        mseg = cap;
-        nseg = s;
-        m = mm;
-        break;
+    if (max_nseg > 0) {
+        // as many segments as allowed (batched sweeps: a fixed number of waves per line
+        // group), of at most 32 rows, the last one at least km rows long
+        for (int s = max_nseg; s >= 2 && !mseg; --s) {
+            int mm = (n + s - 1) / s;
+            if (mm > 32 || mm < km || n - (s - 1) * mm < km) continue;
+            mseg = 32;
+            nseg = s;
+            m = mm;
+        }
+    } else {
+        for (int cap : {16, 32}) {
+            int s = (n + cap - 1) / cap;
+            if (s < 2 || s > 16) continue;
+            int mm = (n + s - 1) / s;  // balanced segments
+            if (mm > cap || n - (s - 1) * mm < km || mm < km) continue;
+            mseg = cap;
+            nseg = s;
+            m = mm;
+            break;
+        }
     }
     if (!mseg) return false;
     out.m = m;

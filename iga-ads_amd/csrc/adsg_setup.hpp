@@ -76,6 +76,8 @@ struct line_segments {
     // with zeros where the band leaves the matrix.
     std::vector<double> pack, packt;
 };
-bool segment_lines(const line_factors& f, line_segments& out);
+// max_nseg > 0: the largest segment count <= max_nseg with segments of <= 32 rows (the
+// batched sweep's fixed waves per workgroup); otherwise 2..16 segments of <= 16 (or 32) rows.
+bool segment_lines(const line_factors& f, line_segments& out, int max_nseg = 0);
 
 }  // namespace adsg

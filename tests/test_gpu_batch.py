@@ -35,11 +35,12 @@ def _u0(i):
     return wl.smooth_unit(p, ne, seed=100 + i)
 
 
-def _batch(precision, shape=0):
+def _batch(precision, shape=0, sweep=0):
     b = ag.Batch(PROBLEMS, precision=precision)
     b.prepare()
     b.set_forms(_forms())
     b.set_tuning(shape)
+    b.set_tuning(sweep, ag.TUNE_SWEEP_VARIANT)
     for i in range(len(PROBLEMS)):
         b.set(i, _u0(i))
     return b
@@ -56,10 +57,10 @@ def _oracle(i, steps):
     return u
 
 
-@pytest.mark.parametrize("shape", [0, 1])
-def test_fp64_batch_matches_oracle_per_problem(shape):
-    _log(f"case: FP64 batch of {len(PROBLEMS)} problems, tile shape {shape}, 10 steps")
-    b = _batch(ag.FP64, shape)
+@pytest.mark.parametrize("shape,sweep", [(0, 0), (1, 0), (0, 1)])
+def test_fp64_batch_matches_oracle_per_problem(shape, sweep):
+    _log(f"case: FP64 batch of {len(PROBLEMS)} problems, tile shape {shape}, sweep {sweep}, 10 steps")
+    b = _batch(ag.FP64, shape, sweep)
     b.step(10)
     worst = 0.0
     for i in range(len(PROBLEMS)):
@@ -84,7 +85,7 @@ def test_fp32_batch_against_fp64_oracle():
 @pytest.mark.parametrize("precision", [ag.FP64, ag.FP32])
 def test_batch_equals_single_problem_path_bitwise(precision):
     _log(f"case: batch vs one-by-one, precision {precision}")
-    b = _batch(precision, 0)
+    b = _batch(precision, 0, sweep=1)  # one lane per line, as the single path below
     b.step(6)
     for i in (0, 2, 5, 11):
         p, ne = PROBLEMS[i]
