@@ -185,6 +185,73 @@ def ensemble_block(ag, np, count, steps, dt):
     return res
 
 
+def adjoint_k_block(ag, np, ne, steps, every, dt):
+    """2D inverse-problem shape: u_t = div(k grad u), p=2, ne^2, Dirichlet, k in [0.5, 2];
+    forward steps and the checkpointed adjoint of ||u(T) - u_target||^2 w.r.t. k and u(0)."""
+    from adsgpu import workloads as wl
+
+    p = 2
+    def kfield(seed):
+        g = wl.sine_source([p, p], [ne, ne], seed=seed)
+        return np.exp(g / np.abs(g).max() * np.log(2.0))
+
+    k, kt = kfield(2030), kfield(2031)
+    sim = ag.Simulation(2, [p, p], [ne, ne])
+    sim.fix_all_boundaries()
+    sim.prepare_matrices()
+    N = sim.size
+    u0 = wl.gaussians([p, p], [ne, ne], seed=2032, count=1, sigma=(0.1, 0.1), noise=1e-3)
+    nx = ne + p
+    u0.reshape(nx, nx)[[0, -1], :] = 0.0
+    u0.reshape(nx, nx)[:, [0, -1]] = 0.0
+    form = ag.rhs_form(dt, kfield=True)
+    flags = ag.STEP_DIRICHLET
+
+    def forward(kv, u):
+        sim.set(ag.KFIELD, kv)
+        sim.set(ag.U, u)
+        sim.step(steps, form, flags)
+        return sim.get(ag.U)
+
+    ut = forward(kt, u0)
+    forward(k, u0)  # graph capture and code load
+    sim.set(ag.U, u0)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps, form, flags)
+    sim.synchronize()
+    tf = time.perf_counter() - t0
+    uT = sim.get(ag.U)
+    sim.set(ag.U, u0)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.forward_checkpointed(steps, every, form, flags)
+    sim.synchronize()
+    tc = time.perf_counter() - t0
+    sim.set(ag.GRAD_K, np.zeros(N))
+    sim.set(ag.U, 2.0 * (uT - ut))
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.adjoint_checkpointed(form, flags)
+    sim.synchronize()
+    ta = time.perf_counter() - t0
+    dk = sim.get(ag.GRAD_K)
+    d = np.random.default_rng(2033).standard_normal(N)
+    h = 1e-6
+    loss = lambda kv: float(((forward(kv, u0) - ut) ** 2).sum())
+    fd = (loss(k + h * d) - loss(k - h * d)) / (2 * h)
+    ad = float(dk @ d)
+    sim.close()
+    return {"workload": (f"2D heat u_t = div(k grad u), p=2, {ne}^2 elements ({N} DOFs), "
+                         f"{steps} steps of dt={dt}, Dirichlet, k seeded in [0.5, 2]; "
+                         f"loss ||u(T) - u_target||^2, checkpoints every {every} steps"),
+            "forward": {"value": N * steps / tf, "ms_per_step": 1e3 * tf / steps},
+            "forward_checkpointed_ms_per_step": 1e3 * tc / steps,
+            "adjoint": {"value": N * steps / ta, "ms_per_step": 1e3 * ta / steps,
+                        "note": "recompute of every segment from its checkpoint included"},
+            "fd_direction_rel_err": abs(fd - ad) / abs(ad), "fd_h": h}
+
+
 def f_elem_source(d, p):
     """Flops per element of the delta-form integrand dt (f v - grad u . grad v), 3D: the
     forward tree needs 8 one-dimensional contractions for grad u (value and derivative in x;
@@ -247,6 +314,7 @@ def main():
     ap.add_argument("--implicit-elements", type=int, default=192)
     ap.add_argument("--implicit-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble block")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D k-field adjoint block")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -350,6 +418,9 @@ def main():
     if not args.no_implicit:
         result.setdefault("forms", {})["implicit_p3"] = implicit_block(
             ag, np, args.implicit_elements, args.implicit_steps, 2)
+
+    if not args.no_adjoint:
+        result.setdefault("forms", {})["adjoint_k_2d"] = adjoint_k_block(ag, np, 128, 200, 20, 1e-6)
 
     if not args.no_batch:
         result.setdefault("forms", {})["batched_ensemble"] = ensemble_block(

@@ -63,6 +63,7 @@ EXPORTS = [
     "adsg_zero_boundary", "adsg_ads_solve", "adsg_ads_solve_transposed", "adsg_step",
     "adsg_step_delta", "adsg_adjoint_step", "adsg_adjoint_step_delta", "adsg_synchronize",
     "adsg_set_vector_device", "adsg_get_vector_device", "adsg_norm", "adsg_set_line_matrices",
+    "adsg_forward_checkpointed", "adsg_adjoint_checkpointed",
     "adsg_batch_create", "adsg_batch_destroy", "adsg_batch_last_error", "adsg_batch_count",
     "adsg_batch_size", "adsg_batch_dofs", "adsg_batch_prepare", "adsg_batch_set_forms",
     "adsg_batch_set_vector", "adsg_batch_get_vector", "adsg_batch_step", "adsg_batch_synchronize",
@@ -127,6 +128,8 @@ def lib():
         L.adsg_batch_autotune.argtypes = [C.c_void_p, _dp]
         L.adsg_set_vector_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_norm.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.adsg_forward_checkpointed.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(RhsForm), C.c_int]
+        L.adsg_adjoint_checkpointed.argtypes = [C.c_void_p, C.POINTER(RhsForm), C.c_int]
         L.adsg_set_line_matrices.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.adsg_get_vector_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_synchronize.argtypes = [C.c_void_p]
@@ -264,6 +267,16 @@ class Simulation:
         out = np.zeros(1)
         self._check(lib().adsg_norm(self.h, which, kind, out.ctypes.data_as(_dp)), "norm")
         return float(out[0])
+
+    def forward_checkpointed(self, n, every, form, flags=0):
+        """n explicit steps from u keeping a checkpoint every `every` steps (adjoint runs)."""
+        self._check(lib().adsg_forward_checkpointed(self.h, n, every, C.byref(form), flags),
+                    "forward_checkpointed")
+
+    def adjoint_checkpointed(self, form, flags=0):
+        """u = dL/du(T) in, dL/du(0) out; dL/dk added into GRAD_K (k-field forms)."""
+        self._check(lib().adsg_adjoint_checkpointed(self.h, C.byref(form), flags),
+                    "adjoint_checkpointed")
 
     def set_device(self, which, ptr):
         """Copy from FP64 device memory at ptr (e.g. a tensor's data_ptr())."""

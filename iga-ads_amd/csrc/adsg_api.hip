@@ -695,6 +695,8 @@ void adsg_destroy(adsg_sim* h) {
     for (auto& v : h->shadow)
         if (v) (void)hipFree(v);
     if (h->norm_scratch) (void)hipFree(h->norm_scratch);
+    if (h->ckpt) (void)hipFree(h->ckpt);
+    if (h->ck_seg) (void)hipFree(h->ck_seg);
     if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -974,6 +976,80 @@ int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
     if (form->use_kfield && !h->vec[ADSG_GRAD_K])
         return fail(h, ADSG_ESTATE, "adjoint_step: set ADSG_GRAD_K (the accumulator) first");
     return one_adjoint_step(h, form, flags);
+}
+
+int adsg_forward_checkpointed(adsg_sim* h, int nsteps, int every, const adsg_rhs_form* form, int flags) {
+    if (!h || !form || nsteps < 1 || every < 1) return fail(h, ADSG_EINVAL, "forward_checkpointed: bad argument");
+    if (!h->prepared) return fail(h, ADSG_ESTATE, "forward_checkpointed: prepare_matrices first");
+    if (h->precision != ADSG_FP64) return fail(h, ADSG_EINVAL, "forward_checkpointed: FP64 handles only");
+    const int nchk = (nsteps + every - 1) / every;
+    const size_t need = static_cast<size_t>(nchk) * h->N;
+    if (h->ckpt_cap < need) {
+        if (h->ckpt) (void)hipFree(h->ckpt);
+        h->ckpt = nullptr;
+        h->ckpt_cap = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&h->ckpt), need * sizeof(double)) != hipSuccess)
+            return fail(h, ADSG_ENOMEM, "forward_checkpointed: no memory for the checkpoints");
+        h->ckpt_cap = need;
+    }
+    for (int c = 0; c < nchk; ++c) {
+        HIP_TRY(h, hipMemcpyAsync(h->ckpt + static_cast<size_t>(c) * h->N, h->vec[ADSG_U],
+                                  h->N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        const int todo = std::min(every, nsteps - c * every);
+        int rc = run_graphs(h, EXPLICIT_STEP, todo, form, flags, nullptr);
+        if (rc) return rc;
+    }
+    h->ck_steps = nsteps;
+    h->ck_every = every;
+    h->ck_flags = flags;
+    h->ck_form = form_key(form);
+    return ADSG_OK;
+}
+
+int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags) {
+    if (!h || !form) return fail(h, ADSG_EINVAL, "adjoint_checkpointed: bad argument");
+    if (!h->ck_steps || h->ck_form != form_key(form) || h->ck_flags != flags)
+        return fail(h, ADSG_ESTATE, "adjoint_checkpointed: no forward_checkpointed run with this form and flags");
+    if (form->reaction != 0.0)
+        return fail(h, ADSG_EINVAL, "adjoint_checkpointed: the form must be linear in u (reaction = 0)");
+    if (form->use_kfield && !h->vec[ADSG_GRAD_K])
+        return fail(h, ADSG_ESTATE, "adjoint_checkpointed: set ADSG_GRAD_K (the accumulator) first");
+    const int nsteps = h->ck_steps, every = h->ck_every;
+    const int nchk = (nsteps + every - 1) / every;
+    const size_t need = static_cast<size_t>(every + 1) * h->N;  // segment states + lambda
+    if (h->ck_seg_cap < need) {
+        if (h->ck_seg) (void)hipFree(h->ck_seg);
+        h->ck_seg = nullptr;
+        h->ck_seg_cap = 0;
+        if (hipMalloc(reinterpret_cast<void**>(&h->ck_seg), need * sizeof(double)) != hipSuccess)
+            return fail(h, ADSG_ENOMEM, "adjoint_checkpointed: no memory for a segment");
+        h->ck_seg_cap = need;
+    }
+    const size_t bytes = h->N * sizeof(double);
+    double* lam_save = h->ck_seg + static_cast<size_t>(every) * h->N;
+    for (int c = nchk - 1; c >= 0; --c) {
+        const int start = c * every, stop = std::min(nsteps, start + every);
+        HIP_TRY(h, hipMemcpyAsync(lam_save, h->vec[ADSG_U], bytes, hipMemcpyDeviceToDevice, h->stream));
+        // recompute the segment's states u(start) .. u(stop-1) from its checkpoint
+        HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U], h->ckpt + static_cast<size_t>(c) * h->N, bytes,
+                                  hipMemcpyDeviceToDevice, h->stream));
+        for (int s = start; s < stop; ++s) {
+            HIP_TRY(h, hipMemcpyAsync(h->ck_seg + static_cast<size_t>(s - start) * h->N, h->vec[ADSG_U],
+                                      bytes, hipMemcpyDeviceToDevice, h->stream));
+            if (s + 1 < stop) {
+                int rc = run_graphs(h, EXPLICIT_STEP, 1, form, flags, nullptr);
+                if (rc) return rc;
+            }
+        }
+        HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U], lam_save, bytes, hipMemcpyDeviceToDevice, h->stream));
+        for (int s = stop - 1; s >= start; --s) {
+            HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U_PREV], h->ck_seg + static_cast<size_t>(s - start) * h->N,
+                                      bytes, hipMemcpyDeviceToDevice, h->stream));
+            int rc = one_adjoint_step(h, form, flags);
+            if (rc) return rc;
+        }
+    }
+    return ADSG_OK;
 }
 
 int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots,

@@ -70,6 +70,13 @@ struct adsg_sim {
     std::string err;
     hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
     double* norm_scratch = nullptr;  // adsg_norm: block partial sums and the total
+    // adsg_forward_checkpointed / adsg_adjoint_checkpointed: u every `ck_every` steps, and the
+    // recomputed states of one segment plus a slot for lambda
+    double* ckpt = nullptr;
+    double* ck_seg = nullptr;
+    size_t ckpt_cap = 0, ck_seg_cap = 0;  // in coefficients
+    int ck_steps = 0, ck_every = 0, ck_flags = 0;
+    std::string ck_form;
     // captured step graphs keyed by (form, flags, u pointer, steps in graph)
     std::map<std::tuple<std::string, int, void*, int>, hipGraphExec_t> graphs;
 };

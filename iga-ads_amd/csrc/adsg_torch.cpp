@@ -1,8 +1,9 @@
 // libtorch autograd Function over the ADS heat step with a B-spline diffusion field k:
 // u(T) = step^N(u0; k), each step the explicit step of adsg_step with the form
 // {mass 1, dt, k field} (validation.hpp:95-119 with k as in flow.hpp:73-100). The backward
-// pass is the hand-written discrete adjoint (adsg_adjoint_step) run over the same steps in
-// reverse, with the forward states recomputed from checkpoints kept every `every` steps.
+// pass is the hand-written discrete adjoint run over the same steps in reverse, with the
+// forward states recomputed from checkpoints kept every `every` steps
+// (adsg_forward_checkpointed / adsg_adjoint_checkpointed).
 //
 // Host code only (plain C++ over the C ABI); tensors are FP64 on the GPU and are exchanged
 // with the handle by device-to-device copies.
@@ -57,19 +58,13 @@ struct HeatK : public torch::autograd::Function<HeatK> {
         check_vector(u0, n, "u0");
         const int flags = dirichlet ? ADSG_STEP_DIRICHLET : 0;
         const adsg_rhs_form form = make_form(dt);
-        const int64_t nchk = (steps + every - 1) / every;
-        Tensor chk = torch::empty({nchk, n}, u0.options());
         device_sync();
         check(h, adsg_set_vector_device(h, ADSG_KFIELD, k.data_ptr<double>()), "set k");
         check(h, adsg_set_vector_device(h, ADSG_U, u0.data_ptr<double>()), "set u0");
-        for (int64_t c = 0; c < nchk; ++c) {
-            check(h, adsg_get_vector_device(h, ADSG_U, chk[c].data_ptr<double>()), "checkpoint");
-            const int64_t todo = std::min(every, steps - c * every);
-            check(h, adsg_step(h, static_cast<int>(todo), &form, flags), "step");
-        }
+        check(h, adsg_step(h, static_cast<int>(steps), &form, flags), "steps");
         Tensor uT = torch::empty_like(u0);
         check(h, adsg_get_vector_device(h, ADSG_U, uT.data_ptr<double>()), "get u(T)");
-        ctx->save_for_backward({k, chk});
+        ctx->save_for_backward({k, u0});
         ctx->saved_data["handle"] = handle;
         ctx->saved_data["steps"] = steps;
         ctx->saved_data["every"] = every;
@@ -78,40 +73,30 @@ struct HeatK : public torch::autograd::Function<HeatK> {
         return uT;
     }
 
+    // the forward is run again with checkpoints every `every` steps (the handle may have run
+    // other work since), then the adjoint recomputes each segment from its checkpoint
     static tensor_list backward(AutogradContext* ctx, tensor_list grads) {
         auto saved = ctx->get_saved_variables();
-        Tensor k = saved[0], chk = saved[1];
+        Tensor k = saved[0], u0 = saved[1];
         auto* h = reinterpret_cast<adsg_sim*>(ctx->saved_data["handle"].toInt());
-        const int64_t steps = ctx->saved_data["steps"].toInt();
-        const int64_t every = ctx->saved_data["every"].toInt();
+        const int steps = static_cast<int>(ctx->saved_data["steps"].toInt());
+        const int every = static_cast<int>(ctx->saved_data["every"].toInt());
         const double dt = ctx->saved_data["dt"].toDouble();
         const int flags = ctx->saved_data["dirichlet"].toBool() ? ADSG_STEP_DIRICHLET : 0;
         const adsg_rhs_form form = make_form(dt);
-        const long n = adsg_size(h);
-        Tensor lam = grads[0].to(torch::kFloat64).contiguous().clone();
+        Tensor lam = grads[0].to(torch::kFloat64).contiguous();
         Tensor dk = torch::zeros_like(k);
-        Tensor seg = torch::empty({every, n}, k.options());
+        Tensor du0 = torch::empty_like(u0);
         device_sync();
         check(h, adsg_set_vector_device(h, ADSG_KFIELD, k.data_ptr<double>()), "set k");
+        check(h, adsg_set_vector_device(h, ADSG_U, u0.data_ptr<double>()), "set u0");
+        check(h, adsg_forward_checkpointed(h, steps, every, &form, flags), "checkpointed forward");
         check(h, adsg_set_vector_device(h, ADSG_GRAD_K, dk.data_ptr<double>()), "clear dL/dk");
-        const int64_t nchk = chk.size(0);
-        for (int64_t c = nchk - 1; c >= 0; --c) {
-            const int64_t start = c * every, stop = std::min(steps, start + every);
-            // recompute the forward states of this segment from its checkpoint
-            check(h, adsg_set_vector_device(h, ADSG_U, chk[c].data_ptr<double>()), "restore");
-            for (int64_t s = start; s < stop; ++s) {
-                check(h, adsg_get_vector_device(h, ADSG_U, seg[s - start].data_ptr<double>()), "keep state");
-                if (s + 1 < stop) check(h, adsg_step(h, 1, &form, flags), "recompute");
-            }
-            for (int64_t s = stop - 1; s >= start; --s) {
-                check(h, adsg_set_vector_device(h, ADSG_U_PREV, seg[s - start].data_ptr<double>()), "u(n)");
-                check(h, adsg_set_vector_device(h, ADSG_U, lam.data_ptr<double>()), "lam(n+1)");
-                check(h, adsg_adjoint_step(h, &form, flags), "adjoint step");
-                check(h, adsg_get_vector_device(h, ADSG_U, lam.data_ptr<double>()), "lam(n)");
-            }
-        }
+        check(h, adsg_set_vector_device(h, ADSG_U, lam.data_ptr<double>()), "dL/du(T)");
+        check(h, adsg_adjoint_checkpointed(h, &form, flags), "adjoint");
         check(h, adsg_get_vector_device(h, ADSG_GRAD_K, dk.data_ptr<double>()), "dL/dk");
-        return {dk, lam, Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+        check(h, adsg_get_vector_device(h, ADSG_U, du0.data_ptr<double>()), "dL/du0");
+        return {dk, du0, Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
     }
 };
 

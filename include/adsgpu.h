@@ -187,6 +187,16 @@ int adsg_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const in
  * buffer is overwritten. No counterpart in the reference (it has no adjoint code). */
 int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags);
 
+/* Checkpoint-and-recompute for the explicit-step adjoint (FP64 handles). The forward call
+ * runs nsteps steps of adsg_step(form, flags) from u, keeping u every `every` steps in
+ * device memory held by the handle. The adjoint call takes u = dL/du(T), runs
+ * adsg_adjoint_step backwards over the same steps -- each segment of `every` steps is
+ * recomputed from its checkpoint -- and returns u = dL/du(0), adding dL/dk into
+ * ADSG_GRAD_K when the form uses the k field. u_prev and buffer are overwritten. No
+ * counterpart in the reference (it keeps no checkpoints and has no adjoint). */
+int adsg_forward_checkpointed(adsg_sim* h, int nsteps, int every, const adsg_rhs_form* form, int flags);
+int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags);
+
 /* nsteps of the adjoint of adsg_step_delta with the same form, slots and flags, applied to
  * u in place (no swap; u_prev and buffer are overwritten): for the step
  * u' = Z (u + A^-1 Z r(u)) with r(u) = R u + c linear (reaction = 0) and Z the boundary

@@ -161,3 +161,29 @@ def test_adjoint_step_rejects_bad_state():
         g.adjoint_step(ag.rhs_form(1e-5, kfield=True))
     with pytest.raises(ag.AdsgError):  # nonlinear form
         g.adjoint_step(ag.rhs_form(1e-5, reaction=1.0))
+
+
+def test_native_checkpointed_adjoint_equals_stepwise_adjoint():
+    """adsg_forward_checkpointed + adsg_adjoint_checkpointed against the same adjoint driven
+    step by step from the host (above): same kernels in the same order."""
+    p, ne, dt, steps, every = 2, (40, 36), 1e-5, 30, 7
+    _log(f"case: native checkpointed adjoint, {steps} steps every {every}")
+    g, o = _pair(p, ne, True)
+    flags = ag.STEP_DIRICHLET
+    k = _kfield(p, ne, seed=51)
+    u0 = wl.gaussians([p, p], list(ne), seed=52, count=2, sigma=(0.1, 0.15), noise=0.0)
+    o.zero_boundary(u0)
+    ut = np.zeros(g.size)
+    dk_ref, du0_ref, uN = _gpu_gradient(g, k, u0, ut, dt, steps, every, flags)
+    form = ag.rhs_form(dt, kfield=True)
+    g.set(ag.KFIELD, k)
+    g.set(ag.U, u0)
+    g.forward_checkpointed(steps, every, form, flags)
+    assert np.array_equal(g.get(ag.U), uN)
+    g.set(ag.GRAD_K, np.zeros(g.size))
+    g.set(ag.U, 2.0 * (uN - ut))
+    g.adjoint_checkpointed(form, flags)
+    assert po.rel_l2(g.get(ag.GRAD_K), dk_ref) <= 1e-13
+    assert po.rel_l2(g.get(ag.U), du0_ref) <= 1e-13
+    with pytest.raises(ag.AdsgError):  # a different form than the forward run's
+        g.adjoint_checkpointed(ag.rhs_form(2 * dt, kfield=True), flags)
