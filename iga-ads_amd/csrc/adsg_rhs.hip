@@ -25,6 +25,16 @@ namespace {
 // planes with LDS atomic adds: a wave's adds are executed in instruction order, so the sums
 // are deterministic, and waves never share a plane. One barrier later the planes of the
 // (P+1) waves that touch a DOF plane are added and the tile's block is added into the rhs.
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, 32 CUs each
+
+// block b (dispatched to XCD b % 8) -> tile index, a bijection of [0, n) that gives every
+// XCD a contiguous range of tiles
+__device__ __forceinline__ int xcd_contiguous(int b, int n) {
+    const int per = n / kXcds, rem = n % kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
 template <typename T, int P, int VAR>
 constexpr int rhs3_waves() {  // minimum waves per SIMD the register allocation must allow
     return VAR == 0 ? 1 : VAR == 1 ? 2 : P == 3 ? 2 : sizeof(T) == sizeof(float) ? 4 : 3;
@@ -49,7 +59,10 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     __shared__ double acc[TZ * M * PL];
 
     const int tid = threadIdx.x;
-    int t = blockIdx.x;
+    // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD gets a
+    // contiguous run of tiles (one slab of the domain), whose halos the next colour's launch
+    // then finds in that XCD's L2.
+    int t = xcd_contiguous(blockIdx.x, gridDim.x);
     const int tx = 2 * (t % ntx2) + (colour & 1);
     t /= ntx2;
     const int ty = 2 * (t % nty2) + ((colour >> 1) & 1);
@@ -487,7 +500,8 @@ template <typename T, int P, int TX, int TY, bool KF, bool SRC>
 __global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __restrict__ rhs,
                                                  dev_tables<T> X, dev_tables<T> Y,
                                                  rhs_params<T> prm, int colour, int ntx2) {
-    rhs2_tile<T, P, TX, TY, KF, SRC>(up, rhs, X, Y, prm, colour, blockIdx.x, ntx2);
+    rhs2_tile<T, P, TX, TY, KF, SRC>(up, rhs, X, Y, prm, colour, xcd_contiguous(blockIdx.x, gridDim.x),
+                                     ntx2);
 }
 
 // Many 2D problems of one degree in one launch: block b belongs to the last item whose
@@ -495,7 +509,7 @@ __global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __
 template <typename T, int P, int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_rhs2_batch(const rhs2_item<T>* __restrict__ items,
                                                        int nitems, int colour) {
-    const int b = blockIdx.x;
+    const int b = xcd_contiguous(blockIdx.x, gridDim.x);
     int lo = 0, hi = nitems - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) / 2;
