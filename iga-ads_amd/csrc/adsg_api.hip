@@ -809,7 +809,10 @@ int adsg_prepare_matrices(adsg_sim* h) {
                 if ((rc = upload<double>(h, &s.rdiag, s.lf.rdiag))) return rc;
             }
             s.seg = adsg::line_segments{};
-            if (adsg::segment_lines(s.lf, s.seg)) {
+            // LDS per segment wave (x-line staging 64 x (rows+1), chain and refinement halos)
+            // within 150 KB: FP64 14 waves of 16-row or 8 of 32-row segments, FP32 16 or 13
+            const bool f64 = h->precision == ADSG_FP64;
+            if (adsg::segment_lines(s.lf, s.seg, f64 ? 14 : 16, f64 ? 8 : 13)) {
                 if (h->precision == ADSG_FP32) {
                     if ((rc = upload<float>(h, &s.pack, s.seg.pack))) return rc;
                     if ((rc = upload<float>(h, &s.packt, s.seg.packt))) return rc;

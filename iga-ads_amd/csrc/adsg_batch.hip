@@ -140,7 +140,7 @@ int build_groups(adsg_batch* b) {
     for (size_t i = 0; i < P && all_seg; ++i)
         for (int d = 0; d < 2; ++d) {
             auto& sg = segs[2 * i + d];
-            if (!adsg::segment_lines(b->sims[i]->dirs[d].slots[0].lf, sg, adsg::kBatchSegments)) {
+            if (!adsg::segment_lines(b->sims[i]->dirs[d].slots[0].lf, sg, 0, adsg::kBatchSegments)) {
                 all_seg = false;
                 break;
             }

@@ -286,33 +286,27 @@ int factor_for_lines(const std::vector<double>& ab_in, int n, int kl, int ku, li
     return info;
 }
 
-bool segment_lines(const line_factors& f, line_segments& out, int max_nseg) {
+bool segment_lines(const line_factors& f, line_segments& out, int max_nseg16, int max_nseg32) {
     const int n = f.n, kl = f.kl, ku = f.ku_eff;
     const int km = std::max(kl, ku);
     if (f.pivoted || kl < 1 || ku < 1 || kl != ku) return false;
     int mseg = 0, nseg = 0, m = 0;
-    if (max_nseg > 0) {
-        // as many segments as allowed (batched sweeps: a fixed number of waves per line
-        // group), of at most 32 rows, the last one at least km rows long
-        for (int s = max_nseg; s >= 2 && !mseg; --s) {
-            int mm = (n + s - 1) / s;
-            if (mm > 32 || mm < km || n - (s - 1) * mm < km) continue;
-            mseg = 32;
-            nseg = s;
-            m = mm;
-        }
-    } else {
-        for (int cap : {16, 32}) {
-            int s = (n + cap - 1) / cap;
-            if (s < 2 || s > 16) continue;
-            int mm = (n + s - 1) / s;  // balanced segments
-            if (mm > cap || n - (s - 1) * mm < km || mm < km) continue;
-            mseg = cap;
-            nseg = s;
-            m = mm;
-            break;
-        }
-    }
+    // the fewest segments of at most `rows` rows (balanced lengths, the last one at least km
+    // rows long) within the caller's count limit: the serial chain between segments costs
+    // more than the extra waves of shorter segments gain (measured on 130-row lines: 9
+    // segments of 15 rows beat 13 of 10)
+    auto pick = [&](int max_s, int rows) {
+        if (mseg) return;
+        const int s = (n + rows - 1) / rows;
+        if (s < 2 || s > max_s) return;
+        const int mm = (n + s - 1) / s;
+        if (mm > rows || mm < km || n - (s - 1) * mm < km) return;
+        mseg = rows;
+        nseg = s;
+        m = mm;
+    };
+    pick(max_nseg16, 16);
+    pick(max_nseg32, 32);  // lines too long for segments of 16 rows
     if (!mseg) return false;
     out.m = m;
     out.nseg = nseg;

@@ -76,8 +76,9 @@ struct line_segments {
     // with zeros where the band leaves the matrix.
     std::vector<double> pack, packt;
 };
-// max_nseg > 0: the largest segment count <= max_nseg with segments of <= 32 rows (the
-// batched sweep's fixed waves per workgroup); otherwise 2..16 segments of <= 16 (or 32) rows.
-bool segment_lines(const line_factors& f, line_segments& out, int max_nseg = 0);
+// The largest segment count <= max_nseg16 with segments of <= 16 rows, else the largest
+// <= max_nseg32 with segments of <= 32 rows (the callers' limits keep the kernels' LDS within
+// budget).
+bool segment_lines(const line_factors& f, line_segments& out, int max_nseg16, int max_nseg32);
 
 }  // namespace adsg
