@@ -319,6 +319,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain D2D copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--eager-only", action="store_true",
+                    help="only run `steps` eager FP64 steps of the 3D p=2 workload (no graph "
+                         "replay; for counter collection) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, 4)
@@ -339,6 +342,22 @@ def main():
                       centre=(0.2, 0.8), noise=1e-3)
     N = u0.size
     E = ne ** 3
+
+    if args.eager_only:
+        sim = ag.Simulation(3, [2, 2, 2], [ne, ne, ne], precision=ag.FP64)
+        sim.prepare_matrices()
+        sim.set(ag.U, u0)
+        form = ag.rhs_form(dt)
+        variant = args.rhs_variant
+        if variant is None:
+            variant = sim.autotune(form)
+        else:
+            sim.set_tuning(ag.TUNE_RHS_VARIANT, variant)
+        phases = sim.time_phases(args.steps, form)
+        sim.close()
+        print(json.dumps({"eager_steps": args.steps, "rhs_variant": variant, "phases_ms": phases}),
+              flush=True)
+        return
 
     t64, ph64, x64, var64 = run_form(ag, ag.FP64, ne, args.steps, args.warmup, u0, dt,
                                      args.rhs_variant)
