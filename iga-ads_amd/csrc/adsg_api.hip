@@ -200,6 +200,41 @@ int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0) {
 
 // delta: the integrand exactly as given, from u_prev (its FP32 shadow in FP32 mode) into
 // buffer; otherwise compute_rhs semantics (FP64: into u; FP32: rhs - M u_prev into buffer)
+// b_i = sum_e sum_q w J f(x_q) B_i(x_q) for the source f: the element kernel with the
+// form {mass 0, dt 1, no gradient, source f}, in the step precision
+template <typename T>
+int source_load_impl(adsg_sim* h) {
+    if (!h->source_load && hipMalloc(&h->source_load, h->N * sizeof(T)) != hipSuccess)
+        return fail(h, ADSG_ENOMEM, "source: no memory for its load vector");
+    adsg::dev_tables<T> t[3];
+    for (int d = 0; d < h->dim; ++d) {
+        t[d].val = static_cast<const T*>(h->dirs[d].val);
+        t[d].der = static_cast<const T*>(h->dirs[d].der);
+        t[d].wj = static_cast<const T*>(h->dirs[d].wj);
+        t[d].elements = h->dirs[d].tab.elements;
+        t[d].dofs = h->dirs[d].tab.dofs;
+    }
+    adsg::rhs_params<T> prm;
+    prm.mass = T(0);
+    prm.dt = T(1);
+    prm.gx = prm.gy = prm.gz = T(0);
+    prm.source = static_cast<const T*>(h->vec[ADSG_SOURCE]);
+    T* b = static_cast<T*>(h->source_load);
+    hipError_t e = adsg::launch_fill<T>(b, h->N, T(0), h->stream);
+    if (e == hipSuccess)
+        e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, prm.source, b, t, prm, h->rhs_variant, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "source load vector");
+    h->source_load_ok = true;
+    return ADSG_OK;
+}
+
+// called before any step that may be captured: the load vector is computed eagerly
+int ensure_source_load(adsg_sim* h, const adsg_rhs_form* form) {
+    if (!form->use_source || h->source_load_ok) return ADSG_OK;
+    if (!h->vec[ADSG_SOURCE]) return fail(h, ADSG_ESTATE, "compute_rhs: source not set");
+    return h->precision == ADSG_FP32 ? source_load_impl<float>(h) : source_load_impl<double>(h);
+}
+
 template <typename T>
 int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
     adsg::dev_tables<T> t[3];
@@ -221,10 +256,10 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
         if (!h->vec[ADSG_KFIELD]) return fail(h, ADSG_ESTATE, "compute_rhs: k field not set");
         prm.kfield = static_cast<const T*>(h->vec[ADSG_KFIELD]);
     }
-    if (form->use_source) {
-        if (!h->vec[ADSG_SOURCE]) return fail(h, ADSG_ESTATE, "compute_rhs: source not set");
-        prm.source = static_cast<const T*>(h->vec[ADSG_SOURCE]);
-    }
+    // the source term dt (f, v) is linear in f and independent of u: it enters as dt * b,
+    // b the source's load vector (ensure_source_load), and the element kernel runs without it
+    if (form->use_source && (!h->vec[ADSG_SOURCE] || !h->source_load_ok))
+        return fail(h, ADSG_ESTATE, "compute_rhs: source not set");
     T* rhs;
     const T* up;
     if constexpr (sizeof(T) == sizeof(float)) {
@@ -236,8 +271,10 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
         rhs = static_cast<T*>(h->vec[delta ? ADSG_BUFFER : ADSG_U]);
         up = static_cast<const T*>(h->vec[ADSG_U_PREV]);
     }
-    hipError_t e = adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
-    if (e != hipSuccess) return hip_fail(h, e, "zero rhs");
+    hipError_t e = form->use_source ? adsg::launch_scale<T>(rhs, static_cast<const T*>(h->source_load),
+                                                             h->N, prm.dt, h->stream)
+                                    : adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
+    if (e != hipSuccess) return hip_fail(h, e, "rhs start");
     mark(h, 1);
     e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->rhs_variant, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
@@ -576,6 +613,7 @@ int run_graphs(adsg_sim* h, step_kind kind, int nsteps, const adsg_rhs_form* for
                const int* slots) {
     hipGraphExec_t g2 = nullptr, g1 = nullptr;
     int rc;
+    if (kind != ADJOINT_DELTA_STEP && (rc = ensure_source_load(h, form))) return rc;  // not captured
     if (nsteps >= 2) {
         if ((rc = get_graph(h, kind, form, flags, 2, slots, &g2))) return rc;
         for (int i = 0; i < nsteps / 2; ++i) HIP_TRY(h, hipGraphLaunch(g2, h->stream));
@@ -695,6 +733,7 @@ void adsg_destroy(adsg_sim* h) {
     for (auto& v : h->shadow)
         if (v) (void)hipFree(v);
     if (h->norm_scratch) (void)hipFree(h->norm_scratch);
+    if (h->source_load) (void)hipFree(h->source_load);
     if (h->ckpt) (void)hipFree(h->ckpt);
     if (h->ck_seg) (void)hipFree(h->ck_seg);
     if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
@@ -861,6 +900,7 @@ int adsg_prepare_matrices(adsg_sim* h) {
 
 int adsg_set_vector(adsg_sim* h, int which, const double* host) {
     if (!h || !host || which < ADSG_U || which > ADSG_GRAD_K) return fail(h, ADSG_EINVAL, "set_vector: bad argument");
+    if (which == ADSG_SOURCE) h->source_load_ok = false;
     const size_t es = elt_of(h, which);
     if (!h->vec[which]) {
         if (hipMalloc(&h->vec[which], h->N * es) != hipSuccess)
@@ -900,12 +940,14 @@ int adsg_get_vector(adsg_sim* h, int which, double* host) {
 
 void* adsg_device_ptr(adsg_sim* h, int which) {
     if (!h || which < ADSG_U || which > ADSG_GRAD_K) return nullptr;
+    if (which == ADSG_SOURCE) h->source_load_ok = false;  // the caller may write through it
     return h->vec[which];
 }
 
 int adsg_set_vector_device(adsg_sim* h, int which, const double* dev) {
     if (!h || !dev || which < ADSG_U || which > ADSG_GRAD_K)
         return fail(h, ADSG_EINVAL, "set_vector_device: bad argument");
+    if (which == ADSG_SOURCE) h->source_load_ok = false;
     if (elt_of(h, which) != sizeof(double))
         return fail(h, ADSG_EINVAL, "set_vector_device: FP32 vectors are set from the host");
     if (!h->vec[which]) {
@@ -941,6 +983,7 @@ int adsg_swap(adsg_sim* h) {
 
 int adsg_compute_rhs(adsg_sim* h, const adsg_rhs_form* form) {
     if (!h || !form) return fail(h, ADSG_EINVAL, "compute_rhs: null argument");
+    if (int rc = ensure_source_load(h, form)) return rc;
     return rhs_dispatch(h, form);
 }
 
@@ -1121,6 +1164,7 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
     if (!h || !form || !ms || nsteps <= 0) return fail(h, ADSG_EINVAL, "time_phases: bad argument");
     if (!h->prepared) return fail(h, ADSG_ESTATE, "time_phases: prepare_matrices first");
     const step_kind kind = slots ? DELTA_STEP : EXPLICIT_STEP;
+    if (int rc = ensure_source_load(h, form)) return rc;
     hipEvent_t ev[8];
     for (auto& e : ev) HIP_TRY(h, hipEventCreate(&e));
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1195,7 +1239,7 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
     int rc = ADSG_OK;
     for (int v = 0; v <= 2 && rc == ADSG_OK; ++v) {
         h->rhs_variant = v;
-        h->sweep_variant = std::min(v, 1);
+        h->sweep_variant = v <= 1 ? v : best_sweep;  // both sweeps timed with variants 0, 1
         double ms[7];
         rc = adsg_time_phases(h, 1, form, slots, flags, ms);  // warm-up (code load, caches)
         if (!rc) rc = adsg_time_phases(h, 3, form, slots, flags, ms);

@@ -40,78 +40,50 @@ constexpr int rhs3_waves() {  // minimum waves per SIMD the register allocation 
     return VAR == 0 ? 1 : VAR == 1 ? 2 : P == 3 ? 2 : sizeof(T) == sizeof(float) ? 4 : 3;
 }
 
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX, int VAR>
-__global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
-    const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
-    dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
-    constexpr int M = P + 1, NT = TX * TY * TZ;
-    constexpr int BX = TX + P, BY = TY + P, BZ = TZ + P, NB = BX * BY * BZ, PL = BX * BY;
-    constexpr bool CREG = VAR == 0;   // coefficient block held in registers
-    constexpr bool FLUSH = VAR == 2;  // element contribution added to the planes per x point
-    static_assert(TX * TY == 64, "a wave must cover exactly one z-layer of the tile");
-    __shared__ T blk[NB];
-    __shared__ T kb[KF ? NB : 1];
-    __shared__ T sb[SRC ? NB : 1];
-    __shared__ T fvx[TX * M * M], fdx[TX * M * M], bvx[TX * M * M], bdx[TX * M * M];
-    __shared__ T fvy[TY * M * M], fdy[TY * M * M], bvy[TY * M * M], bdy[TY * M * M];
-    // wave lz owns planes lz*M .. lz*M+P (DOF z = lz + c); FP64 in both precisions (the FP64
-    // LDS add is the fast one on this part)
-    __shared__ double acc[TZ * M * PL];
-
-    const int tid = threadIdx.x;
-    // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD gets a
-    // contiguous run of tiles (one slab of the domain), whose halos the next colour's launch
-    // then finds in that XCD's L2.
-    int t = xcd_contiguous(blockIdx.x, gridDim.x);
+// first element of tile t of the given parity colour (ntx2, nty2 tiles of that colour per
+// tile row and tile plane)
+struct tile3 {
+    int ex0, ey0, ez0;
+};
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ tile3 rhs3_tile(int t, int colour, int ntx2, int nty2) {
     const int tx = 2 * (t % ntx2) + (colour & 1);
     t /= ntx2;
     const int ty = 2 * (t % nty2) + ((colour >> 1) & 1);
     t /= nty2;
     const int tz = 2 * t + ((colour >> 2) & 1);
-    const int ex0 = tx * TX, ey0 = ty * TY, ez0 = tz * TZ;
-    const int nx = X.dofs, ny = Y.dofs, nz = Z.dofs;
+    return {tx * TX, ty * TY, tz * TZ};
+}
 
-    // the tile's DOF block (first(e) = e for these knot vectors, so dof origin = element origin)
-    for (int i = tid; i < NB; i += NT) {
-        const int bx = i % BX, r = i / BX, by = r % BY, bz = r / BY;
-        const int gx = ex0 + bx, gy = ey0 + by, gz = ez0 + bz;
-        const bool in = gx < nx && gy < ny && gz < nz;
-        const long g = gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz);
-        blk[i] = in ? up[g] : T(0);
-        if constexpr (KF) kb[i] = in ? prm.kfield[g] : T(0);
-        if constexpr (SRC) sb[i] = in ? prm.source[g] : T(0);
-    }
-    const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy;
-    for (int i = tid; i < TX * M * M; i += NT) {
-        const int e = ex0 + i / (M * M), q = (i / M) % M;
-        const bool in = e < X.elements;
-        const T v = in ? X.val[static_cast<long>(ex0) * M * M + i] : T(0);
-        const T d = in ? X.der[static_cast<long>(ex0) * M * M + i] : T(0);
-        const T w = in ? X.wj[static_cast<long>(e) * M + q] : T(0);
-        fvx[i] = v;
-        fdx[i] = d;
-        bvx[i] = v * w;
-        bdx[i] = cx * (d * w);
-    }
-    for (int i = tid; i < TY * M * M; i += NT) {
-        const int e = ey0 + i / (M * M), q = (i / M) % M;
-        const bool in = e < Y.elements;
-        const T v = in ? Y.val[static_cast<long>(ey0) * M * M + i] : T(0);
-        const T d = in ? Y.der[static_cast<long>(ey0) * M * M + i] : T(0);
-        const T w = in ? Y.wj[static_cast<long>(e) * M + q] : T(0);
-        fvy[i] = v;
-        fdy[i] = d;
-        bvy[i] = v * w;
-        bdy[i] = cy * (d * w);
-    }
-    for (int i = tid; i < TZ * M * PL; i += NT) acc[i] = 0.0;
-    __syncthreads();
+// the LDS of one tile: its coefficient block (and k / f blocks), the x and y tables of its
+// elements, and the waves' private DOF planes (wave lz owns planes lz*M .. lz*M+P, DOF
+// z = lz + c; FP64 in both precisions: the FP64 LDS add is the fast one on this part)
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC>
+struct rhs3_lds {
+    static constexpr int M = P + 1, BX = TX + P, BY = TY + P, BZ = TZ + P;
+    static constexpr int NB = BX * BY * BZ, PL = BX * BY;
+    T blk[NB];
+    T kb[KF ? NB : 1];
+    T sb[SRC ? NB : 1];
+    T fvx[TX * M * M], fdx[TX * M * M], bvx[TX * M * M], bdx[TX * M * M];
+    T fvy[TY * M * M], fdy[TY * M * M], bvy[TY * M * M], bdy[TY * M * M];
+    double acc[TZ * M * PL];
+};
 
-    const int lx = tid % TX, ly = (tid / TX) % TY;
-    const int lz = __builtin_amdgcn_readfirstlane(tid / (TX * TY));
+// One element (lane lx, ly of wave lz) of the tile whose first element is (ex0, ey0, ez0):
+// sum-factorised integrand over its Gauss points, contribution added to the wave's planes.
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX, int VAR>
+__device__ __forceinline__ void rhs3_element(rhs3_lds<T, P, TX, TY, TZ, KF, SRC>& L,
+                                             const dev_tables<T>& X, const dev_tables<T>& Y,
+                                             const dev_tables<T>& Z, const rhs_params<T>& prm,
+                                             int lx, int ly, int lz, int ex0, int ey0, int ez0) {
+    using LDS = rhs3_lds<T, P, TX, TY, TZ, KF, SRC>;
+    constexpr int M = P + 1, BX = LDS::BX, BY = LDS::BY, PL = LDS::PL;
+    constexpr bool CREG = VAR == 0;   // coefficient block held in registers
+    constexpr bool FLUSH = VAR == 2;  // element contribution added to the planes per x point
     const int ez = ez0 + lz;
     // the wave's planes: in one add instruction the 64 lanes hit 64 distinct entries
-    double* mine = acc + lz * M * PL;
+    double* mine = L.acc + lz * M * PL;
 
     T R[FLUSH ? 1 : M][FLUSH ? 1 : M][FLUSH ? 1 : M];  // the element's contribution, (a, b, c)
 #pragma unroll
@@ -158,9 +130,9 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
 #pragma unroll
                         for (int c = 0; c < M; ++c) {
                             const int li = (lx + a) + BX * ((ly + b) + BY * (lz + c));
-                            cf[a][b][c] = blk[li];
-                            if constexpr (KF) kc[a][b][c] = kb[li];
-                            if constexpr (SRC) sc[a][b][c] = sb[li];
+                            cf[a][b][c] = L.blk[li];
+                            if constexpr (KF) kc[a][b][c] = L.kb[li];
+                            if constexpr (SRC) sc[a][b][c] = L.sb[li];
                         }
             }
 #pragma unroll 1
@@ -168,8 +140,8 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                 T bx[M], dx[M];
 #pragma unroll
                 for (int a = 0; a < M; ++a) {
-                    bx[a] = fvx[(lx * M + qx) * M + a];
-                    dx[a] = fdx[(lx * M + qx) * M + a];
+                    bx[a] = L.fvx[(lx * M + qx) * M + a];
+                    dx[a] = L.fdx[(lx * M + qx) * M + a];
                 }
                 // stage x: contract the element's first index
                 T A0[M][M], A1[M][M], KA[KF ? M : 1][KF ? M : 1], SA[SRC ? M : 1][SRC ? M : 1];
@@ -181,13 +153,13 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
 #pragma unroll
                         for (int a = 0; a < M; ++a) {
                             const int li = (lx + a) + BX * ((ly + b) + BY * (lz + c));
-                            const T cv = CREG ? cf[CREG ? a : 0][CREG ? b : 0][CREG ? c : 0] : blk[li];
+                            const T cv = CREG ? cf[CREG ? a : 0][CREG ? b : 0][CREG ? c : 0] : L.blk[li];
                             s0 += bx[a] * cv;
                             s1 += dx[a] * cv;
                             if constexpr (KF)
-                                sk += bx[a] * (CREG ? kc[KF && CREG ? a : 0][KF && CREG ? b : 0][KF && CREG ? c : 0] : kb[li]);
+                                sk += bx[a] * (CREG ? kc[KF && CREG ? a : 0][KF && CREG ? b : 0][KF && CREG ? c : 0] : L.kb[li]);
                             if constexpr (SRC)
-                                ss += bx[a] * (CREG ? sc[SRC && CREG ? a : 0][SRC && CREG ? b : 0][SRC && CREG ? c : 0] : sb[li]);
+                                ss += bx[a] * (CREG ? sc[SRC && CREG ? a : 0][SRC && CREG ? b : 0][SRC && CREG ? c : 0] : L.sb[li]);
                         }
                         A0[b][c] = s0;
                         A1[b][c] = s1;
@@ -204,8 +176,8 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                     T by[M], dy[M];
 #pragma unroll
                     for (int b = 0; b < M; ++b) {
-                        by[b] = fvy[(ly * M + qy) * M + b];
-                        dy[b] = fdy[(ly * M + qy) * M + b];
+                        by[b] = L.fvy[(ly * M + qy) * M + b];
+                        dy[b] = L.fdy[(ly * M + qy) * M + b];
                     }
                     // stage y for this Gauss point
                     T a00[M], a01[M], a10[M], ka[KF ? M : 1], sa[SRC ? M : 1];
@@ -267,7 +239,7 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                     // stage y transposed (weights and -dt*grad_y folded into the tables)
 #pragma unroll
                     for (int b = 0; b < M; ++b) {
-                        const T wb = bvy[(ly * M + qy) * M + b], wd = bdy[(ly * M + qy) * M + b];
+                        const T wb = L.bvy[(ly * M + qy) * M + b], wd = L.bdy[(ly * M + qy) * M + b];
 #pragma unroll
                         for (int c = 0; c < M; ++c) {
                             K0[b][c] += wb * H0[c];
@@ -279,7 +251,7 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                 // stage x transposed
 #pragma unroll
                 for (int a = 0; a < M; ++a) {
-                    const T wb = bvx[(lx * M + qx) * M + a], wd = bdx[(lx * M + qx) * M + a];
+                    const T wb = L.bvx[(lx * M + qx) * M + a], wd = L.bdx[(lx * M + qx) * M + a];
 #pragma unroll
                     for (int b = 0; b < M; ++b)
 #pragma unroll
@@ -308,20 +280,84 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
                                            static_cast<double>(R[FLUSH ? 0 : a][FLUSH ? 0 : b][FLUSH ? 0 : c]),
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
-    __syncthreads();
+}
 
-    // DOF plane k of the block collects plane k - lz of the waves lz = k-P .. k
-    for (int i = tid; i < NB; i += NT) {
-        const int bxy = i % PL, bz = i / PL;
-        const int gx = ex0 + bxy % BX, gy = ey0 + bxy / BX, gz = ez0 + bz;
-        if (gx >= nx || gy >= ny || gz >= nz) continue;
-        double sum = 0;
-#pragma unroll
-        for (int d = 0; d < M; ++d) {
-            const int w = bz - d;
-            if (w >= 0 && w < TZ) sum += acc[(w * M + d) * PL + bxy];
+// One workgroup per tile (block b -> tile by XCD-contiguous numbering).
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX, int VAR>
+__global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
+    const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
+    dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
+    using LDS = rhs3_lds<T, P, TX, TY, TZ, KF, SRC>;
+    constexpr int M = P + 1, NT = TX * TY * TZ;
+    constexpr int BX = LDS::BX, BY = LDS::BY, NB = LDS::NB, PL = LDS::PL;
+    static_assert(TX * TY == 64, "a wave must cover exactly one z-layer of the tile");
+    static_assert(TX * M * M <= NT && TY * M * M <= NT, "one table entry per thread");
+    __shared__ LDS L;
+
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = (tid / TX) % TY;
+    const int lz = __builtin_amdgcn_readfirstlane(tid / (TX * TY));
+    const int nx = X.dofs, ny = Y.dofs, nz = Z.dofs;
+    const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy;
+
+    {
+        // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD
+        // gets a contiguous run of tiles (one slab of the domain), whose halos the next
+        // colour's launch then finds in that XCD's L2.
+        const tile3 o = rhs3_tile<TX, TY, TZ>(xcd_contiguous(blockIdx.x, gridDim.x), colour,
+                                              ntx2, nty2);
+        // the tile's DOF block (first(e) = e for these knot vectors: dof origin = element origin)
+        for (int i = tid; i < NB; i += NT) {
+            const int bx = i % BX, r = i / BX, by = r % BY, bz = r / BY;
+            const int gx = o.ex0 + bx, gy = o.ey0 + by, gz = o.ez0 + bz;
+            const bool in = gx < nx && gy < ny && gz < nz;
+            const long g = gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz);
+            L.blk[i] = in ? up[g] : T(0);
+            if constexpr (KF) L.kb[i] = in ? prm.kfield[g] : T(0);
+            if constexpr (SRC) L.sb[i] = in ? prm.source[g] : T(0);
         }
-        rhs[gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz)] += static_cast<T>(sum);
+        if (tid < TX * M * M) {
+            const int e = o.ex0 + tid / (M * M), q = (tid / M) % M;
+            const bool in = e < X.elements;
+            const T v = in ? X.val[static_cast<long>(o.ex0) * M * M + tid] : T(0);
+            const T d = in ? X.der[static_cast<long>(o.ex0) * M * M + tid] : T(0);
+            const T w = in ? X.wj[static_cast<long>(e) * M + q] : T(0);
+            L.fvx[tid] = v;
+            L.fdx[tid] = d;
+            L.bvx[tid] = v * w;
+            L.bdx[tid] = cx * (d * w);
+        }
+        if (tid < TY * M * M) {
+            const int e = o.ey0 + tid / (M * M), q = (tid / M) % M;
+            const bool in = e < Y.elements;
+            const T v = in ? Y.val[static_cast<long>(o.ey0) * M * M + tid] : T(0);
+            const T d = in ? Y.der[static_cast<long>(o.ey0) * M * M + tid] : T(0);
+            const T w = in ? Y.wj[static_cast<long>(e) * M + q] : T(0);
+            L.fvy[tid] = v;
+            L.fdy[tid] = d;
+            L.bvy[tid] = v * w;
+            L.bdy[tid] = cy * (d * w);
+        }
+        for (int i = tid; i < TZ * M * PL; i += NT) L.acc[i] = 0.0;
+        __syncthreads();
+
+        rhs3_element<T, P, TX, TY, TZ, KF, SRC, RX, VAR>(L, X, Y, Z, prm, lx, ly, lz, o.ex0,
+                                                          o.ey0, o.ez0);
+        __syncthreads();
+
+        // DOF plane k of the block collects plane k - lz of the waves lz = k-P .. k
+        for (int i = tid; i < NB; i += NT) {
+            const int bxy = i % PL, bz = i / PL;
+            const int gx = o.ex0 + bxy % BX, gy = o.ey0 + bxy / BX, gz = o.ez0 + bz;
+            if (gx >= nx || gy >= ny || gz >= nz) continue;
+            double sum = 0;
+#pragma unroll
+            for (int d = 0; d < M; ++d) {
+                const int w = bz - d;
+                if (w >= 0 && w < TZ) sum += L.acc[(w * M + d) * PL + bxy];
+            }
+            rhs[gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz)] += static_cast<T>(sum);
+        }
     }
 }
 
@@ -550,13 +586,14 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
     const int ntx = (d[0].elements + TL::X - 1) / TL::X;
     const int nty = (d[1].elements + TL::Y - 1) / TL::Y;
     const int ntz = (d[2].elements + TL::Z - 1) / TL::Z;
+    constexpr int NT = TL::X * TL::Y * TL::Z;
+    const auto kernel = k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, RX, VAR>;
     for (int c = 0; c < 8; ++c) {
         const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
         const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2, nz2 = (ntz - cz + 1) / 2;
         const long blocks = static_cast<long>(nx2) * ny2 * nz2;
         if (blocks == 0) continue;
-        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, RX, VAR>), dim3(blocks),
-                           dim3(TL::X * TL::Y * TL::Z), 0, s, up, rhs, d[0], d[1], d[2], prm, c,
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm, c,
                            nx2, ny2);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -70,6 +70,10 @@ struct adsg_sim {
     std::string err;
     hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
     double* norm_scratch = nullptr;  // adsg_norm: block partial sums and the total
+    // load vector of the source, b_i = sum_e sum_q w J f B_i (step precision): forms with a
+    // source start their rhs from dt * b; recomputed after the source changes
+    void* source_load = nullptr;
+    bool source_load_ok = false;
     // adsg_forward_checkpointed / adsg_adjoint_checkpointed: u every `ck_every` steps, and the
     // recomputed states of one segment plus a slot for lambda
     double* ckpt = nullptr;
