@@ -99,7 +99,7 @@ def implicit_block(ag, np, ne, steps, warmup):
                      f"A = (x)_d (M_d + dt/2 K_d); natural BC"),
         "fp64": {"value": N * steps / t64, "ms_per_step": 1e3 * t64 / steps, "phases_ms": ph64,
                  "rhs_variant": v64,
-                 "rhs_tflops": f_elem_source(3, 3) * ne ** 3 / (ph64["rhs"] * 1e-3) / 1e12},
+                 "rhs_tflops": f_elem(3, 3) * ne ** 3 / (ph64["rhs"] * 1e-3) / 1e12},
     }
     for name, flags in (("fp32", 0), ("fp32_refined", ag.STEP_REFINE)):
         s32, t32, ph32, x32, v32 = run_implicit(ag, ag.FP32, ne, steps, warmup, u0, f, dt, flags)
@@ -250,17 +250,6 @@ def adjoint_k_block(ag, np, ne, steps, every, dt):
             "adjoint": {"value": N * steps / ta, "ms_per_step": 1e3 * ta / steps,
                         "note": "recompute of every segment from its checkpoint included"},
             "fd_direction_rel_err": abs(fd - ad) / abs(ad), "fd_h": h}
-
-
-def f_elem_source(d, p):
-    """Flops per element of the delta-form integrand dt (f v - grad u . grad v), 3D: the
-    forward tree needs 8 one-dimensional contractions for grad u (value and derivative in x;
-    three mixed y stages; three z stages) and 3 for the value of f; the transposed tree
-    needs 9 (the value path carries f v). Each contraction costs 2 n^(d+1); the pointwise
-    work is (d+2) n^d."""
-    assert d == 3
-    n = p + 1
-    return 2 * (8 + 3 + 9) * n ** (d + 1) + (d + 2) * n ** d
 
 
 def copy_calibration(mib, reps):
