@@ -69,7 +69,7 @@ EXPORTS = [
     "adsg_batch_set_vector", "adsg_batch_get_vector", "adsg_batch_step", "adsg_batch_synchronize",
     "adsg_batch_set_tuning", "adsg_batch_get_tuning", "adsg_batch_autotune",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
-    "adsg_autotune",
+    "adsg_autotune", "adsg_project", "adsg_project_device", "adsg_quad_points",
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
@@ -138,6 +138,9 @@ def lib():
         L.adsg_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.adsg_get_tuning.argtypes = [C.c_void_p, C.c_int]
         L.adsg_autotune.argtypes = [C.c_void_p, C.POINTER(RhsForm), _ip, C.c_int]
+        L.adsg_project.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.adsg_project_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.adsg_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -169,6 +172,8 @@ class Simulation:
             raise AdsgError(f"adsg_create: {rc}: {L.adsg_last_error(None).decode()}")
         self.h = h
         self.dim = dim
+        self.p = ps[:dim]
+        self.elements = es[:dim]
         self.precision = precision
         self.shape = tuple(L.adsg_dofs(h, d) for d in range(dim))
         self.size = int(L.adsg_size(h))
@@ -267,6 +272,28 @@ class Simulation:
         out = np.zeros(1)
         self._check(lib().adsg_norm(self.h, which, kind, out.ctypes.data_as(_dp)), "norm")
         return float(out[0])
+
+    def quad_points(self, d):
+        """Gauss points of direction d, element-major (elements*(p+1) abscissae)."""
+        out = np.zeros(self.elements[d] * (self.p[d] + 1))
+        self._check(lib().adsg_quad_points(self.h, d, out.ctypes.data_as(_dp)), "quad_points")
+        return out
+
+    def project(self, f, which=U):
+        """projection(v, f) (projection.hpp:11-153): add the L2 projection's load vector of f
+        into vector `which`; f is a callable of the coordinates (vectorised over numpy arrays)
+        or its values at the Gauss points, x fastest. Follow with ads_solve() to get the
+        coefficients, as the reference's before() does."""
+        if callable(f):
+            pts = [self.quad_points(d) for d in range(self.dim)]
+            grids = np.meshgrid(*pts, indexing="ij")
+            fq = np.asarray(f(*grids), dtype=np.float64).transpose().copy()
+        else:
+            fq = np.ascontiguousarray(f, dtype=np.float64)
+        count = int(np.prod([self.elements[d] * (self.p[d] + 1) for d in range(self.dim)]))
+        if fq.size != count:
+            raise ValueError(f"project: {fq.size} values for {count} Gauss points")
+        self._check(lib().adsg_project(self.h, which, fq.ravel().ctypes.data_as(_dp)), "project")
 
     def forward_checkpointed(self, n, every, form, flags=0):
         """n explicit steps from u keeping a checkpoint every `every` steps (adjoint runs)."""

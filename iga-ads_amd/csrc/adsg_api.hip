@@ -713,6 +713,58 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
     return ADSG_OK;
 }
 
+namespace {
+
+template <typename TT>
+void step_tables(const adsg_sim* h, adsg::dev_tables<TT>* t) {
+    for (int d = 0; d < h->dim; ++d) {
+        t[d].val = static_cast<const TT*>(h->dirs[d].val);
+        t[d].der = static_cast<const TT*>(h->dirs[d].der);
+        t[d].wj = static_cast<const TT*>(h->dirs[d].wj);
+        t[d].elements = h->dirs[d].tab.elements;
+        t[d].dofs = h->dirs[d].tab.dofs;
+    }
+}
+
+long quad_point_count(const adsg_sim* h) {
+    long q = 1;
+    for (int d = 0; d < h->dim; ++d) q *= static_cast<long>(h->dirs[d].tab.elements) * h->dirs[d].tab.nq;
+    return q;
+}
+
+int project_impl(adsg_sim* h, int which, const double* fq_dev) {
+    if (!h->vec[which]) {
+        if (hipMalloc(&h->vec[which], h->N * sizeof(double)) != hipSuccess)
+            return fail(h, ADSG_ENOMEM, "project: device allocation failed");
+        HIP_TRY(h, hipMemsetAsync(h->vec[which], 0, h->N * sizeof(double), h->stream));
+        drop_graphs(h);
+    }
+    if (which == ADSG_SOURCE) h->source_load_ok = false;
+    double* v = static_cast<double*>(h->vec[which]);
+    hipError_t e;
+    if (h->precision == ADSG_FP32) {
+        adsg::dev_tables<float> t[3];
+        step_tables(h, t);
+        e = adsg::launch_project<float>(h->dim, h->dirs[0].tab.p, fq_dev, v, t, h->stream);
+    } else {
+        adsg::dev_tables<double> t[3];
+        step_tables(h, t);
+        e = adsg::launch_project<double>(h->dim, h->dirs[0].tab.p, fq_dev, v, t, h->stream);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "project");
+    if (h->precision == ADSG_FP32 && (which == ADSG_U || which == ADSG_U_PREV)) {
+        e = adsg::launch_convert_d2f(v, static_cast<float*>(h->shadow[which]), h->N, h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "project: shadow");
+    }
+    return ADSG_OK;
+}
+
+bool projectable(const adsg_sim* h, int which) {
+    return which >= ADSG_U && which <= ADSG_GRAD_K && elt_of(h, which) == sizeof(double);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* adsg_last_error(const adsg_sim* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -1144,6 +1196,36 @@ int adsg_norm(adsg_sim* h, int which, int kind, double* out) {
     HIP_TRY(h, hipMemcpyAsync(&sq, h->norm_scratch + kBlocks, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     *out = std::sqrt(sq);
+    return ADSG_OK;
+}
+
+int adsg_project(adsg_sim* h, int which, const double* fq) {
+    if (!h || !fq || !projectable(h, which)) return fail(h, ADSG_EINVAL, "project: bad argument (an FP64 vector)");
+    const size_t bytes = quad_point_count(h) * sizeof(double);
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return fail(h, ADSG_ENOMEM, "project: no memory for f");
+    int rc = ADSG_OK;
+    hipError_t e = hipMemcpyAsync(d, fq, bytes, hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) rc = hip_fail(h, e, "project: upload");
+    if (rc == ADSG_OK) rc = project_impl(h, which, static_cast<const double*>(d));
+    e = hipStreamSynchronize(h->stream);
+    if (rc == ADSG_OK && e != hipSuccess) rc = hip_fail(h, e, "project");
+    (void)hipFree(d);
+    return rc;
+}
+
+int adsg_project_device(adsg_sim* h, int which, const double* fq) {
+    if (!h || !fq || !projectable(h, which)) return fail(h, ADSG_EINVAL, "project_device: bad argument (an FP64 vector)");
+    int rc = project_impl(h, which, fq);
+    if (rc) return rc;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return ADSG_OK;
+}
+
+int adsg_quad_points(const adsg_sim* h, int dir, double* x) {
+    if (!h || !x || dir < 0 || dir >= h->dim) return ADSG_EINVAL;
+    const auto& xq = h->dirs[dir].tab.xq;
+    std::copy(xq.begin(), xq.end(), x);
     return ADSG_OK;
 }
 

@@ -149,6 +149,12 @@ template <typename TT>
 hipError_t launch_norm(int dim, int p, const double* u, const dev_tables<TT>* dirs, int kind,
                        double* partial, int blocks, double* out, hipStream_t s);
 
+// Load vector of the L2 projection of f (given at the Gauss points, fq[ix + NX*(iy +
+// NY*iz)], ix = e_x*(p+1) + q_x, NX = elements_x*(p+1)), added into out (FP64)
+template <typename TT>
+hipError_t launch_project(int dim, int p, const double* fq, double* out, const dev_tables<TT>* dirs,
+                          hipStream_t s);
+
 template <typename T>
 hipError_t launch_fill(T* p, long n, T v, hipStream_t s);
 

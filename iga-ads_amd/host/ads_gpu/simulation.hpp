@@ -131,6 +131,7 @@ class gpu_simulation : public simulation_base {
 protected:
     adsg_sim* h_ = nullptr;
     int dim_ = 0;
+    int quad_count_[3] = {0, 0, 0};  // Gauss points per direction
 
     void check(int rc, const char* what) const {
         if (rc != ADSG_OK) throw error(rc, std::string(what) + ": " + adsg_last_error(h_));
@@ -139,6 +140,8 @@ protected:
     gpu_simulation(const adsg_config& cfg, const timesteps_config& steps) : simulation_base{steps}, dim_{cfg.dim} {
         int rc = adsg_create(&cfg, &h_);
         if (rc != ADSG_OK) throw error(rc, std::string("adsg_create: ") + adsg_last_error(nullptr));
+        const adsg_dim_config* dc[3] = {&cfg.x, &cfg.y, &cfg.z};
+        for (int d = 0; d < dim_; ++d) quad_count_[d] = dc[d]->elements * (dc[d]->p + 1);
     }
 
     static adsg_dim_config to_c(const dim_config& d) {
@@ -215,6 +218,19 @@ protected:
     void set_state(const std::vector<double>& v) { set_vector(ADSG_U, v); }
     std::vector<double> state() { return vector(ADSG_U); }
 
+    // basis_data x[e][q]: the Gauss points of direction dir, element-major
+    std::vector<double> quad_points(int dir) const {
+        if (dir < 0 || dir >= dim_) throw error(ADSG_EINVAL, "quad_points: bad direction");
+        std::vector<double> x(quad_count_[dir]);
+        check(adsg_quad_points(h_, dir, x.data()), "quad_points");
+        return x;
+    }
+    // projection(v, f) (simulation_2d.hpp:43-46 -> projection.hpp:11-153): f evaluated at the
+    // Gauss points on the host, its load vector added into vector `which` on the device
+    void project_values(int which, const std::vector<double>& fq) {
+        check(adsg_project(h_, which, fq.data()), "project");
+    }
+
 public:
     ~gpu_simulation() override { adsg_destroy(h_); }
     int dofs(int dir) const { return adsg_dofs(h_, dir); }
@@ -240,6 +256,16 @@ class gpu_simulation_2d : public gpu_simulation {
 protected:
     std::array<int, 2> shape() const { return {dofs(0), dofs(1)}; }
 
+    // projection(v, f) with f(x, y) (simulation_2d.hpp:43-46)
+    template <typename Function>
+    void projection(int which, Function&& f) {
+        const auto px = quad_points(0), py = quad_points(1);
+        std::vector<double> fq(px.size() * py.size());
+        for (size_t j = 0; j < py.size(); ++j)
+            for (size_t i = 0; i < px.size(); ++i) fq[i + px.size() * j] = f(px[i], py[j]);
+        project_values(which, fq);
+    }
+
 public:
     explicit gpu_simulation_2d(const config_2d& c) : gpu_simulation{make(c), c.steps} {}
 };
@@ -261,6 +287,18 @@ class gpu_simulation_3d : public gpu_simulation {
 
 protected:
     std::array<int, 3> shape() const { return {dofs(0), dofs(1), dofs(2)}; }
+
+    // projection(v, f) with f(x, y, z) (simulation_3d.hpp:45-48)
+    template <typename Function>
+    void projection(int which, Function&& f) {
+        const auto px = quad_points(0), py = quad_points(1), pz = quad_points(2);
+        std::vector<double> fq(px.size() * py.size() * pz.size());
+        for (size_t k = 0; k < pz.size(); ++k)
+            for (size_t j = 0; j < py.size(); ++j)
+                for (size_t i = 0; i < px.size(); ++i)
+                    fq[i + px.size() * (j + py.size() * k)] = f(px[i], py[j], pz[k]);
+        project_values(which, fq);
+    }
 
 public:
     explicit gpu_simulation_3d(const config_3d& c) : gpu_simulation{make(c), c.steps} {}

@@ -152,6 +152,22 @@ int adsg_ads_solve(adsg_sim* h, const int* slots);
 #define ADSG_NORM_H1 2
 int adsg_norm(adsg_sim* h, int which, int kind, double* out);
 
+/* projection(v, f) (simulation_2d.hpp:43-46 / simulation_3d.hpp:45-48 -> projector,
+ * projection.hpp:11-153): adds the load vector of the L2 projection of f,
+ *   v_i += sum_e sum_q f(x_q) B_i(x_q) w_q J_e,
+ * into vector `which`, which must hold FP64 coefficients (ADSG_U, ADSG_U_PREV, ADSG_GRAD_K,
+ * and in FP64 handles every vector; it is allocated and zeroed if absent). f is given at
+ * the Gauss points: fq[ix + NX*(iy + NY*iz)] with ix = e_x*(p+1) + q_x, NX = elements_x*(p+1)
+ * (likewise y and z; adsg_quad_points returns the abscissae). As in the reference's
+ * before() (validation.hpp:65-71), adsg_ads_solve then turns the load into coefficients
+ * (FP64 handles). Accumulated in FP64; FP32 handles use their FP32 basis tables.
+ * adsg_project reads fq from host memory, adsg_project_device from device memory. */
+int adsg_project(adsg_sim* h, int which, const double* fq);
+int adsg_project_device(adsg_sim* h, int which, const double* fq);
+/* the Gauss points of direction dir (basis_data x[e][q], basis_data.cpp:63-114):
+ * elements*(p+1) abscissae, element-major. */
+int adsg_quad_points(const adsg_sim* h, int dir, double* x);
+
 /* The transpose of ads_solve: u := (A_x (x) A_y [(x) A_z])^-T u, direction by direction
  * with dgbtrs 'T' semantics on the same FP64 factors (band_solve.hpp:20-31 with
  * trans = 'T'; the reference calls dgbtrs only with 'N'). Needed by adjoint steps. FP64 handles; every

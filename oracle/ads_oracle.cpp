@@ -893,6 +893,57 @@ double orc_sim_l2norm_sq(const orc_sim* s, const double* u) {
 // for the form's gradient term -dt * k * sum_d grad_d u_d v_d (validation.hpp:110-115 with
 // k as in flow.hpp:73-100): out_b += -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d
 // (d_d u)(x_q) (d_d w)(x_q). No counterpart in the reference (it has no adjoint code).
+// projection.hpp:55-110: element loop (x outermost), Gauss-point loop, local-DOF loop;
+// J, w and B are products over the directions taken in order from 1.
+void orc_sim_project(const orc_sim* s, const double* fq, double* v) {
+    const int dim = s->dim;
+    const orc_dim* D[3] = {&s->d[0], &s->d[1], &s->d[2]};
+    const int ez_n = dim == 3 ? D[2]->elements : 1;
+    const int qz_n = dim == 3 ? D[2]->q : 1;
+    const int az_n = dim == 3 ? D[2]->p + 1 : 1;
+    const long NX = static_cast<long>(D[0]->elements) * D[0]->q;
+    const long NY = static_cast<long>(D[1]->elements) * D[1]->q;
+    for (int ex = 0; ex < D[0]->elements; ++ex)
+        for (int ey = 0; ey < D[1]->elements; ++ey)
+            for (int ez = 0; ez < ez_n; ++ez) {
+                const int e[3] = {ex, ey, ez};
+                double J = 1;
+                for (int i = 0; i < dim; ++i) J *= D[i]->J[e[i]];
+                for (int qx = 0; qx < D[0]->q; ++qx)
+                    for (int qy = 0; qy < D[1]->q; ++qy)
+                        for (int qz = 0; qz < qz_n; ++qz) {
+                            const int q[3] = {qx, qy, qz};
+                            double w = 1;
+                            for (int i = 0; i < dim; ++i) w *= D[i]->w[q[i]];
+                            const long ix = static_cast<long>(ex) * D[0]->q + qx;
+                            const long iy = static_cast<long>(ey) * D[1]->q + qy;
+                            const long iz = dim == 3 ? static_cast<long>(ez) * D[2]->q + qz : 0;
+                            const double fx = fq[ix + NX * (iy + NY * iz)];
+                            for (int ax = 0; ax <= D[0]->p; ++ax)
+                                for (int ay = 0; ay <= D[1]->p; ++ay)
+                                    for (int az = 0; az < az_n; ++az) {
+                                        const int a[3] = {ax, ay, az};
+                                        double B = 1;
+                                        long idx[3] = {0, 0, 0};
+                                        for (int i = 0; i < dim; ++i) {
+                                            B *= tab(*D[i], e[i], q[i])[a[i]];
+                                            idx[i] = a[i] + D[i]->first[e[i]];
+                                        }
+                                        const long gi = idx[0] + D[0]->n * (idx[1] + static_cast<long>(D[1]->n) * idx[2]);
+                                        v[gi] += fx * B * w * J;
+                                    }
+                        }
+            }
+}
+
+int orc_sim_quad_points(const orc_sim* s, int dir, double* out) {
+    if (dir < 0 || dir >= s->dim) return -1;
+    const orc_dim& D = s->d[dir];
+    for (int e = 0; e < D.elements; ++e)
+        for (int q = 0; q < D.q; ++q) out[e * D.q + q] = D.x[static_cast<long>(e) * D.q + q];
+    return 0;
+}
+
 void orc_sim_kgrad(const orc_sim* s, const orc_rhs_form* f, const double* u, const double* w,
                    double* out) {
     const long n = element_count(s);

@@ -125,6 +125,12 @@ void orc_sim_compute_rhs_threaded(const orc_sim* s, const orc_rhs_form* f, const
  * into out (no counterpart in the reference; the k-field adjoint). */
 void orc_sim_kgrad(const orc_sim* s, const orc_rhs_form* f, const double* u, const double* w,
                    double* out);
+/* projector<N> (projection.hpp:11-153, via simulation_2d.hpp:43-46): v_i += sum_e sum_q
+ * f(x_q) B_i(x_q) w_q J_e, f given at the Gauss points: fq[ix + NX*(iy + NY*iz)] with
+ * ix = e_x*q + q_x, NX = elements_x*q (likewise y, z). Adds into v. */
+void orc_sim_project(const orc_sim* s, const double* fq, double* v);
+/* the Gauss points of direction dir (basis_data x[e][q]), elements*q values */
+int orc_sim_quad_points(const orc_sim* s, int dir, double* out);
 void orc_greville(const double* knot, int knot_size, int p, double* out);
 void orc_init_gaussians(const orc_sim* s, unsigned long long seed, int count, double sigma_lo,
                         double sigma_hi, double centre_lo, double centre_hi, double noise,

@@ -59,6 +59,8 @@ def lib():
         L.orc_sim_solve_transposed.argtypes = [C.c_void_p, _dp, _ip]
         L.orc_sim_tables.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _ip]
         L.orc_sim_kgrad.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp]
+        L.orc_sim_project.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_sim_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_sim_l2norm_sq.argtypes = [C.c_void_p, _dp]
         L.orc_sim_l2norm_sq.restype = C.c_double
         L.orc_sim_norm_sq.argtypes = [C.c_void_p, _dp, C.c_int]
@@ -211,6 +213,19 @@ class Sim:
         out = self.zeros()
         lib().orc_sim_kgrad(self.h, C.byref(form), _d(u), _d(w), _d(out))
         return out
+
+    def quad_points(self, d):
+        q = int(self._q[d]) if self._q[d] > 0 else self.p[d] + 1
+        out = np.zeros(self.elements[d] * q)
+        lib().orc_sim_quad_points(self.h, d, _d(out))
+        return out
+
+    def project(self, fq, v=None):
+        """projection.hpp:55-110: v += load vector of f given at the Gauss points (x fastest)."""
+        fq = np.ascontiguousarray(fq, dtype=np.float64).ravel()
+        v = self.zeros() if v is None else v
+        lib().orc_sim_project(self.h, _d(fq), _d(v))
+        return v
 
     def zero_boundary(self, v):
         lib().orc_sim_zero_boundary(self.h, _d(v))
