@@ -69,7 +69,7 @@ EXPORTS = [
     "adsg_batch_set_vector", "adsg_batch_get_vector", "adsg_batch_step", "adsg_batch_synchronize",
     "adsg_batch_set_tuning", "adsg_batch_get_tuning", "adsg_batch_autotune",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
-    "adsg_autotune", "adsg_project", "adsg_project_device", "adsg_quad_points",
+    "adsg_autotune", "adsg_project", "adsg_project_device", "adsg_quad_points", "adsg_error",
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
@@ -141,6 +141,7 @@ def lib():
         L.adsg_project.argtypes = [C.c_void_p, C.c_int, _dp]
         L.adsg_project_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.adsg_error.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -279,15 +280,34 @@ class Simulation:
         self._check(lib().adsg_quad_points(self.h, d, out.ctypes.data_as(_dp)), "quad_points")
         return out
 
+    def quad_grid(self):
+        """Coordinate arrays of the Gauss points, indexed [z][y][x] (x fastest when raveled):
+        f(*sim.quad_grid()) gives f at the Gauss points in the layout adsg_project reads."""
+        pts = [self.quad_points(d) for d in range(self.dim)]
+        return [g.transpose() for g in np.meshgrid(*pts, indexing="ij")]
+
+    def error(self, s, kind=0, which=U):
+        """error(u, ..., norm, fun) (basic_simulation_2d.hpp:318-336): s at the Gauss points,
+        planes [value, d/dx, d/dy(, d/dz)] (value only for kind 0); which=None gives the norm
+        of s itself (normL2/normH1(Ux, Uy, fun))."""
+        sq = np.ascontiguousarray(s, dtype=np.float64)
+        count = int(np.prod([self.elements[d] * (self.p[d] + 1) for d in range(self.dim)]))
+        planes = 1 if kind == 0 else 1 + self.dim
+        if sq.size < planes * count:
+            raise ValueError(f"error: {sq.size} values for {planes} planes of {count} points")
+        out = np.zeros(1)
+        w = -1 if which is None else which
+        self._check(lib().adsg_error(self.h, w, kind, sq.ravel().ctypes.data_as(_dp),
+                                     out.ctypes.data_as(_dp)), "error")
+        return float(out[0])
+
     def project(self, f, which=U):
         """projection(v, f) (projection.hpp:11-153): add the L2 projection's load vector of f
         into vector `which`; f is a callable of the coordinates (vectorised over numpy arrays)
         or its values at the Gauss points, x fastest. Follow with ads_solve() to get the
         coefficients, as the reference's before() does."""
         if callable(f):
-            pts = [self.quad_points(d) for d in range(self.dim)]
-            grids = np.meshgrid(*pts, indexing="ij")
-            fq = np.asarray(f(*grids), dtype=np.float64).transpose().copy()
+            fq = np.ascontiguousarray(f(*self.quad_grid()), dtype=np.float64)
         else:
             fq = np.ascontiguousarray(f, dtype=np.float64)
         count = int(np.prod([self.elements[d] * (self.p[d] + 1) for d in range(self.dim)]))

@@ -120,18 +120,22 @@ hipError_t kgrad_dim(int dim, const T* u, const T* w, T* out, const dev_tables<T
     return hipErrorInvalidValue;
 }
 
-// Squared norm functionals (basic_simulation_2d.hpp:42-52, :291-316): per element and Gauss
-// point w J phi(u) with phi = u^2 (kind 0), |grad u|^2 (1) or both (2). One lane per element
-// (grid-stride), FP64 sums reduced in a fixed order: lanes -> block partial -> one total.
+// Squared norm and error functionals (basic_simulation_2d.hpp:42-52, :266-362): per element
+// and Gauss point w J phi(d), d = u_h - s, with u_h the spline of the coefficients u (or 0
+// when u is null) and s a function given at the Gauss points as planes of Q values
+// [value | d/dx | d/dy (| d/dz)] (or 0 when sq is null); phi = d^2 (kind 0), |grad d|^2 (1)
+// or both (2). One lane per element (grid-stride), FP64 sums reduced in a fixed order:
+// lanes -> block partial -> one total.
 template <typename TT, int DIM, int P>
-__global__ __launch_bounds__(256) void k_norm(const double* __restrict__ u, dev_tables<TT> tx,
-                                              dev_tables<TT> ty, dev_tables<TT> tz, int kind,
-                                              double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_norm(const double* __restrict__ u, const double* __restrict__ sq,
+                                              long Q, dev_tables<TT> tx, dev_tables<TT> ty,
+                                              dev_tables<TT> tz, int kind, double* __restrict__ partial) {
     constexpr int n = P + 1;
     constexpr int NZ = DIM == 3 ? n : 1;
     __shared__ double red[256];
     const long E = static_cast<long>(tx.elements) * ty.elements * (DIM == 3 ? tz.elements : 1);
     const long nx = tx.dofs, ny = ty.dofs;
+    const long NX = static_cast<long>(tx.elements) * n, NY = static_cast<long>(ty.elements) * n;
     double acc = 0;
     for (long l = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; l < E;
          l += static_cast<long>(gridDim.x) * blockDim.x) {
@@ -142,7 +146,7 @@ __global__ __launch_bounds__(256) void k_norm(const double* __restrict__ u, dev_
         for (int c = 0; c < NZ; ++c)
             for (int b = 0; b < n; ++b)
                 for (int a = 0; a < n; ++a)
-                    U[c][b][a] = u[(ex + a) + nx * ((ey + b) + ny * static_cast<long>(ez + c))];
+                    U[c][b][a] = u ? u[(ex + a) + nx * ((ey + b) + ny * static_cast<long>(ez + c))] : 0.0;
         const TT* vx = tx.val + static_cast<long>(ex) * n * n;
         const TT* dx = tx.der + static_cast<long>(ex) * n * n;
         const TT* vy = ty.val + static_cast<long>(ey) * n * n;
@@ -166,6 +170,16 @@ __global__ __launch_bounds__(256) void k_norm(const double* __restrict__ u, dev_
                                 gy += cu * (bx * bdy * bz);
                                 if (DIM == 3) gz += cu * (bx * by * bdz);
                             }
+                        }
+                    }
+                    if (sq) {
+                        const long pt = (static_cast<long>(ex) * n + qx) +
+                                        NX * ((static_cast<long>(ey) * n + qy) + NY * (static_cast<long>(ez) * n + qz));
+                        v -= sq[pt];
+                        if (kind != 0) {
+                            gx -= sq[Q + pt];
+                            gy -= sq[2 * Q + pt];
+                            if (DIM == 3) gz -= sq[3 * Q + pt];
                         }
                     }
                     double w = double(tx.wj[ex * n + qx]) * double(ty.wj[ey * n + qy]);
@@ -198,15 +212,17 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
 }
 
 template <typename TT, int P>
-hipError_t norm_dim(int dim, const double* u, const dev_tables<TT>* d, int kind, double* partial,
-                    int blocks, double* out, hipStream_t s) {
+hipError_t norm_dim(int dim, const double* u, const double* sq, const dev_tables<TT>* d, int kind,
+                    double* partial, int blocks, double* out, hipStream_t s) {
     const dev_tables<TT> none{};
+    long Q = 1;
+    for (int i = 0; i < dim; ++i) Q *= static_cast<long>(d[i].elements) * (P + 1);
     if (dim == 2)
-        hipLaunchKernelGGL((k_norm<TT, 2, P>), dim3(blocks), dim3(256), 0, s, u, d[0], d[1], none,
-                           kind, partial);
+        hipLaunchKernelGGL((k_norm<TT, 2, P>), dim3(blocks), dim3(256), 0, s, u, sq, Q, d[0], d[1],
+                           none, kind, partial);
     else
-        hipLaunchKernelGGL((k_norm<TT, 3, P>), dim3(blocks), dim3(256), 0, s, u, d[0], d[1], d[2],
-                           kind, partial);
+        hipLaunchKernelGGL((k_norm<TT, 3, P>), dim3(blocks), dim3(256), 0, s, u, sq, Q, d[0], d[1],
+                           d[2], kind, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, blocks, out);
@@ -216,25 +232,25 @@ hipError_t norm_dim(int dim, const double* u, const dev_tables<TT>* d, int kind,
 }  // namespace
 
 template <typename TT>
-hipError_t launch_norm(int dim, int p, const double* u, const dev_tables<TT>* dirs, int kind,
-                       double* partial, int blocks, double* out, hipStream_t s) {
+hipError_t launch_norm(int dim, int p, const double* u, const double* sq, const dev_tables<TT>* dirs,
+                       int kind, double* partial, int blocks, double* out, hipStream_t s) {
     if (blocks <= 0 || (dim != 2 && dim != 3)) return hipErrorInvalidValue;
     switch (p) {
         case 1:
-            return norm_dim<TT, 1>(dim, u, dirs, kind, partial, blocks, out, s);
+            return norm_dim<TT, 1>(dim, u, sq, dirs, kind, partial, blocks, out, s);
         case 2:
-            return norm_dim<TT, 2>(dim, u, dirs, kind, partial, blocks, out, s);
+            return norm_dim<TT, 2>(dim, u, sq, dirs, kind, partial, blocks, out, s);
         case 3:
-            return norm_dim<TT, 3>(dim, u, dirs, kind, partial, blocks, out, s);
+            return norm_dim<TT, 3>(dim, u, sq, dirs, kind, partial, blocks, out, s);
         default:
             return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_norm<double>(int, int, const double*, const dev_tables<double>*, int,
-                                        double*, int, double*, hipStream_t);
-template hipError_t launch_norm<float>(int, int, const double*, const dev_tables<float>*, int,
-                                       double*, int, double*, hipStream_t);
+template hipError_t launch_norm<double>(int, int, const double*, const double*, const dev_tables<double>*,
+                                        int, double*, int, double*, hipStream_t);
+template hipError_t launch_norm<float>(int, int, const double*, const double*, const dev_tables<float>*,
+                                       int, double*, int, double*, hipStream_t);
 
 template <typename T>
 hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,

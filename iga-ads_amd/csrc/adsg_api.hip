@@ -1162,9 +1162,10 @@ int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, 
     return run_graphs(h, ADJOINT_DELTA_STEP, nsteps, form, flags, slots);
 }
 
-int adsg_norm(adsg_sim* h, int which, int kind, double* out) {
-    if (!h || !out || (which != ADSG_U && which != ADSG_U_PREV) || kind < 0 || kind > 2)
-        return fail(h, ADSG_EINVAL, "norm: bad argument (U or U_PREV; kind 0, 1 or 2)");
+namespace {
+
+// sqrt of the functional of launch_norm: u = vec[which] (null for which = -1), sq on the device
+int norm_impl(adsg_sim* h, int which, const double* sq, int kind, double* out) {
     constexpr int kBlocks = 1024;
     if (!h->norm_scratch &&
         hipMalloc(reinterpret_cast<void**>(&h->norm_scratch), sizeof(double) * (kBlocks + 1)) != hipSuccess)
@@ -1172,7 +1173,7 @@ int adsg_norm(adsg_sim* h, int which, int kind, double* out) {
     long E = 1;
     for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
     const int blocks = static_cast<int>(std::min<long>(kBlocks, (E + 255) / 256));
-    const double* u = static_cast<const double*>(h->vec[which]);
+    const double* u = which >= 0 ? static_cast<const double*>(h->vec[which]) : nullptr;
     hipError_t e;
     if (h->precision == ADSG_FP32) {
         adsg::dev_tables<float> t[3];
@@ -1183,20 +1184,44 @@ int adsg_norm(adsg_sim* h, int which, int kind, double* out) {
             t[d].elements = h->dirs[d].tab.elements;
             t[d].dofs = h->dirs[d].tab.dofs;
         }
-        e = adsg::launch_norm<float>(h->dim, h->dirs[0].tab.p, u, t, kind, h->norm_scratch, blocks,
+        e = adsg::launch_norm<float>(h->dim, h->dirs[0].tab.p, u, sq, t, kind, h->norm_scratch, blocks,
                                      h->norm_scratch + kBlocks, h->stream);
     } else {
         adsg::dev_tables<double> t[3];
         fp64_tables(h, t);
-        e = adsg::launch_norm<double>(h->dim, h->dirs[0].tab.p, u, t, kind, h->norm_scratch, blocks,
+        e = adsg::launch_norm<double>(h->dim, h->dirs[0].tab.p, u, sq, t, kind, h->norm_scratch, blocks,
                                       h->norm_scratch + kBlocks, h->stream);
     }
     if (e != hipSuccess) return hip_fail(h, e, "norm");
-    double sq = 0;
-    HIP_TRY(h, hipMemcpyAsync(&sq, h->norm_scratch + kBlocks, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    double total = 0;
+    HIP_TRY(h, hipMemcpyAsync(&total, h->norm_scratch + kBlocks, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    *out = std::sqrt(sq);
+    *out = std::sqrt(total);
     return ADSG_OK;
+}
+
+}  // namespace
+
+int adsg_norm(adsg_sim* h, int which, int kind, double* out) {
+    if (!h || !out || (which != ADSG_U && which != ADSG_U_PREV) || kind < 0 || kind > 2)
+        return fail(h, ADSG_EINVAL, "norm: bad argument (U or U_PREV; kind 0, 1 or 2)");
+    return norm_impl(h, which, nullptr, kind, out);
+}
+
+int adsg_error(adsg_sim* h, int which, int kind, const double* sq, double* out) {
+    if (!h || !out || !sq || (which != ADSG_U && which != ADSG_U_PREV && which != -1) || kind < 0 || kind > 2)
+        return fail(h, ADSG_EINVAL, "error: bad argument (U, U_PREV or -1; kind 0, 1 or 2)");
+    const size_t planes = kind == 0 ? 1 : 1 + h->dim;
+    const size_t bytes = planes * quad_point_count(h) * sizeof(double);
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return fail(h, ADSG_ENOMEM, "error: no memory for the function values");
+    int rc = ADSG_OK;
+    hipError_t e = hipMemcpyAsync(d, sq, bytes, hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) rc = hip_fail(h, e, "error: upload");
+    if (rc == ADSG_OK) rc = norm_impl(h, which, static_cast<const double*>(d), kind, out);
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    return rc;
 }
 
 int adsg_project(adsg_sim* h, int which, const double* fq) {

@@ -143,11 +143,13 @@ template <typename T>
 hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,
                         const rhs_params<T>& prm, hipStream_t s);
 
-// Squared norm of the FP64 coefficient vector u: sum_e sum_q w J phi(u), phi = u^2 (kind 0),
-// |grad u|^2 (1), u^2 + |grad u|^2 (2); `blocks` partial sums in partial[], total in out[0].
+// Squared norm / error functional: sum_e sum_q w J phi(d), d = u_h - s, u_h the spline of
+// the FP64 coefficients u (0 when u is null), s given at the Gauss points as planes
+// [value | d/dx | d/dy (| d/dz)] (0 when sq is null); phi = d^2 (kind 0), |grad d|^2 (1),
+// d^2 + |grad d|^2 (2); `blocks` partial sums in partial[], total in out[0].
 template <typename TT>
-hipError_t launch_norm(int dim, int p, const double* u, const dev_tables<TT>* dirs, int kind,
-                       double* partial, int blocks, double* out, hipStream_t s);
+hipError_t launch_norm(int dim, int p, const double* u, const double* sq, const dev_tables<TT>* dirs,
+                       int kind, double* partial, int blocks, double* out, hipStream_t s);
 
 // Load vector of the L2 projection of f (given at the Gauss points, fq[ix + NX*(iy +
 // NY*iz)], ix = e_x*(p+1) + q_x, NX = elements_x*(p+1)), added into out (FP64)

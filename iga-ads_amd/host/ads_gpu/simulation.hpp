@@ -230,6 +230,12 @@ protected:
     void project_values(int which, const std::vector<double>& fq) {
         check(adsg_project(h_, which, fq.data()), "project");
     }
+    // error / norm functionals of values at the Gauss points (planes: value, d/dx, d/dy[, d/dz])
+    double error_values(int which, int kind, const std::vector<double>& sq) {
+        double v = 0;
+        check(adsg_error(h_, which, kind, sq.data(), &v), "error");
+        return v;
+    }
 
 public:
     ~gpu_simulation() override { adsg_destroy(h_); }
@@ -266,6 +272,37 @@ protected:
         project_values(which, fq);
     }
 
+    // errorL2 / errorH1 (basic_simulation_2d.hpp:318-336) of vector `which` against sol(x, y),
+    // which returns a value_type {val, dx, dy}; which = -1 gives normL2 / normH1 of sol
+    // (:266-290)
+    template <typename Sol>
+    double errorL2(int which, Sol&& sol) { return error_values(which, ADSG_NORM_L2, planes(sol)); }
+    template <typename Sol>
+    double errorH1(int which, Sol&& sol) { return error_values(which, ADSG_NORM_H1, planes(sol)); }
+    template <typename Sol>
+    double normL2(Sol&& sol) { return errorL2(-1, sol); }
+    template <typename Sol>
+    double normH1(Sol&& sol) { return errorH1(-1, sol); }
+    using gpu_simulation::normH1;  // of u
+    using gpu_simulation::normL2;
+
+private:
+    template <typename Sol>
+    std::vector<double> planes(Sol&& sol) {
+        const auto px = quad_points(0), py = quad_points(1);
+        const size_t Q = px.size() * py.size();
+        std::vector<double> sq(3 * Q);
+        for (size_t j = 0; j < py.size(); ++j)
+            for (size_t i = 0; i < px.size(); ++i) {
+                const auto v = sol(px[i], py[j]);
+                const size_t k = i + px.size() * j;
+                sq[k] = v.val;
+                sq[Q + k] = v.dx;
+                sq[2 * Q + k] = v.dy;
+            }
+        return sq;
+    }
+
 public:
     explicit gpu_simulation_2d(const config_2d& c) : gpu_simulation{make(c), c.steps} {}
 };
@@ -298,6 +335,38 @@ protected:
                 for (size_t i = 0; i < px.size(); ++i)
                     fq[i + px.size() * (j + py.size() * k)] = f(px[i], py[j], pz[k]);
         project_values(which, fq);
+    }
+
+    // errorL2 / errorH1 (basic_simulation_3d.hpp:336-400) of vector `which` against
+    // sol(x, y, z) returning {val, dx, dy, dz}; which = -1 gives normL2 / normH1 of sol
+    template <typename Sol>
+    double errorL2(int which, Sol&& sol) { return error_values(which, ADSG_NORM_L2, planes(sol)); }
+    template <typename Sol>
+    double errorH1(int which, Sol&& sol) { return error_values(which, ADSG_NORM_H1, planes(sol)); }
+    template <typename Sol>
+    double normL2(Sol&& sol) { return errorL2(-1, sol); }
+    template <typename Sol>
+    double normH1(Sol&& sol) { return errorH1(-1, sol); }
+    using gpu_simulation::normH1;  // of u
+    using gpu_simulation::normL2;
+
+private:
+    template <typename Sol>
+    std::vector<double> planes(Sol&& sol) {
+        const auto px = quad_points(0), py = quad_points(1), pz = quad_points(2);
+        const size_t Q = px.size() * py.size() * pz.size();
+        std::vector<double> sq(4 * Q);
+        for (size_t k = 0; k < pz.size(); ++k)
+            for (size_t j = 0; j < py.size(); ++j)
+                for (size_t i = 0; i < px.size(); ++i) {
+                    const auto v = sol(px[i], py[j], pz[k]);
+                    const size_t m = i + px.size() * (j + py.size() * k);
+                    sq[m] = v.val;
+                    sq[Q + m] = v.dx;
+                    sq[2 * Q + m] = v.dy;
+                    sq[3 * Q + m] = v.dz;
+                }
+        return sq;
     }
 
 public:

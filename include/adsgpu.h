@@ -151,6 +151,14 @@ int adsg_ads_solve(adsg_sim* h, const int* slots);
 #define ADSG_NORM_H10 1
 #define ADSG_NORM_H1 2
 int adsg_norm(adsg_sim* h, int which, int kind, double* out);
+/* error(u, Ux, Uy, norm, fun) and norm(Ux, Uy, norm, fun) (basic_simulation_2d.hpp:266-362,
+ * basic_simulation_3d.hpp:281-400): out = sqrt(sum_e sum_q w J phi(u_h(x_q) - s(x_q))) with
+ * u_h the spline of vector `which` (ADSG_U or ADSG_U_PREV; which = -1 takes u_h = 0, the
+ * norm of s) and phi of kind ADSG_NORM_L2 / _H10 / _H1 as in adsg_norm. s is given at the
+ * Gauss points (layout of adsg_project) as planes of Q = prod_d elements_d*(p+1) values:
+ * [value] for ADSG_NORM_L2, [value | d/dx | d/dy (| d/dz)] otherwise; host memory. The
+ * reference's relative errors (error_relative_L2/H1, :338-356) are the ratio of the two. */
+int adsg_error(adsg_sim* h, int which, int kind, const double* sq, double* out);
 
 /* projection(v, f) (simulation_2d.hpp:43-46 / simulation_3d.hpp:45-48 -> projector,
  * projection.hpp:11-153): adds the load vector of the L2 projection of f,

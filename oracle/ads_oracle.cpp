@@ -936,6 +936,65 @@ void orc_sim_project(const orc_sim* s, const double* fq, double* v) {
             }
 }
 
+// error(): element loop, Gauss-point loop; uu = eval(u, e, q), d = uu - fun(x),
+// error += norm(d) * w * J
+double orc_sim_error_sq(const orc_sim* s, const double* u, const double* sq, int kind) {
+    const int dim = s->dim;
+    const orc_dim& X = s->d[0];
+    const orc_dim& Y = s->d[1];
+    const orc_dim& Z = s->d[2];
+    const int mx = X.p + 1, my = Y.p + 1, mz = dim == 3 ? Z.p + 1 : 1;
+    const long NX = static_cast<long>(X.elements) * X.q, NY = static_cast<long>(Y.elements) * Y.q;
+    const long Q = NX * NY * (dim == 3 ? static_cast<long>(Z.elements) * Z.q : 1);
+    const long n = element_count(s);
+    int e[3];
+    double err = 0;
+    for (long l = 0; l < n; ++l) {
+        element_index(s, l, e);
+        double J = X.J[e[0]] * Y.J[e[1]];
+        if (dim == 3) J = J * Z.J[e[2]];
+        const int qz_n = dim == 3 ? Z.q : 1;
+        for (int qx = 0; qx < X.q; ++qx)
+            for (int qy = 0; qy < Y.q; ++qy)
+                for (int qz = 0; qz < qz_n; ++qz) {
+                    double w = X.w[qx] * Y.w[qy];
+                    if (dim == 3) w = w * Z.w[qz];
+                    const double* tx = tab(X, e[0], qx);
+                    const double* ty = tab(Y, e[1], qy);
+                    const double* tz = dim == 3 ? tab(Z, e[2], qz) : nullptr;
+                    fval v;
+                    if (u)
+                        for (int bx = 0; bx < mx; ++bx)
+                            for (int by = 0; by < my; ++by)
+                                for (int bz = 0; bz < mz; ++bz) {
+                                    const long gi = (X.first[e[0]] + bx) +
+                                                    X.n * ((Y.first[e[1]] + by) +
+                                                           static_cast<long>(Y.n) * ((dim == 3 ? Z.first[e[2]] : 0) + bz));
+                                    fval B = dim == 3 ? eval_basis3(tx, ty, tz, mx, my, mz, bx, by, bz)
+                                                      : eval_basis2(tx, ty, mx, my, bx, by);
+                                    v.val += u[gi] * B.val;
+                                    v.dx += u[gi] * B.dx;
+                                    v.dy += u[gi] * B.dy;
+                                    v.dz += u[gi] * B.dz;
+                                }
+                    const long pt = (static_cast<long>(e[0]) * X.q + qx) +
+                                    NX * ((static_cast<long>(e[1]) * Y.q + qy) +
+                                          NY * (dim == 3 ? static_cast<long>(e[2]) * Z.q + qz : 0));
+                    fval d = v;
+                    d.val -= sq[pt];
+                    if (kind != 0) {
+                        d.dx -= sq[Q + pt];
+                        d.dy -= sq[2 * Q + pt];
+                        if (dim == 3) d.dz -= sq[3 * Q + pt];
+                    }
+                    const double g2 = d.dx * d.dx + d.dy * d.dy + (dim == 3 ? d.dz * d.dz : 0.0);
+                    const double phi = kind == 0 ? d.val * d.val : kind == 1 ? g2 : d.val * d.val + g2;
+                    err += phi * w * J;
+                }
+    }
+    return err;
+}
+
 int orc_sim_quad_points(const orc_sim* s, int dir, double* out) {
     if (dir < 0 || dir >= s->dim) return -1;
     const orc_dim& D = s->d[dir];

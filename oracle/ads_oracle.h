@@ -129,6 +129,11 @@ void orc_sim_kgrad(const orc_sim* s, const orc_rhs_form* f, const double* u, con
  * f(x_q) B_i(x_q) w_q J_e, f given at the Gauss points: fq[ix + NX*(iy + NY*iz)] with
  * ix = e_x*q + q_x, NX = elements_x*q (likewise y, z). Adds into v. */
 void orc_sim_project(const orc_sim* s, const double* fq, double* v);
+/* error(u, Ux, Uy, norm, fun) (basic_simulation_2d.hpp:318-336, basic_simulation_3d.hpp:
+ * 336-357): sum_e sum_q phi(u_h(x_q) - s(x_q)) w J (squared, no sqrt), u_h the spline of u
+ * (0 when u is NULL), s at the Gauss points as planes [value | d/dx | d/dy (| d/dz)] of
+ * Q values (only the value plane for kind 0); phi: 0 L2, 1 H10, 2 H1. */
+double orc_sim_error_sq(const orc_sim* s, const double* u, const double* sq, int kind);
 /* the Gauss points of direction dir (basis_data x[e][q]), elements*q values */
 int orc_sim_quad_points(const orc_sim* s, int dir, double* out);
 void orc_greville(const double* knot, int knot_size, int p, double* out);

@@ -61,6 +61,8 @@ def lib():
         L.orc_sim_kgrad.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp]
         L.orc_sim_project.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_sim_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_sim_error_sq.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.orc_sim_error_sq.restype = C.c_double
         L.orc_sim_l2norm_sq.argtypes = [C.c_void_p, _dp]
         L.orc_sim_l2norm_sq.restype = C.c_double
         L.orc_sim_norm_sq.argtypes = [C.c_void_p, _dp, C.c_int]
@@ -226,6 +228,12 @@ class Sim:
         v = self.zeros() if v is None else v
         lib().orc_sim_project(self.h, _d(fq), _d(v))
         return v
+
+    def error_sq(self, u, sq, kind=0):
+        """error(u, ..., norm, fun)^2: u may be None (u_h = 0); sq planes at the Gauss points."""
+        sq = np.ascontiguousarray(sq, dtype=np.float64).ravel()
+        up = _d(np.ascontiguousarray(u, dtype=np.float64)) if u is not None else None
+        return lib().orc_sim_error_sq(self.h, up, _d(sq), kind)
 
     def zero_boundary(self, v):
         lib().orc_sim_zero_boundary(self.h, _d(v))

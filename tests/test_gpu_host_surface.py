@@ -90,3 +90,37 @@ def test_implicit_example_program_matches_oracle(tmp_path):
         assert r.returncode == 0, r.stderr
         got = np.fromfile(tmp_path / "out.bin", dtype="<f8")
         assert po.rel_l2(got, ref) <= tol, prec
+
+
+def test_validation_example_matches_oracle():
+    """examples/validation_gpu.cpp, the reference's validation problem (projection of
+    sin(pi x) sin(pi y) in before(), explicit steps with zero Dirichlet data, relative L2
+    and H1 errors against the exact decay in after()), against the same flow on the oracle."""
+    p, n, nsteps = 2, 16, 4000
+    exe = os.path.join(ROOT, "iga-ads_amd", "examples", "bin", "validation_gpu")
+    print(f"case: validation_gpu {p} {n} {nsteps}", flush=True)
+    r = subprocess.run([exe, str(p), str(n), str(nsteps)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    eL2, eH1 = map(float, r.stdout.split())
+    dt, steps = 1.0 / nsteps, nsteps // 10
+    o = po.Sim(2, [p, p], [n, n])
+    o.fix_all_boundaries()
+    o.prepare_matrices()
+    pts = [o.quad_points(d) for d in range(2)]
+    X, Y = [g.transpose() for g in np.meshgrid(*pts, indexing="ij")]
+    u = o.project(np.sin(np.pi * X) * np.sin(np.pi * Y))
+    o.zero_boundary(u)
+    u = o.solve(u)
+    for _ in range(steps):
+        rhs = o.compute_rhs(u, po.heat_form(dt, 2))
+        o.zero_boundary(rhs)
+        u = o.solve(rhs)
+    sc = np.exp(-2 * np.pi ** 2 * dt * steps)
+    sol = np.concatenate([(sc * np.sin(np.pi * X) * np.sin(np.pi * Y)).ravel(),
+                          (sc * np.pi * np.cos(np.pi * X) * np.sin(np.pi * Y)).ravel(),
+                          (sc * np.pi * np.sin(np.pi * X) * np.cos(np.pi * Y)).ravel()])
+    rel = lambda kind: np.sqrt(o.error_sq(u, sol, kind) / o.error_sq(None, sol, kind)) * 100
+    print(f"  gpu {eL2:.12e} {eH1:.12e}  oracle {rel(0):.12e} {rel(2):.12e}", flush=True)
+    assert abs(eL2 - rel(0)) <= 1e-8 * rel(0)
+    assert abs(eH1 - rel(2)) <= 1e-8 * rel(2)
+    assert eL2 < 1.0 and eH1 < 5.0  # the discrete solution tracks the exact decay

@@ -91,3 +91,47 @@ def test_fp32_handle_projects_in_fp64():
     with pytest.raises(ValueError):
         g.project(fq[:-1])
     g.close()
+
+
+def _exact(dim, grid):
+    """s = sin(pi x) sin(pi y) (sin(pi z)) and its gradient at the Gauss points."""
+    s = [np.sin(np.pi * g) for g in grid]
+    c = [np.pi * np.cos(np.pi * g) for g in grid]
+    val = np.prod(s, axis=0)
+    planes = [val]
+    for d in range(dim):
+        planes.append(np.prod([c[i] if i == d else s[i] for i in range(dim)], axis=0))
+    return np.concatenate([p.ravel() for p in planes])
+
+
+@pytest.mark.parametrize("dim,p,ne,a,b", CASES)
+def test_error_functionals_match_oracle(dim, p, ne, a, b):
+    _log(f"case: error functionals dim={dim} p={p} ne={ne}")
+    g, o = _pair(dim, p, ne, a, b)
+    sq = _exact(dim, g.quad_grid())
+    u = np.random.default_rng(2).standard_normal(g.size) * 1e-2
+    g.set(ag.U, u)
+    for kind in (0, 1, 2):
+        for which, uu in ((ag.U, u), (None, None)):
+            got = g.error(sq, kind, which)
+            expect = np.sqrt(o.error_sq(uu, sq, kind))
+            assert abs(got - expect) <= 1e-12 * expect, (kind, which)
+    g.close()
+
+
+def test_error_of_the_projection_converges():
+    """The L2 projection of sin(pi x) sin(pi y) at p = 2 converges at order p + 1 = 3 in L2
+    and p = 2 in H1 (halving h divides the errors by about 8 and 4)."""
+    _log("case: projection error convergence 2D p=2")
+    errs = []
+    for n in (16, 32):
+        g = ag.Simulation(2, [2, 2], [n, n])
+        g.prepare_matrices()
+        g.set(ag.U, np.zeros(g.size))
+        g.project(lambda x, y: np.sin(np.pi * x) * np.sin(np.pi * y))
+        g.ads_solve()
+        sq = _exact(2, g.quad_grid())
+        errs.append((g.error(sq, 0) / g.error(sq, 0, None), g.error(sq, 2) / g.error(sq, 2, None)))
+        g.close()
+    assert 7.0 < errs[0][0] / errs[1][0] < 9.0
+    assert 3.5 < errs[0][1] / errs[1][1] < 4.5

@@ -1,5 +1,6 @@
-"""Pins of the oracle's L2 projection (orc_sim_project, projection.hpp:11-153) before it is
-trusted as the checker of adsg_project. The reference's projector needs basis_data and
+"""Pins of the oracle's L2 projection (orc_sim_project, projection.hpp:11-153) and error
+functional (orc_sim_error_sq, basic_simulation_2d.hpp:318-336) before they are trusted as
+the checkers of adsg_project and adsg_error. The reference's projector needs basis_data and
 Boost, so it is not compiled here (oracle/_ref); the pins are what mathematics fixes:
 
 * a spline f = sum_j c_j B_j is integrated exactly by p+1 Gauss points per direction
@@ -7,7 +8,10 @@ Boost, so it is not compiled here (oracle/_ref); the pins are what mathematics f
   (themselves pinned by closed forms in test_oracle_pins), and the projection followed by
   the Gram solve returns c;
 * f = 1 gives b = M 1, whose entries sum to the volume of the box (partition of unity);
-* the load is linear in f and adds into the given vector.
+* the load is linear in f and adds into the given vector;
+* the error of u against its own spline values (and derivatives) at the Gauss points is 0,
+  against the spline of v it is the norm of u - v (orc_sim_norm_sq, pinned by quadratic
+  forms in test_oracle_rhs), and with u = 0 it is the sum of the load of s^2.
 """
 import numpy as np
 import pytest
@@ -30,16 +34,27 @@ def _dense(ab, n, p):
     return A
 
 
-def _collocation(sim, d):
-    """Phi[e*q + qx, first(e) + a] = B_a(x_q) on element e: the spline's values at the
-    Gauss points of direction d are Phi @ c along that direction."""
+def _collocation(sim, d, der=0):
+    """Phi[e*q + qx, first(e) + a] = B_a(x_q) (der=1: B_a'(x_q)) on element e: the spline's
+    values (derivatives) at the Gauss points of direction d are Phi @ c along d."""
     t = sim.tables(d)
     ne, q = t["x"].shape
     Phi = np.zeros((ne * q, sim.shape[d]))
     for e in range(ne):
         for k in range(q):
-            Phi[e * q + k, t["first"][e]:t["first"][e] + sim.p[d] + 1] = t["b"][e, k, 0, :]
+            Phi[e * q + k, t["first"][e]:t["first"][e] + sim.p[d] + 1] = t["b"][e, k, der, :]
     return Phi
+
+
+def _spline_planes(sim, c):
+    """[value | d/dx | d/dy (| d/dz)] of the spline with coefficients c at the Gauss points."""
+    V = [_collocation(sim, d) for d in range(sim.dim)]
+    D = [_collocation(sim, d, 1) for d in range(sim.dim)]
+    planes = [_apply(V, c, sim.shape)]
+    for d in range(sim.dim):
+        mats = [D[i] if i == d else V[i] for i in range(sim.dim)]
+        planes.append(_apply(mats, c, sim.shape))
+    return np.concatenate(planes)
 
 
 def _apply(mats, c, shape):
@@ -101,3 +116,20 @@ def test_quad_points_are_the_mapped_gauss_points():
         h = (hi - lo) / ne
         expect = np.concatenate([lo + h * e + h * (X + 1) / 2 for e in range(ne)])
         assert np.max(np.abs(s.quad_points(d) - expect)) <= 1e-15
+
+
+@pytest.mark.parametrize("dim,p,ne,a,b", CASES)
+def test_error_functional_pins(dim, p, ne, a, b):
+    s = _sim(dim, p, ne, a, b)
+    rng = np.random.default_rng(11)
+    u, v = rng.standard_normal(s.size), rng.standard_normal(s.size)
+    su, sv = _spline_planes(s, u), _spline_planes(s, v)
+    for kind in (0, 1, 2):
+        ref = s.norm_sq(u, kind)
+        assert s.error_sq(u, su, kind) <= 1e-26 * ref
+        assert abs(s.error_sq(u, sv, kind) - s.norm_sq(u - v, kind)) <= 1e-13 * s.norm_sq(u - v, kind)
+        assert abs(s.error_sq(None, su, kind) - ref) <= 1e-13 * ref
+    # u = 0, kind 0: the integral of s^2, i.e. the sum of the load vector of s^2
+    Q = su.size // (1 + dim)
+    load = s.project(su[:Q] ** 2)
+    assert abs(s.error_sq(None, su[:Q], 0) - load.sum()) <= 1e-13 * load.sum()
