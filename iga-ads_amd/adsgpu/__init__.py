@@ -70,6 +70,7 @@ EXPORTS = [
     "adsg_batch_set_tuning", "adsg_batch_get_tuning", "adsg_batch_autotune",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
     "adsg_autotune", "adsg_project", "adsg_project_device", "adsg_quad_points", "adsg_error",
+    "adsg_sample",
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
@@ -142,8 +143,15 @@ def lib():
         L.adsg_project_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
         L.adsg_error.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.adsg_sample.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, C.c_int, _dp]
         _lib = L
     return _lib
+
+
+def linspace(a, b, n):
+    """util.hpp:28-34: n + 1 points (1 - t) a + t b, t = i / n."""
+    t = np.arange(n + 1, dtype=np.float64) / float(n)
+    return (1.0 - t) * a + t * b
 
 
 class Simulation:
@@ -175,6 +183,8 @@ class Simulation:
         self.dim = dim
         self.p = ps[:dim]
         self.elements = es[:dim]
+        self.a = [a[i] if a else 0.0 for i in range(dim)]
+        self.b = [b[i] if b else 1.0 for i in range(dim)]
         self.precision = precision
         self.shape = tuple(L.adsg_dofs(h, d) for d in range(dim))
         self.size = int(L.adsg_size(h))
@@ -300,6 +310,24 @@ class Simulation:
         self._check(lib().adsg_error(self.h, w, kind, sq.ravel().ctypes.data_as(_dp),
                                      out.ctypes.data_as(_dp)), "error")
         return float(out[0])
+
+    def sample(self, pts=None, intervals=None, ders=False, which=U):
+        """The spline of vector `which` on a tensor grid (bspline::eval / eval_ders): pts a
+        list of abscissae per direction, or `intervals` per direction for the output
+        manager's uniform grid linspace(a, b, intervals) (output/axis.hpp:15-25). Returns an
+        array indexed [z][y][x] (with ders, a leading axis over value, d/dx, d/dy(, d/dz))."""
+        if pts is None:
+            iv = list(intervals) if hasattr(intervals, "__len__") else [intervals] * self.dim
+            pts = [linspace(self.a[d], self.b[d], iv[d]) for d in range(self.dim)]
+        P = [np.ascontiguousarray(p, dtype=np.float64) for p in pts] + [np.zeros(1)] * (3 - len(pts))
+        counts = np.array([len(p) for p in pts] + [1] * (3 - len(pts)), dtype=np.int32)
+        shape = tuple(len(p) for p in pts[::-1])
+        planes = (1 + self.dim) if ders else 1
+        out = np.zeros(planes * int(np.prod(shape)))
+        self._check(lib().adsg_sample(self.h, which, counts.ctypes.data_as(_ip), P[0].ctypes.data_as(_dp),
+                                      P[1].ctypes.data_as(_dp), P[2].ctypes.data_as(_dp), int(ders),
+                                      out.ctypes.data_as(_dp)), "sample")
+        return out.reshape((planes,) + shape) if ders else out.reshape(shape)
 
     def project(self, f, which=U):
         """projection(v, f) (projection.hpp:11-153): add the L2 projection's load vector of f

@@ -1247,6 +1247,69 @@ int adsg_project_device(adsg_sim* h, int which, const double* fq) {
     return ADSG_OK;
 }
 
+int adsg_sample(adsg_sim* h, int which, const int* counts, const double* px, const double* py,
+                const double* pz, int ders, double* out) {
+    const double* pts[3] = {px, py, pz};
+    if (!h || !counts || !out || !projectable(h, which) || !h->vec[which])
+        return fail(h, ADSG_EINVAL, "sample: bad argument (an FP64 vector that is set)");
+    long total = 1;
+    for (int d = 0; d < h->dim; ++d) {
+        if (counts[d] <= 0 || !pts[d]) return fail(h, ADSG_EINVAL, "sample: empty direction");
+        total *= counts[d];
+    }
+    const int p = h->dirs[0].tab.p, m = p + 1;
+    // per direction: [point][value 0..p | derivative 0..p] and the first nonzero function
+    std::vector<double> tab[3];
+    std::vector<int> off[3];
+    for (int d = 0; d < h->dim; ++d) {
+        const auto& knot = h->dirs[d].tab.knot;
+        tab[d].resize(static_cast<size_t>(counts[d]) * 2 * m);
+        off[d].resize(counts[d]);
+        for (int i = 0; i < counts[d]; ++i) {
+            const int span = adsg::span_of(pts[d][i], knot, p);
+            adsg::basis_and_derivative(span, pts[d][i], knot, p, &tab[d][static_cast<size_t>(i) * 2 * m],
+                                       &tab[d][static_cast<size_t>(i) * 2 * m + m]);
+            off[d][i] = span - p;
+        }
+    }
+    const size_t planes = ders ? 1 + h->dim : 1;
+    size_t bytes = planes * total * sizeof(double);
+    for (int d = 0; d < h->dim; ++d) bytes += tab[d].size() * sizeof(double) + off[d].size() * sizeof(int) + 256;
+    char* buf = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(&buf), bytes) != hipSuccess)
+        return fail(h, ADSG_ENOMEM, "sample: no memory for the grid");
+    int rc = ADSG_OK;
+    double* dout = reinterpret_cast<double*>(buf);
+    size_t at = planes * total * sizeof(double);
+    const double* dtab[3] = {nullptr, nullptr, nullptr};
+    const int* doff[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < h->dim && rc == ADSG_OK; ++d) {
+        at = (at + 255) / 256 * 256;
+        dtab[d] = reinterpret_cast<const double*>(buf + at);
+        hipError_t e = hipMemcpyAsync(buf + at, tab[d].data(), tab[d].size() * sizeof(double),
+                                      hipMemcpyHostToDevice, h->stream);
+        at += tab[d].size() * sizeof(double);
+        doff[d] = reinterpret_cast<const int*>(buf + at);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(buf + at, off[d].data(), off[d].size() * sizeof(int), hipMemcpyHostToDevice, h->stream);
+        at += off[d].size() * sizeof(int);
+        if (e != hipSuccess) rc = hip_fail(h, e, "sample: upload");
+    }
+    if (rc == ADSG_OK) {
+        const int c3[3] = {counts[0], counts[1], h->dim == 3 ? counts[2] : 1};
+        hipError_t e = adsg::launch_sample(h->dim, p, static_cast<const double*>(h->vec[which]),
+                                           h->dirs[0].tab.dofs, h->dirs[1].tab.dofs, dtab, doff, c3,
+                                           ders ? 1 : 0, dout, h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, dout, planes * total * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = hip_fail(h, e, "sample");
+    }
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(buf);
+    return rc;
+}
+
 int adsg_quad_points(const adsg_sim* h, int dir, double* x) {
     if (!h || !x || dir < 0 || dir >= h->dim) return ADSG_EINVAL;
     const auto& xq = h->dirs[dir].tab.xq;

@@ -157,6 +157,13 @@ template <typename TT>
 hipError_t launch_project(int dim, int p, const double* fq, double* out, const dev_tables<TT>* dirs,
                           hipStream_t s);
 
+// The spline u (FP64, nx x ny [x nz] coefficients) on a tensor grid of counts[d] points per
+// direction: tab[d] holds per point the p+1 basis values then the p+1 derivatives, off[d]
+// the first nonzero function (span - p); out: values, then with ders the d/dx, d/dy[, d/dz]
+// planes, x fastest
+hipError_t launch_sample(int dim, int p, const double* u, long nx, long ny, const double* const* tab,
+                         const int* const* off, const int* counts, int ders, double* out, hipStream_t s);
+
 template <typename T>
 hipError_t launch_fill(T* p, long n, T v, hipStream_t s);
 

@@ -1,4 +1,4 @@
-// HIP kernel of the L2 projection's load vector (gfx950): projection(v, f)
+// HIP kernels of the L2 projection's load vector and of grid sampling (gfx950). projection(v, f)
 // (simulation_2d.hpp:43-46, simulation_3d.hpp:45-48 -> projection.hpp:11-153) adds
 //   v_i += sum_e sum_q f(x_q) B_i(x_q) w_q J_e
 // for f given at the Gauss points. One lane per element, sum-factorised over the Gauss
@@ -98,7 +98,74 @@ hipError_t project_dim(int dim, const double* fq, double* out, const dev_tables<
     return hipErrorInvalidValue;
 }
 
+// The spline u on a tensor grid of points (bspline/eval.hpp:73-234): per direction and
+// point, off[i] = span - p and tab[i] = the p+1 nonzero basis values then their derivatives;
+// one lane per point (x fastest), FP64. out: values, and with ders the derivative planes.
+template <int DIM, int P>
+__global__ __launch_bounds__(256) void k_sample(const double* __restrict__ u, long nx, long ny,
+                                                const double* __restrict__ tx, const int* __restrict__ ox,
+                                                const double* __restrict__ ty, const int* __restrict__ oy,
+                                                const double* __restrict__ tz, const int* __restrict__ oz,
+                                                int cx, int cy, int cz, int ders, double* __restrict__ out) {
+    constexpr int m = P + 1;
+    constexpr int MZ = DIM == 3 ? m : 1;
+    const long total = static_cast<long>(cx) * cy * cz;
+    const long pt = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (pt >= total) return;
+    const int i = static_cast<int>(pt % cx);
+    const long r = pt / cx;
+    const int j = static_cast<int>(r % cy), k = static_cast<int>(r / cy);
+    const double* bx = tx + static_cast<long>(i) * 2 * m;
+    const double* by = ty + static_cast<long>(j) * 2 * m;
+    const double* bz = DIM == 3 ? tz + static_cast<long>(k) * 2 * m : nullptr;
+    const int x0 = ox[i], y0 = oy[j], z0 = DIM == 3 ? oz[k] : 0;
+    double v = 0, gx = 0, gy = 0, gz = 0;
+    for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b)
+            for (int c = 0; c < MZ; ++c) {
+                const double cu = u[(x0 + a) + nx * ((y0 + b) + ny * static_cast<long>(z0 + c))];
+                const double vz = DIM == 3 ? bz[c] : 1.0, dz = DIM == 3 ? bz[m + c] : 0.0;
+                v += cu * bx[a] * by[b] * vz;
+                gx += cu * bx[m + a] * by[b] * vz;
+                gy += cu * bx[a] * by[m + b] * vz;
+                gz += cu * bx[a] * by[b] * dz;
+            }
+    out[pt] = v;
+    if (ders) {
+        out[total + pt] = gx;
+        out[2 * total + pt] = gy;
+        if (DIM == 3) out[3 * total + pt] = gz;
+    }
+}
+
+template <int DIM>
+hipError_t sample_dim(int p, const double* u, long nx, long ny, const double* const* tab,
+                      const int* const* off, const int* counts, int ders, double* out, hipStream_t s) {
+    const long total = static_cast<long>(counts[0]) * counts[1] * (DIM == 3 ? counts[2] : 1);
+    const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
+    const int cz = DIM == 3 ? counts[2] : 1;
+#define ADSG_SAMPLE(P_)                                                                             \
+    if (p == P_) {                                                                                  \
+        hipLaunchKernelGGL((k_sample<DIM, P_>), dim3(blocks), dim3(256), 0, s, u, nx, ny, tab[0], off[0], \
+                           tab[1], off[1], tab[2], off[2], counts[0], counts[1], cz, ders, out);  \
+        return hipGetLastError();                                                                   \
+    }
+    ADSG_SAMPLE(1)
+    ADSG_SAMPLE(2)
+    ADSG_SAMPLE(3)
+#undef ADSG_SAMPLE
+    return hipErrorInvalidValue;
+}
+
 }  // namespace
+
+hipError_t launch_sample(int dim, int p, const double* u, long nx, long ny, const double* const* tab,
+                         const int* const* off, const int* counts, int ders, double* out, hipStream_t s) {
+    if (counts[0] <= 0 || counts[1] <= 0 || (dim == 3 && counts[2] <= 0)) return hipSuccess;
+    if (dim == 2) return sample_dim<2>(p, u, nx, ny, tab, off, counts, ders, out, s);
+    if (dim == 3) return sample_dim<3>(p, u, nx, ny, tab, off, counts, ders, out, s);
+    return hipErrorInvalidValue;
+}
 
 template <typename TT>
 hipError_t launch_project(int dim, int p, const double* fq, double* out, const dev_tables<TT>* dirs,

@@ -5,12 +5,19 @@
 // problem's own solve does), each step assembles the explicit rhs, zeroes its boundary rows
 // and solves, and after() prints the relative L2 and H1 errors at T in percent.
 //
-//   validation_gpu <p> <n> <nsteps>     (T = 1, dt = T / nsteps, nsteps / 10 steps run)
+//   validation_gpu <p> <n> <nsteps> [out.data]   (T = 1, dt = T / nsteps, nsteps / 10 steps)
+//
+// With out.data, the final solution is also sampled on the output manager's grid of 200
+// intervals per side (validation.hpp:24, output_manager.hpp:52-75) and written as gnuplot
+// rows.
 #include <cmath>
 #include <cstdlib>
+#include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <string>
 
+#include "ads_gpu/output.hpp"
 #include "ads_gpu/simulation.hpp"
 
 namespace {
@@ -20,8 +27,10 @@ struct value_type {
 };
 
 class validation : public ads_gpu::gpu_simulation_2d {
+    std::string out_;
+
 public:
-    explicit validation(const ads_gpu::config_2d& c) : gpu_simulation_2d{c} {}
+    validation(const ads_gpu::config_2d& c, std::string out) : gpu_simulation_2d{c}, out_{std::move(out)} {}
 
 private:
     static constexpr double k = 2 * M_PI * M_PI;
@@ -63,6 +72,11 @@ private:
         const double eL2 = errorL2(ADSG_U, sol) / normL2(sol) * 100;
         const double eH1 = errorH1(ADSG_U, sol) / normH1(sol) * 100;
         std::cout << std::setprecision(17) << eL2 << "  " << eH1 << std::endl;
+        if (!out_.empty()) {
+            const auto xs = ads_gpu::output::linspace(0, 1, 200), ys = xs;
+            std::ofstream os{out_};
+            ads_gpu::output::gnuplot_2d(os, xs, ys, sample(ADSG_U, xs, ys));
+        }
     }
 };
 
@@ -87,7 +101,7 @@ int main(int argc, char* argv[]) {
         ads_gpu::dim_config dim{p, n};
         ads_gpu::timesteps_config steps{nsteps, dt};
         ads_gpu::config_2d c{dim, dim, steps, 1, ADSG_FP64};
-        validation sim{c};
+        validation sim{c, argc > 4 ? argv[4] : ""};
         sim.run();
     } catch (const std::exception& e) {
         std::cerr << "validation_gpu: " << e.what() << std::endl;

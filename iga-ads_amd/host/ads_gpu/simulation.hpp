@@ -230,6 +230,19 @@ protected:
     void project_values(int which, const std::vector<double>& fq) {
         check(adsg_project(h_, which, fq.data()), "project");
     }
+    // bspline::eval / eval_ders of vector `which` on the tensor grid px x py [x pz] (x fastest;
+    // with ders the derivative planes follow the values)
+    std::vector<double> sample(int which, const std::vector<double>& px, const std::vector<double>& py,
+                               const std::vector<double>& pz = {}, bool ders = false) {
+        const int counts[3] = {static_cast<int>(px.size()), static_cast<int>(py.size()),
+                               dim_ == 3 ? static_cast<int>(pz.size()) : 1};
+        const size_t total = px.size() * py.size() * (dim_ == 3 ? pz.size() : 1);
+        std::vector<double> out(total * (ders ? 1 + dim_ : 1));
+        check(adsg_sample(h_, which, counts, px.data(), py.data(), dim_ == 3 ? pz.data() : nullptr,
+                          ders ? 1 : 0, out.data()),
+              "sample");
+        return out;
+    }
     // error / norm functionals of values at the Gauss points (planes: value, d/dx, d/dy[, d/dz])
     double error_values(int which, int kind, const std::vector<double>& sq) {
         double v = 0;

@@ -172,6 +172,13 @@ int adsg_error(adsg_sim* h, int which, int kind, const double* sq, double* out);
  * adsg_project reads fq from host memory, adsg_project_device from device memory. */
 int adsg_project(adsg_sim* h, int which, const double* fq);
 int adsg_project_device(adsg_sim* h, int which, const double* fq);
+/* bspline::eval / eval_ders (bspline/eval.hpp:73-234), as output_manager samples a solution
+ * (output_manager.hpp:28-146): the spline of vector `which` (an FP64 vector) on the tensor
+ * grid px[0..counts[0]) x py [x pz] (host abscissae; points outside [a, b] use the end
+ * elements, as find_span does), into out (host): prod(counts) values, x fastest, and with
+ * ders = 1 the d/dx, d/dy [, d/dz] planes after them. */
+int adsg_sample(adsg_sim* h, int which, const int* counts, const double* px, const double* py,
+                const double* pz, int ders, double* out);
 /* the Gauss points of direction dir (basis_data x[e][q], basis_data.cpp:63-114):
  * elements*(p+1) abscissae, element-major. */
 int adsg_quad_points(const adsg_sim* h, int dir, double* x);

@@ -995,6 +995,64 @@ double orc_sim_error_sq(const orc_sim* s, const double* u, const double* sq, int
     return err;
 }
 
+// eval: span by find_span, the p+1 nonzero functions (and derivatives) by A2.3, offset
+// span - p, value = sum over the tensor product of local functions (x outermost)
+void orc_sim_sample(const orc_sim* s, const double* u, const int* counts, const double* px,
+                    const double* py, const double* pz, int ders, double* out) {
+    const int dim = s->dim;
+    const double* pts[3] = {px, py, pz};
+    std::vector<std::vector<double>> tabv(dim), tabd(dim);
+    std::vector<std::vector<int>> off(dim);
+    for (int d = 0; d < dim; ++d) {
+        const orc_dim& D = s->d[d];
+        const int m = D.p + 1;
+        tabv[d].resize(static_cast<size_t>(counts[d]) * m);
+        tabd[d].resize(static_cast<size_t>(counts[d]) * m);
+        off[d].resize(counts[d]);
+        std::vector<double> buf(2 * m);
+        for (int i = 0; i < counts[d]; ++i) {
+            const int span = orc_find_span(pts[d][i], D.knot.data(), static_cast<int>(D.knot.size()), D.p);
+            orc_eval_basis_with_derivatives(span, pts[d][i], D.knot.data(), D.p, 1, buf.data());
+            for (int a = 0; a < m; ++a) {
+                tabv[d][static_cast<size_t>(i) * m + a] = buf[a];
+                tabd[d][static_cast<size_t>(i) * m + a] = buf[m + a];
+            }
+            off[d][i] = span - D.p;
+        }
+    }
+    const int cz = dim == 3 ? counts[2] : 1;
+    const long P = static_cast<long>(counts[0]) * counts[1] * cz;
+    const int mx = s->d[0].p + 1, my = s->d[1].p + 1, mz = dim == 3 ? s->d[2].p + 1 : 1;
+    const long nx = s->d[0].n, ny = s->d[1].n;
+    for (int k = 0; k < cz; ++k)
+        for (int j = 0; j < counts[1]; ++j)
+            for (int i = 0; i < counts[0]; ++i) {
+                double v = 0, gx = 0, gy = 0, gz = 0;
+                for (int a = 0; a < mx; ++a)
+                    for (int b = 0; b < my; ++b)
+                        for (int c = 0; c < mz; ++c) {
+                            const long gi = (off[0][i] + a) + nx * ((off[1][j] + b) + ny * (dim == 3 ? off[2][k] + c : 0));
+                            const double bx = tabv[0][static_cast<size_t>(i) * mx + a];
+                            const double by = tabv[1][static_cast<size_t>(j) * my + b];
+                            const double bz = dim == 3 ? tabv[2][static_cast<size_t>(k) * mz + c] : 1.0;
+                            const double dx = tabd[0][static_cast<size_t>(i) * mx + a];
+                            const double dy = tabd[1][static_cast<size_t>(j) * my + b];
+                            const double dz = dim == 3 ? tabd[2][static_cast<size_t>(k) * mz + c] : 0.0;
+                            v += u[gi] * bx * by * bz;
+                            gx += u[gi] * dx * by * bz;
+                            gy += u[gi] * bx * dy * bz;
+                            gz += u[gi] * bx * by * dz;
+                        }
+                const long pt = i + static_cast<long>(counts[0]) * (j + static_cast<long>(counts[1]) * k);
+                out[pt] = v;
+                if (ders) {
+                    out[P + pt] = gx;
+                    out[2 * P + pt] = gy;
+                    if (dim == 3) out[3 * P + pt] = gz;
+                }
+            }
+}
+
 int orc_sim_quad_points(const orc_sim* s, int dir, double* out) {
     if (dir < 0 || dir >= s->dim) return -1;
     const orc_dim& D = s->d[dir];

@@ -134,6 +134,11 @@ void orc_sim_project(const orc_sim* s, const double* fq, double* v);
  * (0 when u is NULL), s at the Gauss points as planes [value | d/dx | d/dy (| d/dz)] of
  * Q values (only the value plane for kind 0); phi: 0 L2, 1 H10, 2 H1. */
 double orc_sim_error_sq(const orc_sim* s, const double* u, const double* sq, int kind);
+/* bspline::eval / eval_ders (bspline/eval.hpp:73-234) of the spline u on the tensor grid
+ * px[0..counts[0]) x py x pz (x fastest): out[pt] the value; with ders, planes
+ * [value | d/dx | d/dy (| d/dz)] of prod(counts) values each. */
+void orc_sim_sample(const orc_sim* s, const double* u, const int* counts, const double* px,
+                    const double* py, const double* pz, int ders, double* out);
 /* the Gauss points of direction dir (basis_data x[e][q]), elements*q values */
 int orc_sim_quad_points(const orc_sim* s, int dir, double* out);
 void orc_greville(const double* knot, int knot_size, int p, double* out);

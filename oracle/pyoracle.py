@@ -63,6 +63,7 @@ def lib():
         L.orc_sim_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_sim_error_sq.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
         L.orc_sim_error_sq.restype = C.c_double
+        L.orc_sim_sample.argtypes = [C.c_void_p, _dp, _ip, _dp, _dp, _dp, C.c_int, _dp]
         L.orc_sim_l2norm_sq.argtypes = [C.c_void_p, _dp]
         L.orc_sim_l2norm_sq.restype = C.c_double
         L.orc_sim_norm_sq.argtypes = [C.c_void_p, _dp, C.c_int]
@@ -106,6 +107,9 @@ def ref_lib():
         R.ref_solve_with_factorized.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, C.c_int]
         R.ref_cyclic_transpose.argtypes = [C.c_int, _ip, _dp, _dp]
         R.ref_ads_solve.argtypes = [C.c_int, _ip, _dp, C.POINTER(_dp), C.POINTER(_ip), _ip, _ip]
+        R.ref_eval.argtypes = [C.c_int, _dp, _dp, C.POINTER(_dp), _ip, _ip, C.c_int, _dp]
+        R.ref_gnuplot2.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_char_p, C.c_long]
+        R.ref_gnuplot2.restype = C.c_long
     return _ref
 
 
@@ -234,6 +238,17 @@ class Sim:
         sq = np.ascontiguousarray(sq, dtype=np.float64).ravel()
         up = _d(np.ascontiguousarray(u, dtype=np.float64)) if u is not None else None
         return lib().orc_sim_error_sq(self.h, up, _d(sq), kind)
+
+    def sample(self, u, pts, ders=False):
+        """The spline u on the tensor grid pts[0] x pts[1] (x pts[2]), x fastest; with ders
+        the planes [value, d/dx, d/dy(, d/dz)]."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        P = [np.ascontiguousarray(p, dtype=np.float64) for p in pts] + [np.zeros(1)] * (3 - len(pts))
+        counts = np.array([len(p) for p in pts] + [1] * (3 - len(pts)), dtype=np.int32)
+        n = int(np.prod(counts))
+        out = np.zeros(n * ((1 + self.dim) if ders else 1))
+        lib().orc_sim_sample(self.h, _d(u), _i(counts), _d(P[0]), _d(P[1]), _d(P[2]), int(ders), _d(out))
+        return out
 
     def zero_boundary(self, v):
         lib().orc_sim_zero_boundary(self.h, _d(v))

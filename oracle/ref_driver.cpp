@@ -9,13 +9,19 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "ads/bspline/bspline.hpp"
+#include "ads/bspline/eval.hpp"
 #include "ads/lin/band_matrix.hpp"
 #include "ads/lin/band_solve.hpp"
 #include "ads/lin/solver_ctx.hpp"
 #include "ads/lin/tensor.hpp"
+#include "ads/output/gnuplot.hpp"
+#include "ads/output/grid.hpp"
+#include "ads/output/range.hpp"
 #include "ads/quad/gauss.hpp"
 #include "ads/solver.hpp"
 
@@ -25,6 +31,18 @@ namespace lin = ads::lin;
 static bsp::basis make_basis(const double* knot, int knot_size, int p) {
     return bsp::basis{bsp::knot_vector(knot, knot + knot_size), p};
 }
+
+// coefficient tensors (first index fastest) as the callables bspline::eval reads
+struct coeffs2 {
+    const double* u;
+    int nx;
+    double operator()(int i, int j) const { return u[i + static_cast<long>(nx) * j]; }
+};
+struct coeffs3 {
+    const double* u;
+    int nx, ny;
+    double operator()(int i, int j, int k) const { return u[i + static_cast<long>(nx) * (j + static_cast<long>(ny) * k)]; }
+};
 
 extern "C" {
 
@@ -143,6 +161,56 @@ void ref_ads_solve(int rank, const int* sizes, double* rhs, const double* const*
                        ads::dim_data{M[2], C[2]});
         std::memcpy(rhs, r.data(), sizeof(double) * total);
     }
+}
+
+// bspline::eval / eval_ders (bspline/eval.hpp:104-234): the spline of u (first index
+// fastest) at (x, y[, z]); out = value (ders = 0) or value, d/dx, d/dy[, d/dz] (ders = 1)
+void ref_eval(int dim, const double* pt, const double* u, const double* const* knot,
+              const int* knot_size, const int* p, int ders, double* out) {
+    std::vector<bsp::basis> b;
+    for (int d = 0; d < dim; ++d) b.push_back(make_basis(knot[d], knot_size[d], p[d]));
+    if (dim == 2) {
+        coeffs2 U{u, b[0].dofs()};
+        if (!ders) {
+            bsp::eval_ctx cx(p[0]), cy(p[1]);
+            out[0] = bsp::eval(pt[0], pt[1], U, b[0], b[1], cx, cy);
+        } else {
+            bsp::eval_ders_ctx cx(p[0], 1), cy(p[1], 1);
+            auto v = bsp::eval_ders(pt[0], pt[1], U, b[0], b[1], cx, cy);
+            out[0] = v.val;
+            out[1] = v.dx;
+            out[2] = v.dy;
+        }
+    } else {
+        coeffs3 U{u, b[0].dofs(), b[1].dofs()};
+        if (!ders) {
+            bsp::eval_ctx cx(p[0]), cy(p[1]), cz(p[2]);
+            out[0] = bsp::eval(pt[0], pt[1], pt[2], U, b[0], b[1], b[2], cx, cy, cz);
+        } else {
+            bsp::eval_ders_ctx cx(p[0], 1), cy(p[1], 1), cz(p[2], 1);
+            auto v = bsp::eval_ders(pt[0], pt[1], pt[2], U, b[0], b[1], b[2], cx, cy, cz);
+            out[0] = v.val;
+            out[1] = v.dx;
+            out[2] = v.dy;
+            out[3] = v.dz;
+        }
+    }
+}
+
+// output::gnuplot_printer<2> with the output manager's format (fixed_format(10, 18),
+// output_manager.hpp:23, gnuplot.hpp:39-55): vals[i + nx*j] at (xs[i], ys[j]); the text is
+// copied into out (at most cap bytes) and its length returned
+long ref_gnuplot2(const double* xs, int nx, const double* ys, int ny, const double* vals, char* out,
+                  long cap) {
+    std::vector<double> X(xs, xs + nx), Y(ys, ys + ny);
+    auto grid = ads::output::make_grid(ads::output::from_container(X), ads::output::from_container(Y));
+    ads::output::gnuplot_printer<2> printer{ads::output::fixed_format(10, 18)};
+    auto value = [&](int i, int j) { return vals[i + static_cast<long>(nx) * j]; };
+    std::ostringstream os;
+    printer.print(os, grid, value);
+    const std::string t = os.str();
+    if (out) std::memcpy(out, t.data(), std::min<long>(cap, static_cast<long>(t.size())));
+    return static_cast<long>(t.size());
 }
 
 }  // extern "C"

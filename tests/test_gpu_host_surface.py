@@ -92,14 +92,16 @@ def test_implicit_example_program_matches_oracle(tmp_path):
         assert po.rel_l2(got, ref) <= tol, prec
 
 
-def test_validation_example_matches_oracle():
+def test_validation_example_matches_oracle(tmp_path):
     """examples/validation_gpu.cpp, the reference's validation problem (projection of
     sin(pi x) sin(pi y) in before(), explicit steps with zero Dirichlet data, relative L2
     and H1 errors against the exact decay in after()), against the same flow on the oracle."""
     p, n, nsteps = 2, 16, 4000
     exe = os.path.join(ROOT, "iga-ads_amd", "examples", "bin", "validation_gpu")
     print(f"case: validation_gpu {p} {n} {nsteps}", flush=True)
-    r = subprocess.run([exe, str(p), str(n), str(nsteps)], capture_output=True, text=True, timeout=120)
+    data = tmp_path / "out.data"
+    r = subprocess.run([exe, str(p), str(n), str(nsteps), str(data)], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     eL2, eH1 = map(float, r.stdout.split())
     dt, steps = 1.0 / nsteps, nsteps // 10
@@ -124,3 +126,12 @@ def test_validation_example_matches_oracle():
     assert abs(eL2 - rel(0)) <= 1e-8 * rel(0)
     assert abs(eH1 - rel(2)) <= 1e-8 * rel(2)
     assert eL2 < 1.0 and eH1 < 5.0  # the discrete solution tracks the exact decay
+    # the gnuplot rows of the final state on the output manager's 200-interval grid
+    from adsgpu import linspace, output
+    xs = linspace(0.0, 1.0, 200)
+    vals = o.sample(u, [xs, xs]).reshape(xs.size, xs.size)
+    want = np.array([[float(v) for v in l.split()] for l in output.gnuplot_2d(xs, xs, vals).splitlines()])
+    text = data.read_text()
+    got = np.array([[float(v) for v in l.split()] for l in text.splitlines()])
+    assert got.shape == (201 * 201, 3) and np.max(np.abs(got - want)) <= 2e-10
+    assert all(len(l) == 54 for l in text.splitlines())
