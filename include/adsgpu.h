@@ -121,7 +121,10 @@ int adsg_prepare_matrices(adsg_sim* h);
 /* copy a vector in/out (FP64 host data whatever the precision). */
 int adsg_set_vector(adsg_sim* h, int which, const double* host);
 int adsg_get_vector(adsg_sim* h, int which, double* host);
-/* device pointer of a vector (double* or float* by precision), for zero-copy interop */
+/* device pointer of a vector (double* or float* by precision), for zero-copy interop.
+ * Steps with a source use its load vector (integral of f v), computed once per source:
+ * taking ADSG_SOURCE's pointer marks it stale, so write the source through the pointer
+ * before the next step, or take the pointer again after writing. */
 void* adsg_device_ptr(adsg_sim* h, int which);
 /* the same copies from/to FP64 device memory of the caller (e.g. a framework's tensors);
  * they complete before returning. */
