@@ -362,3 +362,27 @@ def test_device_norms_match_oracle(dim, p, ne, prec):
     for kind in range(3):
         ref = np.sqrt(o.norm_sq(u, kind))
         assert abs(g.norm(kind) - ref) <= tol * ref, kind
+
+
+def test_source_changes_reach_the_cached_load_vector():
+    """Forms with a source start from dt * (load vector of f), cached per source: a new
+    source set from the host, or written through adsg_device_ptr, is used by the next call."""
+    import ctypes as C
+
+    _log("case: source load vector refresh 3D p=2")
+    g, o = _pair(3, 2, (12, 10, 9))
+    rng = np.random.default_rng(23)
+    c = rng.standard_normal(g.size)
+    g.set(ag.U_PREV, c)
+    form = ag.rhs_form(1e-3, source=True)
+    for k in range(2):
+        f = rng.standard_normal(g.size)
+        g.set(ag.SOURCE, f)
+        g.compute_rhs(form)
+        assert po.rel_l2(g.get(ag.U), o.compute_rhs(c, po.heat_form(1e-3, 3, source=f))) < 1e-13, k
+    f = rng.standard_normal(g.size)
+    ptr = g.device_ptr(ag.SOURCE)
+    hip = C.CDLL("libamdhip64.so")
+    assert hip.hipMemcpy(C.c_void_p(ptr), f.ctypes.data_as(C.c_void_p), C.c_size_t(f.nbytes), 1) == 0
+    g.compute_rhs(form)
+    assert po.rel_l2(g.get(ag.U), o.compute_rhs(c, po.heat_form(1e-3, 3, source=f))) < 1e-13
