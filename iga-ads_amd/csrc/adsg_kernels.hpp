@@ -38,8 +38,8 @@ hipError_t launch_line_sweep(T* u, const line_geometry& g, const dev_factors<F>&
 template <typename F>
 struct seg_factors {
     dev_factors<F> f;
-    const F* pack = nullptr;   // [nseg*m][2(kl+ku)+1]
-    const F* packt = nullptr;  // [nseg*m][2(kl+ku)+1]
+    const F* pack = nullptr;   // [nseg*mseg][2(kl+ku)+1]: mseg rows per segment
+    const F* packt = nullptr;  // [nseg*mseg][2(kl+ku)+1]
     const double* arow = nullptr;  // [n*(kl+ku+1)]
     int m = 0, nseg = 0, mseg = 0;  // mseg: register capacity per lane (16 or 32)
     bool transposed = false, refine = false;

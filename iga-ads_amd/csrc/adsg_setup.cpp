@@ -379,15 +379,17 @@ bool segment_lines(const line_factors& f, line_segments& out, int max_nseg16, in
             }
         }
     }
-    // packed rows for the sweep kernels (see adsg_setup.hpp), padded to nseg*m rows with
-    // identity rows so that every segment runs m uniform steps
-    const int np = nseg * m, w = 2 * (kl + ku) + 1;
+    // packed rows for the sweep kernels (see adsg_setup.hpp): segment s owns the block of
+    // mseg rows s*mseg .. s*mseg+mseg-1, its m rows first, then identity rows (no coupling,
+    // unit diagonal, no influence), so that every segment runs mseg uniform steps
+    const int np = nseg * mseg, w = 2 * (kl + ku) + 1;
     out.pack.assign(static_cast<size_t>(np) * w, 0.0);
     out.packt.assign(static_cast<size_t>(np) * w, 0.0);
-    for (int j = 0; j < np; ++j) {
-        double* r = &out.pack[static_cast<size_t>(j) * w];
-        double* rt = &out.packt[static_cast<size_t>(j) * w];
-        if (j >= n) {
+    for (int row = 0; row < np; ++row) {
+        const int sg = row / mseg, k = row % mseg, j = sg * m + k;
+        double* r = &out.pack[static_cast<size_t>(row) * w];
+        double* rt = &out.packt[static_cast<size_t>(row) * w];
+        if (k >= m || j >= n) {
             r[kl + ku] = rt[ku + kl] = 1.0;
             continue;
         }

@@ -68,17 +68,17 @@ struct line_segments {
     int m = 0, nseg = 0, mseg = 0;
     std::vector<double> gf, hb;    // A = L U
     std::vector<double> gft, hbt;  // A^T = U^T L^T: gft [n*ku], hbt [n*kl]
-    // Everything row j of a sweep reads, packed per row (w = 2(kl+ku)+1 values) over
-    // nseg*m rows (rows >= n are identity rows): for A = L U
+    // Everything row j of a sweep reads, packed per row (w = 2(kl+ku)+1 values) in blocks
+    // of mseg rows per segment (row j = s*m + k at s*mseg + k; the rows k >= m of a block
+    // and the rows j >= n are identity rows): for A = L U
     //   [L(j,j-t), t=1..kl | U(j,j+t), t=1..ku | 1/U(j,j) | gf[j][0..kl) | hb[j][0..ku)]
     // and for A^T (packt)
     //   [U(j-t,j), t=1..ku | L(j+t,j), t=1..kl | 1/U(j,j) | gft[j][0..ku) | hbt[j][0..kl)]
     // with zeros where the band leaves the matrix.
     std::vector<double> pack, packt;
 };
-// The largest segment count <= max_nseg16 with segments of <= 16 rows, else the largest
-// <= max_nseg32 with segments of <= 32 rows (the callers' limits keep the kernels' LDS within
-// budget).
+// The fewest segments of <= 16 rows if that count is <= max_nseg16, else the fewest of
+// <= 32 rows if <= max_nseg32 (the callers' limits keep the kernels' LDS within budget).
 bool segment_lines(const line_factors& f, line_segments& out, int max_nseg16, int max_nseg32);
 
 }  // namespace adsg
