@@ -34,9 +34,10 @@ def f_elem(d, p):
     return 2 * c * 2 * n ** (d + 1) + (d + 2) * n ** d
 
 
-def run_form(ag, precision, ne, steps, warmup, u0, dt, variant=None):
+def run_form(ag, precision, ne, steps, warmup, u0, dt, variant=None, kron=False):
     """Plain explicit heat step (c2 form) in the given precision: autotune, warm up, time
-    `steps` graph-replayed steps, then time the phases of 3 eager steps."""
+    `steps` graph-replayed steps, then time the phases of 3 eager steps. kron: the rhs of
+    this separable integrand through the 1D matrices instead of the element kernels."""
     sim = ag.Simulation(3, [2, 2, 2], [ne, ne, ne], precision=precision)
     sim.prepare_matrices()
     sim.set(ag.U, u0)
@@ -45,6 +46,7 @@ def run_form(ag, precision, ne, steps, warmup, u0, dt, variant=None):
         variant = sim.autotune(form)  # picks the element-kernel variant; state is restored
     else:
         sim.set_tuning(ag.TUNE_RHS_VARIANT, variant)
+    sim.set_tuning(ag.TUNE_RHS_KRONECKER, 1 if kron else 0)
     sim.step(warmup, form)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -57,7 +59,7 @@ def run_form(ag, precision, ne, steps, warmup, u0, dt, variant=None):
     return (t1 - t0), phases, out, variant
 
 
-def run_implicit(ag, precision, ne, steps, warmup, u0, f, dt, flags):
+def run_implicit(ag, precision, ne, steps, warmup, u0, f, dt, flags, kron=False):
     """Implicit delta-form heat step with source (p=3): u += A^-1 dt (f - K u),
     A = (x)_d (M_d + dt/2 K_d)."""
     sim = ag.Simulation(3, [3, 3, 3], [ne, ne, ne], precision=precision)
@@ -69,6 +71,7 @@ def run_implicit(ag, precision, ne, steps, warmup, u0, f, dt, flags):
     form = ag.rhs_form(dt, mass=0.0, source=True)
     slots = [1, 1, 1]
     variant = sim.autotune(form, slots, flags)
+    sim.set_tuning(ag.TUNE_RHS_KRONECKER, 1 if kron else 0)
     sim.step_delta(warmup, form, slots, flags)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -108,6 +111,16 @@ def implicit_block(ag, np, ne, steps, warmup):
                      "speedup_vs_fp64": t64 / t32, "phases_ms": ph32, "rhs_variant": v32,
                      f"rel_l2_vs_fp64_after_{total}_steps":
                          float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))}
+    # the same steps with the rhs of this separable integrand through the 1D matrices
+    kr = {}
+    for name, prec, flags in (("fp64", ag.FP64, 0), ("fp32_refined", ag.FP32, ag.STEP_REFINE)):
+        sk, tk, phk, xk, _ = run_implicit(ag, prec, ne, steps, warmup, u0, f, dt, flags, kron=True)
+        sk.close()
+        kr[name] = {"value": N * steps / tk, "ms_per_step": 1e3 * tk / steps, "phases_ms": phk,
+                    "speedup_vs_element_fp64": t64 / tk,
+                    f"rel_l2_vs_element_fp64_after_{total}_steps":
+                        float(np.linalg.norm(xk - x64) / np.linalg.norm(x64))}
+    res["kronecker_rhs"] = kr
     # adjoint: lam(T) = 2 u(T) for L = ||u(T)||^2, run back over the same number of steps
     form = ag.rhs_form(dt, mass=0.0, source=True)
     slots = [1, 1, 1]
@@ -303,6 +316,8 @@ def main():
     ap.add_argument("--implicit-elements", type=int, default=192)
     ap.add_argument("--implicit-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble block")
+    ap.add_argument("--no-kron", action="store_true",
+                    help="skip the separable-integrand (1D-matrix) rhs block")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D k-field adjoint block")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -422,6 +437,28 @@ def main():
                 "phases_ms": ph32,
             },
         }
+
+    if not args.no_kron:
+        kron = {}
+        for name, prec in (("fp64", ag.FP64), ("fp32", ag.FP32)):
+            tk, phk, xk, _ = run_form(ag, prec, ne, args.steps, args.warmup, u0, dt, args.rhs_variant,
+                                      kron=True)
+            kron[name] = {"value": N * args.steps / tk, "ms_per_step": 1e3 * tk / args.steps,
+                          "speedup_vs_element_fp64": t64 / tk, "phases_ms": phk,
+                          "rel_l2_vs_element_fp64_after_all_steps":
+                              float(np.linalg.norm(xk - x64) / np.linalg.norm(x64))}
+            # the rhs kernel reads u and writes rhs once: 2 N s bytes per step
+            nbytes = 2 * N * (8 if prec == ag.FP64 else 4)
+            kron[name]["rhs_roofline"] = {
+                "kernel": "k_rhs_kron3", "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                "achieved": nbytes / (phk["rhs"] * 1e-3) / 1e9,
+                "frac": nbytes / (phk["rhs"] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                "bytes_per_launch": nbytes}
+        kron["note"] = ("the same heat step with the rhs of this separable integrand assembled "
+                        "from the 1D Gram and stiffness matrices (ADSG_TUNE_RHS_KRONECKER); the "
+                        "p+1-point Gauss rule is exact for it, so it equals the element loop to "
+                        "rounding")
+        result.setdefault("forms", {})["kronecker_rhs"] = kron
 
     if not args.no_implicit:
         result.setdefault("forms", {})["implicit_p3"] = implicit_block(

@@ -74,6 +74,7 @@ EXPORTS = [
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
+TUNE_RHS_KRONECKER = 3
 TUNE_BATCH_TILE = 2
 
 

@@ -64,16 +64,35 @@ int upload(adsg_sim* h, void** dst, const std::vector<double>& src) {
     return ADSG_OK;
 }
 
+// band storage (ld = 3p+1, A(i,j) at j*ld + 2p + i - j) -> rows M[i][b] = A(i, i+b-p)
+std::vector<double> band_rows(const std::vector<double>& ab, int n, int p) {
+    const int ld = 3 * p + 1, w = 2 * p + 1;
+    std::vector<double> r(static_cast<size_t>(n) * w, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int b = 0; b < w; ++b) {
+            const int j = i + b - p;
+            if (j >= 0 && j < n) r[static_cast<size_t>(i) * w + b] = ab[static_cast<size_t>(j) * ld + 2 * p + i - j];
+        }
+    return r;
+}
+
 int upload_tables(adsg_sim* h, dev_dir& d) {
     int rc;
+    // form_matrix.cpp:8-42 on the same tables: the Gram and stiffness matrices, unfixed
+    const std::vector<double> M = band_rows(adsg::band_form(d.tab, 1.0, 0.0), d.tab.dofs, d.tab.p);
+    const std::vector<double> K = band_rows(adsg::band_form(d.tab, 0.0, 1.0), d.tab.dofs, d.tab.p);
     if (h->precision == ADSG_FP32) {
         if ((rc = upload<float>(h, &d.val, d.tab.val))) return rc;
         if ((rc = upload<float>(h, &d.der, d.tab.der))) return rc;
         if ((rc = upload<float>(h, &d.wj, d.tab.wj))) return rc;
+        if ((rc = upload<float>(h, &d.kM, M))) return rc;
+        if ((rc = upload<float>(h, &d.kK, K))) return rc;
     } else {
         if ((rc = upload<double>(h, &d.val, d.tab.val))) return rc;
         if ((rc = upload<double>(h, &d.der, d.tab.der))) return rc;
         if ((rc = upload<double>(h, &d.wj, d.tab.wj))) return rc;
+        if ((rc = upload<double>(h, &d.kM, M))) return rc;
+        if ((rc = upload<double>(h, &d.kK, K))) return rc;
     }
     return ADSG_OK;
 }
@@ -271,12 +290,29 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
         rhs = static_cast<T*>(h->vec[delta ? ADSG_BUFFER : ADSG_U]);
         up = static_cast<const T*>(h->vec[ADSG_U_PREV]);
     }
-    hipError_t e = form->use_source ? adsg::launch_scale<T>(rhs, static_cast<const T*>(h->source_load),
-                                                             h->N, prm.dt, h->stream)
-                                    : adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
+    // separable integrand (no k field, no reaction): the 1D-matrix form, which writes every
+    // entry, so only a source needs a start value
+    const bool kron = h->rhs_kron && h->dim == 3 && !form->use_kfield && form->reaction == 0.0;
+    hipError_t e = hipSuccess;
+    if (form->use_source)
+        e = adsg::launch_scale<T>(rhs, static_cast<const T*>(h->source_load), h->N, prm.dt, h->stream);
+    else if (!kron)
+        e = adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
     if (e != hipSuccess) return hip_fail(h, e, "rhs start");
     mark(h, 1);
-    e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->rhs_variant, h->stream);
+    if (kron) {
+        adsg::kron_dir<T> kd[3];
+        for (int d = 0; d < 3; ++d) {
+            kd[d].M = static_cast<const T*>(h->dirs[d].kM);
+            kd[d].K = static_cast<const T*>(h->dirs[d].kK);
+            kd[d].n = h->dirs[d].tab.dofs;
+        }
+        const T grad[3] = {prm.gx, prm.gy, prm.gz};
+        e = adsg::launch_rhs_kron<T>(h->dim, h->dirs[0].tab.p, up, rhs, kd, prm.mass, prm.dt, grad,
+                                     form->use_source, h->stream);
+    } else {
+        e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->rhs_variant, h->stream);
+    }
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     mark(h, 2);
     if constexpr (sizeof(T) == sizeof(float)) {
@@ -777,7 +813,7 @@ void adsg_destroy(adsg_sim* h) {
     drop_graphs(h);
     for (auto& d : h->dirs) {
         for (auto& s : d.slots) free_slot(s);
-        for (void** p : {&d.val, &d.der, &d.wj, &d.rows})
+        for (void** p : {&d.val, &d.der, &d.wj, &d.rows, &d.kM, &d.kK})
             if (*p) (void)hipFree(*p);
     }
     for (auto& v : h->vec)
@@ -1366,8 +1402,9 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
 
 int adsg_set_tuning(adsg_sim* h, int key, int value) {
     if (!h) return ADSG_EINVAL;
-    int* slot = key == ADSG_TUNE_RHS_VARIANT ? &h->rhs_variant
+    int* slot = key == ADSG_TUNE_RHS_VARIANT     ? &h->rhs_variant
                 : key == ADSG_TUNE_SWEEP_VARIANT ? &h->sweep_variant
+                : key == ADSG_TUNE_RHS_KRONECKER ? &h->rhs_kron
                                                  : nullptr;
     const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 2 : 1;
     if (!slot || value < 0 || value > max_value) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
@@ -1380,6 +1417,7 @@ int adsg_get_tuning(const adsg_sim* h, int key) {
     if (!h) return ADSG_EINVAL;
     if (key == ADSG_TUNE_RHS_VARIANT) return h->rhs_variant;
     if (key == ADSG_TUNE_SWEEP_VARIANT) return h->sweep_variant;
+    if (key == ADSG_TUNE_RHS_KRONECKER) return h->rhs_kron;
     return ADSG_EINVAL;
 }
 

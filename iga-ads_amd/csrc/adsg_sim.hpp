@@ -42,6 +42,10 @@ struct dev_dir {
     void* der = nullptr;
     void* wj = nullptr;
     void* rows = nullptr;  // FP32 mode: slot-0 matrix as n rows of 2p+1 entries
+    // the 1D Gram and stiffness matrices as rows of 2p+1 entries (step precision), for the
+    // separable-integrand rhs (launch_rhs_kron)
+    void* kM = nullptr;
+    void* kK = nullptr;
 };
 
 }  // namespace adsg
@@ -67,6 +71,7 @@ struct adsg_sim {
     bool prepared = false;
     int rhs_variant = 0;    // element-kernel tuning choice (adsg_set_tuning)
     int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
+    int rhs_kron = 0;       // 1: separable integrands through the 1D matrices (adsg_set_tuning)
     std::string err;
     hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
     double* norm_scratch = nullptr;  // adsg_norm: block partial sums and the total

@@ -264,6 +264,13 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
  * vectors are restored afterwards. */
 #define ADSG_TUNE_RHS_VARIANT 0
 #define ADSG_TUNE_SWEEP_VARIANT 1
+/* ADSG_TUNE_RHS_KRONECKER = 1 (default 0; 3D; the autotuner leaves it): forms without a k
+ * field and without a reaction term (mass u v - dt sum_d g_d (d_d u)(d_d v), plus a source)
+ * are assembled as (mass (x)_d M_d - dt sum_d g_d K_d (x) M_others) u from the 1D Gram and
+ * stiffness matrices (form_matrix.cpp:8-42). The p+1-point Gauss rule integrates these
+ * products exactly per element, so this is the element loop's sum in another order (equal
+ * to rounding) at a few dozen operations per coefficient instead of thousands per element. */
+#define ADSG_TUNE_RHS_KRONECKER 3
 int adsg_set_tuning(adsg_sim* h, int key, int value);
 int adsg_get_tuning(const adsg_sim* h, int key);
 int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags);

@@ -130,9 +130,9 @@ class RhsForm(C.Structure):
 
 
 def heat_form(dt, dim=3, grad=(1.0, 1.0, 1.0), reaction=0.0, kfield=None, source=None,
-              as_shipped=False):
+              as_shipped=False, mass=1.0):
     f = RhsForm()
-    f.mass = 1.0
+    f.mass = mass
     f.dt = dt
     for i in range(3):
         f.grad[i] = grad[i]
