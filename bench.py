@@ -113,13 +113,25 @@ def implicit_block(ag, np, ne, steps, warmup):
                          float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))}
     # the same steps with the rhs of this separable integrand through the 1D matrices
     kr = {}
+    form = ag.rhs_form(dt, mass=0.0, source=True)
+    slots = [1, 1, 1]
     for name, prec, flags in (("fp64", ag.FP64, 0), ("fp32_refined", ag.FP32, ag.STEP_REFINE)):
         sk, tk, phk, xk, _ = run_implicit(ag, prec, ne, steps, warmup, u0, f, dt, flags, kron=True)
-        sk.close()
         kr[name] = {"value": N * steps / tk, "ms_per_step": 1e3 * tk / steps, "phases_ms": phk,
                     "speedup_vs_element_fp64": t64 / tk,
                     f"rel_l2_vs_element_fp64_after_{total}_steps":
                         float(np.linalg.norm(xk - x64) / np.linalg.norm(x64))}
+        if prec == ag.FP64:  # the adjoint of ||u(T)||^2 with the same rhs form
+            sk.set(ag.U, 2.0 * xk)
+            sk.adjoint_step_delta(1, form, slots)
+            sk.set(ag.U, 2.0 * xk)
+            sk.synchronize()
+            ta = time.perf_counter()
+            sk.adjoint_step_delta(total, form, slots)
+            sk.synchronize()
+            ta = time.perf_counter() - ta
+            kr["adjoint_fp64"] = {"value": N * total / ta, "ms_per_step": 1e3 * ta / total, "steps": total}
+        sk.close()
     res["kronecker_rhs"] = kr
     # adjoint: lam(T) = 2 u(T) for L = ||u(T)||^2, run back over the same number of steps
     form = ag.rhs_form(dt, mass=0.0, source=True)

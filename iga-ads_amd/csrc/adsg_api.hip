@@ -456,6 +456,21 @@ int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, d
         prm.kfield = static_cast<const double*>(h->vec[ADSG_KFIELD]);
     }
     hipError_t e;
+    if (h->rhs_kron && h->dim == 3 && !form->use_kfield) {  // separable: the 1D-matrix form
+        mark(h, 1);
+        adsg::kron_dir<double> kd[3];
+        for (int d = 0; d < 3; ++d) {
+            kd[d].M = static_cast<const double*>(h->dirs[d].kM);
+            kd[d].K = static_cast<const double*>(h->dirs[d].kK);
+            kd[d].n = h->dirs[d].tab.dofs;
+        }
+        const double grad[3] = {prm.gx, prm.gy, prm.gz};
+        e = adsg::launch_rhs_kron<double>(h->dim, h->dirs[0].tab.p, v, out, kd, prm.mass, prm.dt, grad, false,
+                                          h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
+        mark(h, 2);
+        return ADSG_OK;
+    }
     if ((e = adsg::launch_fill<double>(out, h->N, 0.0, h->stream)) != hipSuccess) return hip_fail(h, e, "zero");
     mark(h, 1);
     e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, h->rhs_variant, h->stream);

@@ -104,3 +104,23 @@ def test_kronecker_implicit_delta_step_with_source():
         out.append(g.get(ag.U))
     assert po.rel_l2(out[1], out[0]) <= 1e-12
     g.close()
+
+
+def test_kronecker_adjoint_delta_step_equals_element_adjoint():
+    """adsg_adjoint_step_delta applies R^T = R (the linear form without source) to the
+    transposed solve's result: the 1D-matrix form gives the element kernels' result."""
+    p, ne, dt = 3, (24, 20, 18), 1e-3
+    print("case: kron adjoint delta step p=3", flush=True)
+    g, _ = _pair(p, ne)
+    for d in range(3):
+        g.set_matrix(d, 1, 1.0, 0.5 * dt)
+    g.prepare_matrices()
+    lam0 = np.random.default_rng(6).standard_normal(g.size)
+    out = []
+    for kron in (0, 1):
+        g.set_tuning(ag.TUNE_RHS_KRONECKER, kron)
+        g.set(ag.U, lam0)
+        g.adjoint_step_delta(3, ag.rhs_form(dt, mass=0.0, source=True), [1, 1, 1])
+        out.append(g.get(ag.U))
+    assert po.rel_l2(out[1], out[0]) <= 1e-12
+    g.close()
