@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libadsgpu.so")
 
 OK, EINVAL, EHIP, ENOMEM, ESTATE, ENONFINITE = 0, -1, -2, -3, -4, -5
 FP64, FP32 = 0, 1
-U, U_PREV, BUFFER, KFIELD, SOURCE, GRAD_K = 0, 1, 2, 3, 4, 5
+U, U_PREV, BUFFER, KFIELD, SOURCE, GRAD_K, GRAD_F = 0, 1, 2, 3, 4, 5, 6
 STEP_DIRICHLET = 1
 STEP_REFINE = 2
 

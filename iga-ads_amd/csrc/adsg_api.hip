@@ -25,7 +25,8 @@ namespace {
 
 // bytes per coefficient of a vector
 size_t elt_of(const adsg_sim* h, int which) {
-    if (which == ADSG_U || which == ADSG_U_PREV || which == ADSG_GRAD_K) return sizeof(double);
+    if (which == ADSG_U || which == ADSG_U_PREV || which == ADSG_GRAD_K || which == ADSG_GRAD_F)
+        return sizeof(double);
     return h->precision == ADSG_FP32 ? sizeof(float) : sizeof(double);
 }
 
@@ -479,6 +480,43 @@ int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, d
     return ADSG_OK;
 }
 
+// out = M v with M the Gram matrix of the whole basis (no fixed rows): the element loop of
+// the form {mass 1, dt 0}, or its 1D-matrix form when that path is selected
+int gram_apply(adsg_sim* h, const double* v, double* out) {
+    hipError_t e;
+    if (h->rhs_kron && h->dim == 3) {
+        adsg::kron_dir<double> kd[3];
+        for (int d = 0; d < 3; ++d) {
+            kd[d].M = static_cast<const double*>(h->dirs[d].kM);
+            kd[d].K = static_cast<const double*>(h->dirs[d].kK);
+            kd[d].n = h->dirs[d].tab.dofs;
+        }
+        const double none[3] = {0.0, 0.0, 0.0};
+        e = adsg::launch_rhs_kron<double>(h->dim, h->dirs[0].tab.p, v, out, kd, 1.0, 0.0, none, false, h->stream);
+    } else {
+        adsg::dev_tables<double> t[3];
+        fp64_tables(h, t);
+        adsg::rhs_params<double> prm;
+        prm.mass = 1.0;
+        prm.dt = 0.0;
+        e = adsg::launch_fill<double>(out, h->N, 0.0, h->stream);
+        if (e == hipSuccess)
+            e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, h->rhs_variant, h->stream);
+    }
+    return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "Gram product");
+}
+
+// dL/df += dt M w for forms with a source, when the caller has set ADSG_GRAD_F; scratch is
+// overwritten
+int accumulate_source_gradient(adsg_sim* h, const adsg_rhs_form* form, const double* w, double* scratch) {
+    if (!form->use_source || !h->vec[ADSG_GRAD_F]) return ADSG_OK;
+    int rc = gram_apply(h, w, scratch);
+    if (rc) return rc;
+    hipError_t e = adsg::launch_axpy<double>(static_cast<double*>(h->vec[ADSG_GRAD_F]), scratch, h->N, form->dt,
+                                             h->stream);
+    return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "source gradient");
+}
+
 // One step of the adjoint of one_step (FP64): see adsg_adjoint_step in adsgpu.h.
 int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
     int nx, ny, nz;
@@ -506,6 +544,9 @@ int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
                                        static_cast<double*>(h->vec[ADSG_GRAD_K]), t, prm, h->stream);
         if (e != hipSuccess) return hip_fail(h, e, "k-field gradient");
     }
+    // the source enters as dt M f: dL/df += dt M w (u_prev, read above, is free now)
+    int rs = accumulate_source_gradient(h, form, w, static_cast<double*>(h->vec[ADSG_U_PREV]));
+    if (rs) return rs;
     return apply_linear_form(h, form, w, lam);
 }
 
@@ -530,6 +571,8 @@ int one_adjoint_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* sl
     if (rc) return rc;
     if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, v, nx, ny, nz, h->stream)) != hipSuccess)
         return hip_fail(h, e, "zero boundary");
+    // r = R u + dt M f: dL/df += dt M v
+    if ((rc = accumulate_source_gradient(h, form, v, w))) return rc;
     if ((rc = apply_linear_form(h, form, v, w))) return rc;
     e = adsg::launch_add_increment(lam, w, lam, h->dim, nx, ny, nz, false, h->stream);
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "adjoint update");
@@ -811,7 +854,7 @@ int project_impl(adsg_sim* h, int which, const double* fq_dev) {
 }
 
 bool projectable(const adsg_sim* h, int which) {
-    return which >= ADSG_U && which <= ADSG_GRAD_K && elt_of(h, which) == sizeof(double);
+    return which >= ADSG_U && which <= ADSG_GRAD_F && elt_of(h, which) == sizeof(double);
 }
 
 }  // namespace
@@ -1002,7 +1045,7 @@ int adsg_prepare_matrices(adsg_sim* h) {
 }
 
 int adsg_set_vector(adsg_sim* h, int which, const double* host) {
-    if (!h || !host || which < ADSG_U || which > ADSG_GRAD_K) return fail(h, ADSG_EINVAL, "set_vector: bad argument");
+    if (!h || !host || which < ADSG_U || which > ADSG_GRAD_F) return fail(h, ADSG_EINVAL, "set_vector: bad argument");
     if (which == ADSG_SOURCE) h->source_load_ok = false;
     const size_t es = elt_of(h, which);
     if (!h->vec[which]) {
@@ -1028,7 +1071,7 @@ int adsg_set_vector(adsg_sim* h, int which, const double* host) {
 }
 
 int adsg_get_vector(adsg_sim* h, int which, double* host) {
-    if (!h || !host || which < ADSG_U || which > ADSG_GRAD_K || !h->vec[which])
+    if (!h || !host || which < ADSG_U || which > ADSG_GRAD_F || !h->vec[which])
         return fail(h, ADSG_EINVAL, "get_vector: bad argument");
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     if (elt_of(h, which) == sizeof(float)) {
@@ -1042,13 +1085,13 @@ int adsg_get_vector(adsg_sim* h, int which, double* host) {
 }
 
 void* adsg_device_ptr(adsg_sim* h, int which) {
-    if (!h || which < ADSG_U || which > ADSG_GRAD_K) return nullptr;
+    if (!h || which < ADSG_U || which > ADSG_GRAD_F) return nullptr;
     if (which == ADSG_SOURCE) h->source_load_ok = false;  // the caller may write through it
     return h->vec[which];
 }
 
 int adsg_set_vector_device(adsg_sim* h, int which, const double* dev) {
-    if (!h || !dev || which < ADSG_U || which > ADSG_GRAD_K)
+    if (!h || !dev || which < ADSG_U || which > ADSG_GRAD_F)
         return fail(h, ADSG_EINVAL, "set_vector_device: bad argument");
     if (which == ADSG_SOURCE) h->source_load_ok = false;
     if (elt_of(h, which) != sizeof(double))
@@ -1069,7 +1112,7 @@ int adsg_set_vector_device(adsg_sim* h, int which, const double* dev) {
 }
 
 int adsg_get_vector_device(adsg_sim* h, int which, double* dev) {
-    if (!h || !dev || which < ADSG_U || which > ADSG_GRAD_K || !h->vec[which])
+    if (!h || !dev || which < ADSG_U || which > ADSG_GRAD_F || !h->vec[which])
         return fail(h, ADSG_EINVAL, "get_vector_device: bad argument");
     if (elt_of(h, which) != sizeof(double))
         return fail(h, ADSG_EINVAL, "get_vector_device: FP32 vectors are read to the host");

@@ -23,6 +23,13 @@ __global__ void k_scale(T* __restrict__ out, const T* __restrict__ in, long n, T
 }
 
 template <typename T>
+__global__ void k_axpy(T* __restrict__ out, const T* __restrict__ x, long n, T a) {
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x)
+        out[i] += a * x[i];
+}
+
+template <typename T>
 __global__ void k_zero_boundary(T* __restrict__ u, int dim, int nx, int ny, int nz) {
     const long total = static_cast<long>(nx) * ny * nz;
     for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
@@ -138,6 +145,12 @@ hipError_t launch_scale(T* out, const T* in, long n, T a, hipStream_t s) {
 }
 
 template <typename T>
+hipError_t launch_axpy(T* out, const T* x, long n, T a, hipStream_t s) {
+    hipLaunchKernelGGL((k_axpy<T>), dim3(grid_for(n)), dim3(256), 0, s, out, x, n, a);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_zero_boundary(int dim, T* u, int nx, int ny, int nz, hipStream_t s) {
     const long n = static_cast<long>(nx) * ny * nz;
     hipLaunchKernelGGL((k_zero_boundary<T>), dim3(grid_for(n)), dim3(256), 0, s, u, dim, nx, ny, nz);
@@ -184,6 +197,7 @@ template hipError_t launch_fill<double>(double*, long, double, hipStream_t);
 template hipError_t launch_fill<float>(float*, long, float, hipStream_t);
 template hipError_t launch_scale<double>(double*, const double*, long, double, hipStream_t);
 template hipError_t launch_scale<float>(float*, const float*, long, float, hipStream_t);
+template hipError_t launch_axpy<double>(double*, const double*, long, double, hipStream_t);
 template hipError_t launch_zero_boundary<double>(int, double*, int, int, int, hipStream_t);
 template hipError_t launch_zero_boundary<float>(int, float*, int, int, int, hipStream_t);
 

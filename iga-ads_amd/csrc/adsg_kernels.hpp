@@ -184,6 +184,9 @@ hipError_t launch_fill(T* p, long n, T v, hipStream_t s);
 // out = a * in
 template <typename T>
 hipError_t launch_scale(T* out, const T* in, long n, T a, hipStream_t s);
+// out += a * x
+template <typename T>
+hipError_t launch_axpy(T* out, const T* x, long n, T a, hipStream_t s);
 
 template <typename T>
 hipError_t launch_zero_boundary(int dim, T* u, int nx, int ny, int nz, hipStream_t s);

@@ -64,7 +64,7 @@ struct adsg_sim {
     // with them) and a step is computed as an increment: compute_rhs assembles
     // rhs - M u_prev in FP32 into buffer, ads_solve turns it into M^-1 (rhs - M u_prev) with
     // FP32 sweeps on FP64-computed factors and adds it to u_prev in FP64.
-    void* vec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* vec[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     void* shadow[2] = {nullptr, nullptr};
     bool pending = false;            // FP32: buffer holds an increment based on u_prev
     bool pending_dirichlet = false;  // FP32: boundary rows of the increment were fixed

@@ -44,6 +44,8 @@ extern "C" {
 #define ADSG_KFIELD 3 /* coefficients of a diffusion field k(x) on the same basis */
 #define ADSG_SOURCE 4 /* coefficients of a source f(x) on the same basis */
 #define ADSG_GRAD_K 5 /* accumulated dL/dk of adsg_adjoint_step (FP64 in both precisions) */
+#define ADSG_GRAD_F 6 /* accumulated dL/df of the adjoint steps for forms with a source (FP64;
+                         accumulated only when the caller has set it, e.g. to zeros) */
 
 typedef struct adsg_sim adsg_sim;
 
@@ -166,7 +168,7 @@ int adsg_error(adsg_sim* h, int which, int kind, const double* sq, double* out);
 /* projection(v, f) (simulation_2d.hpp:43-46 / simulation_3d.hpp:45-48 -> projector,
  * projection.hpp:11-153): adds the load vector of the L2 projection of f,
  *   v_i += sum_e sum_q f(x_q) B_i(x_q) w_q J_e,
- * into vector `which`, which must hold FP64 coefficients (ADSG_U, ADSG_U_PREV, ADSG_GRAD_K,
+ * into vector `which`, which must hold FP64 coefficients (ADSG_U, ADSG_U_PREV, ADSG_GRAD_K, ADSG_GRAD_F,
  * and in FP64 handles every vector; it is allocated and zeroed if absent). f is given at
  * the Gauss points: fq[ix + NX*(iy + NY*iz)] with ix = e_x*(p+1) + q_x, NX = elements_x*(p+1)
  * (likewise y and z; adsg_quad_points returns the abscissae). As in the reference's
@@ -217,8 +219,10 @@ int adsg_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const in
  * ADSG_STEP_DIRICHLET) and r(u) = R(k) u + c:
  *   w = Z M^-T lam(n+1);  lam(n) = R(k) w  (R symmetric);
  *   if use_kfield: grad_k += d/dk [w^T R(k) u(n)] = -dt sum_e sum_q w_q J_e B_b sum_d grad_d
- *   (d_d u)(d_d w)  (added into ADSG_GRAD_K, which the caller sets, e.g. to zeros, first).
- * buffer is overwritten. No counterpart in the reference (it has no adjoint code). */
+ *   (d_d u)(d_d w)  (added into ADSG_GRAD_K, which the caller sets, e.g. to zeros, first);
+ *   if use_source and ADSG_GRAD_F is set: grad_f += dt M w (c = dt M f, M the Gram matrix).
+ * buffer is overwritten, and u_prev too when grad_f is accumulated. No counterpart in the
+ * reference (it has no adjoint code). */
 int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags);
 
 /* Checkpoint-and-recompute for the explicit-step adjoint (FP64 handles). The forward call
@@ -226,7 +230,8 @@ int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags);
  * device memory held by the handle. The adjoint call takes u = dL/du(T), runs
  * adsg_adjoint_step backwards over the same steps -- each segment of `every` steps is
  * recomputed from its checkpoint -- and returns u = dL/du(0), adding dL/dk into
- * ADSG_GRAD_K when the form uses the k field. u_prev and buffer are overwritten. No
+ * ADSG_GRAD_K when the form uses the k field (and dL/df into ADSG_GRAD_F when the form uses
+ * a source and that vector is set). u_prev and buffer are overwritten. No
  * counterpart in the reference (it keeps no checkpoints and has no adjoint). */
 int adsg_forward_checkpointed(adsg_sim* h, int nsteps, int every, const adsg_rhs_form* form, int flags);
 int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags);
@@ -236,8 +241,9 @@ int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags)
  * u' = Z (u + A^-1 Z r(u)) with r(u) = R u + c linear (reaction = 0) and Z the boundary
  * zeroing of ADSG_STEP_DIRICHLET, u := Z u + R Z A^-T Z u. R = mass*M - dt*K is symmetric,
  * so R applies as the form's integrand without its source. Starting from dL/du(T) this
- * yields dL/du(0) after the same number of steps. No counterpart in the reference (it has
- * no adjoint code); FP64 handles. */
+ * yields dL/du(0) after the same number of steps; for a form with a source and ADSG_GRAD_F
+ * set, dL/df += dt M Z A^-T Z u per step (c = dt M f). No counterpart in the reference (it
+ * has no adjoint code); FP64 handles. */
 int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots,
                             int flags);
 

@@ -187,3 +187,109 @@ def test_native_checkpointed_adjoint_equals_stepwise_adjoint():
     assert po.rel_l2(g.get(ag.U), du0_ref) <= 1e-13
     with pytest.raises(ag.AdsgError):  # a different form than the forward run's
         g.adjoint_checkpointed(ag.rhs_form(2 * dt, kfield=True), flags)
+
+
+def test_source_gradient_explicit_steps():
+    """dL/df for a source field f (the rhs gains dt M f): accumulated into GRAD_F by the
+    checkpointed adjoint, against the oracle's discrete adjoint (<= 1e-10) and central
+    differences (L is quadratic in f: exact up to rounding for any h)."""
+    p, ne, dt, steps, every = 2, (40, 36), 1e-5, 20, 6
+    _log(f"case: source gradient, explicit, {steps} steps every {every}")
+    g, o = _pair(p, ne, True)
+    flags = ag.STEP_DIRICHLET
+    k = _kfield(p, ne, seed=61)
+    rng = np.random.default_rng(62)
+    f = wl.sine_source([p, p], list(ne), seed=63) * 50.0
+    u0 = wl.gaussians([p, p], list(ne), seed=64, count=2, sigma=(0.1, 0.15), noise=0.0)
+    o.zero_boundary(u0)
+    ut = rng.standard_normal(g.size) * 0.01
+    form = ag.rhs_form(dt, kfield=True, source=True)
+
+    def forward(ff):
+        g.set(ag.KFIELD, k)
+        g.set(ag.SOURCE, ff)
+        g.set(ag.U, u0)
+        g.step(steps, form, flags)
+        return g.get(ag.U)
+
+    g.set(ag.KFIELD, k)
+    g.set(ag.SOURCE, f)
+    g.set(ag.U, u0)
+    g.forward_checkpointed(steps, every, form, flags)
+    uT = g.get(ag.U)
+    g.set(ag.GRAD_K, np.zeros(g.size))
+    g.set(ag.GRAD_F, np.zeros(g.size))
+    g.set(ag.U, 2.0 * (uT - ut))
+    g.adjoint_checkpointed(form, flags)
+    df = g.get(ag.GRAD_F)
+    # oracle: forward with the source, adjoint with dL/df += dt M w
+    fo = po.heat_form(dt, 2, kfield=k, source=f)
+    traj = [u0]
+    for _ in range(steps):
+        r = o.compute_rhs(traj[-1], fo)
+        o.zero_boundary(r)
+        traj.append(o.solve(r))
+    assert po.rel_l2(uT, traj[-1]) <= 1e-12
+    lam = 2.0 * (traj[-1] - ut)
+    df_o = np.zeros(g.size)
+    lin = po.heat_form(dt, 2, kfield=k)
+    mass = po.heat_form(0.0, 2)
+    for n in range(steps - 1, -1, -1):
+        w = lam.copy()
+        o.solve(w, transposed=True)
+        o.zero_boundary(w)
+        df_o += dt * o.compute_rhs(w, mass)
+        lam = o.compute_rhs(w, lin)
+    _log(f"  df vs oracle {po.rel_l2(df, df_o):.3e}")
+    assert po.rel_l2(df, df_o) <= 1e-10
+    for j in range(3):
+        d = rng.standard_normal(g.size)
+        h = 1e-3
+        lp = forward(f + h * d)
+        lm = forward(f - h * d)
+        fd = (float((lp - ut) @ (lp - ut)) - float((lm - ut) @ (lm - ut))) / (2 * h)
+        ad = float(df @ d)
+        _log(f"  direction {j}: adjoint {ad:.9e} fd {fd:.9e}")
+        assert abs(fd - ad) <= 1e-6 * abs(ad)
+
+
+def test_source_gradient_implicit_delta_steps():
+    """The c5 step u += A^-1 dt (f - K u): dL/df of L = ||u(T)||^2 from adsg_adjoint_step_delta
+    with GRAD_F set, against central differences; also through the 1D-matrix path."""
+    p, ne, dt, steps = 2, (20, 18, 16), 1e-3, 5
+    _log("case: source gradient, implicit delta steps 3D p=2")
+    g = ag.Simulation(3, [p] * 3, list(ne))
+    for d in range(3):
+        g.set_matrix(d, 1, 1.0, 0.5 * dt)
+    g.prepare_matrices()
+    rng = np.random.default_rng(71)
+    f = rng.standard_normal(g.size)
+    u0 = rng.standard_normal(g.size) * 0.1
+    form = ag.rhs_form(dt, mass=0.0, source=True)
+    slots = [1, 1, 1]
+
+    def forward(ff):
+        g.set(ag.SOURCE, ff)
+        g.set(ag.U, u0)
+        g.step_delta(steps, form, slots)
+        return g.get(ag.U)
+
+    grads = []
+    for kron in (0, 1):
+        g.set_tuning(ag.TUNE_RHS_KRONECKER, kron)
+        uT = forward(f)
+        g.set(ag.GRAD_F, np.zeros(g.size))
+        g.set(ag.U, 2.0 * uT)
+        g.adjoint_step_delta(steps, form, slots)
+        grads.append(g.get(ag.GRAD_F))
+    assert po.rel_l2(grads[1], grads[0]) <= 1e-12
+    g.set_tuning(ag.TUNE_RHS_KRONECKER, 0)
+    for j in range(3):
+        d = rng.standard_normal(g.size)
+        h = 1e-2
+        lp, lm = forward(f + h * d), forward(f - h * d)
+        fd = (float(lp @ lp) - float(lm @ lm)) / (2 * h)
+        ad = float(grads[0] @ d)
+        _log(f"  direction {j}: adjoint {ad:.9e} fd {fd:.9e}")
+        assert abs(fd - ad) <= 1e-8 * abs(ad)
+    g.close()
