@@ -82,20 +82,53 @@ int upload_tables(adsg_sim* h, dev_dir& d) {
     // form_matrix.cpp:8-42 on the same tables: the Gram and stiffness matrices, unfixed
     const std::vector<double> M = band_rows(adsg::band_form(d.tab, 1.0, 0.0), d.tab.dofs, d.tab.p);
     const std::vector<double> K = band_rows(adsg::band_form(d.tab, 0.0, 1.0), d.tab.dofs, d.tab.p);
+    // the element tables zero-padded to whole kTablePad-element tiles, and their weighted copies
+    const int per = d.tab.nq * (d.tab.p + 1);
+    const int padded = (d.tab.elements + adsg::kTablePad - 1) / adsg::kTablePad * adsg::kTablePad;
+    std::vector<double> val(static_cast<size_t>(padded) * per, 0.0), der(val.size(), 0.0);
+    std::vector<double> bval(val.size(), 0.0), bder(val.size(), 0.0);
+    for (int e = 0; e < d.tab.elements; ++e)
+        for (int q = 0; q < d.tab.nq; ++q)
+            for (int i = 0; i <= d.tab.p; ++i) {
+                const size_t k = (static_cast<size_t>(e) * d.tab.nq + q) * (d.tab.p + 1) + i;
+                const double w = d.tab.wj[static_cast<size_t>(e) * d.tab.nq + q];
+                val[k] = d.tab.val[k];
+                der[k] = d.tab.der[k];
+                bval[k] = d.tab.val[k] * w;
+                bder[k] = d.tab.der[k] * w;
+            }
     if (h->precision == ADSG_FP32) {
-        if ((rc = upload<float>(h, &d.val, d.tab.val))) return rc;
-        if ((rc = upload<float>(h, &d.der, d.tab.der))) return rc;
+        if ((rc = upload<float>(h, &d.val, val))) return rc;
+        if ((rc = upload<float>(h, &d.der, der))) return rc;
+        if ((rc = upload<float>(h, &d.bval, bval))) return rc;
+        if ((rc = upload<float>(h, &d.bder, bder))) return rc;
         if ((rc = upload<float>(h, &d.wj, d.tab.wj))) return rc;
         if ((rc = upload<float>(h, &d.kM, M))) return rc;
         if ((rc = upload<float>(h, &d.kK, K))) return rc;
     } else {
-        if ((rc = upload<double>(h, &d.val, d.tab.val))) return rc;
-        if ((rc = upload<double>(h, &d.der, d.tab.der))) return rc;
+        if ((rc = upload<double>(h, &d.val, val))) return rc;
+        if ((rc = upload<double>(h, &d.der, der))) return rc;
+        if ((rc = upload<double>(h, &d.bval, bval))) return rc;
+        if ((rc = upload<double>(h, &d.bder, bder))) return rc;
         if ((rc = upload<double>(h, &d.wj, d.tab.wj))) return rc;
         if ((rc = upload<double>(h, &d.kM, M))) return rc;
         if ((rc = upload<double>(h, &d.kK, K))) return rc;
     }
     return ADSG_OK;
+}
+
+// the device tables of every direction, in the handle's precision TT
+template <typename TT>
+void dir_tables(const adsg_sim* h, adsg::dev_tables<TT>* t) {
+    for (int d = 0; d < h->dim; ++d) {
+        t[d].val = static_cast<const TT*>(h->dirs[d].val);
+        t[d].der = static_cast<const TT*>(h->dirs[d].der);
+        t[d].wj = static_cast<const TT*>(h->dirs[d].wj);
+        t[d].bval = static_cast<const TT*>(h->dirs[d].bval);
+        t[d].bder = static_cast<const TT*>(h->dirs[d].bder);
+        t[d].elements = h->dirs[d].tab.elements;
+        t[d].dofs = h->dirs[d].tab.dofs;
+    }
 }
 
 void free_slot(dev_slot& s) {
@@ -227,13 +260,7 @@ int source_load_impl(adsg_sim* h) {
     if (!h->source_load && hipMalloc(&h->source_load, h->N * sizeof(T)) != hipSuccess)
         return fail(h, ADSG_ENOMEM, "source: no memory for its load vector");
     adsg::dev_tables<T> t[3];
-    for (int d = 0; d < h->dim; ++d) {
-        t[d].val = static_cast<const T*>(h->dirs[d].val);
-        t[d].der = static_cast<const T*>(h->dirs[d].der);
-        t[d].wj = static_cast<const T*>(h->dirs[d].wj);
-        t[d].elements = h->dirs[d].tab.elements;
-        t[d].dofs = h->dirs[d].tab.dofs;
-    }
+    dir_tables(h, t);
     adsg::rhs_params<T> prm;
     prm.mass = T(0);
     prm.dt = T(1);
@@ -258,13 +285,7 @@ int ensure_source_load(adsg_sim* h, const adsg_rhs_form* form) {
 template <typename T>
 int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
     adsg::dev_tables<T> t[3];
-    for (int d = 0; d < h->dim; ++d) {
-        t[d].val = static_cast<const T*>(h->dirs[d].val);
-        t[d].der = static_cast<const T*>(h->dirs[d].der);
-        t[d].wj = static_cast<const T*>(h->dirs[d].wj);
-        t[d].elements = h->dirs[d].tab.elements;
-        t[d].dofs = h->dirs[d].tab.dofs;
-    }
+    dir_tables(h, t);
     adsg::rhs_params<T> prm;
     prm.mass = static_cast<T>(form->mass);
     prm.dt = static_cast<T>(form->dt);
@@ -431,20 +452,10 @@ int one_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "apply increment");
 }
 
-void fp64_tables(const adsg_sim* h, adsg::dev_tables<double>* t) {
-    for (int d = 0; d < h->dim; ++d) {
-        t[d].val = static_cast<const double*>(h->dirs[d].val);
-        t[d].der = static_cast<const double*>(h->dirs[d].der);
-        t[d].wj = static_cast<const double*>(h->dirs[d].wj);
-        t[d].elements = h->dirs[d].tab.elements;
-        t[d].dofs = h->dirs[d].tab.dofs;
-    }
-}
-
 // R^T v = R v for the linear part of form (no source, no reaction), into out (zeroed first)
 int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, double* out) {
     adsg::dev_tables<double> t[3];
-    fp64_tables(h, t);
+    dir_tables(h, t);
     adsg::rhs_params<double> prm;
     prm.mass = form->mass;
     prm.dt = form->dt;
@@ -495,7 +506,7 @@ int gram_apply(adsg_sim* h, const double* v, double* out) {
         e = adsg::launch_rhs_kron<double>(h->dim, h->dirs[0].tab.p, v, out, kd, 1.0, 0.0, none, false, h->stream);
     } else {
         adsg::dev_tables<double> t[3];
-        fp64_tables(h, t);
+        dir_tables(h, t);
         adsg::rhs_params<double> prm;
         prm.mass = 1.0;
         prm.dt = 0.0;
@@ -534,7 +545,7 @@ int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
     mark(h, 3);
     if (form->use_kfield) {
         adsg::dev_tables<double> t[3];
-        fp64_tables(h, t);
+        dir_tables(h, t);
         adsg::rhs_params<double> prm;
         prm.dt = form->dt;
         prm.gx = form->grad[0];
@@ -809,17 +820,6 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
 
 namespace {
 
-template <typename TT>
-void step_tables(const adsg_sim* h, adsg::dev_tables<TT>* t) {
-    for (int d = 0; d < h->dim; ++d) {
-        t[d].val = static_cast<const TT*>(h->dirs[d].val);
-        t[d].der = static_cast<const TT*>(h->dirs[d].der);
-        t[d].wj = static_cast<const TT*>(h->dirs[d].wj);
-        t[d].elements = h->dirs[d].tab.elements;
-        t[d].dofs = h->dirs[d].tab.dofs;
-    }
-}
-
 long quad_point_count(const adsg_sim* h) {
     long q = 1;
     for (int d = 0; d < h->dim; ++d) q *= static_cast<long>(h->dirs[d].tab.elements) * h->dirs[d].tab.nq;
@@ -838,11 +838,11 @@ int project_impl(adsg_sim* h, int which, const double* fq_dev) {
     hipError_t e;
     if (h->precision == ADSG_FP32) {
         adsg::dev_tables<float> t[3];
-        step_tables(h, t);
+        dir_tables(h, t);
         e = adsg::launch_project<float>(h->dim, h->dirs[0].tab.p, fq_dev, v, t, h->stream);
     } else {
         adsg::dev_tables<double> t[3];
-        step_tables(h, t);
+        dir_tables(h, t);
         e = adsg::launch_project<double>(h->dim, h->dirs[0].tab.p, fq_dev, v, t, h->stream);
     }
     if (e != hipSuccess) return hip_fail(h, e, "project");
@@ -871,7 +871,7 @@ void adsg_destroy(adsg_sim* h) {
     drop_graphs(h);
     for (auto& d : h->dirs) {
         for (auto& s : d.slots) free_slot(s);
-        for (void** p : {&d.val, &d.der, &d.wj, &d.rows, &d.kM, &d.kK})
+        for (void** p : {&d.val, &d.der, &d.wj, &d.bval, &d.bder, &d.rows, &d.kM, &d.kK})
             if (*p) (void)hipFree(*p);
     }
     for (auto& v : h->vec)
@@ -1271,18 +1271,12 @@ int norm_impl(adsg_sim* h, int which, const double* sq, int kind, double* out) {
     hipError_t e;
     if (h->precision == ADSG_FP32) {
         adsg::dev_tables<float> t[3];
-        for (int d = 0; d < h->dim; ++d) {
-            t[d].val = static_cast<const float*>(h->dirs[d].val);
-            t[d].der = static_cast<const float*>(h->dirs[d].der);
-            t[d].wj = static_cast<const float*>(h->dirs[d].wj);
-            t[d].elements = h->dirs[d].tab.elements;
-            t[d].dofs = h->dirs[d].tab.dofs;
-        }
+        dir_tables(h, t);
         e = adsg::launch_norm<float>(h->dim, h->dirs[0].tab.p, u, sq, t, kind, h->norm_scratch, blocks,
                                      h->norm_scratch + kBlocks, h->stream);
     } else {
         adsg::dev_tables<double> t[3];
-        fp64_tables(h, t);
+        dir_tables(h, t);
         e = adsg::launch_norm<double>(h->dim, h->dirs[0].tab.p, u, sq, t, kind, h->norm_scratch, blocks,
                                       h->norm_scratch + kBlocks, h->stream);
     }
@@ -1464,7 +1458,7 @@ int adsg_set_tuning(adsg_sim* h, int key, int value) {
                 : key == ADSG_TUNE_SWEEP_VARIANT ? &h->sweep_variant
                 : key == ADSG_TUNE_RHS_KRONECKER ? &h->rhs_kron
                                                  : nullptr;
-    const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 2 : 1;
+    const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 3 : 1;
     if (!slot || value < 0 || value > max_value) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
     if (*slot != value) drop_graphs(h);
     *slot = value;
@@ -1503,7 +1497,7 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
     int best = h->rhs_variant, best_sweep = h->sweep_variant;
     double best_ms = 1e30, best_sweep_ms = 1e30;
     int rc = ADSG_OK;
-    for (int v = 0; v <= 2 && rc == ADSG_OK; ++v) {
+    for (int v = 0; v <= 3 && rc == ADSG_OK; ++v) {
         h->rhs_variant = v;
         h->sweep_variant = v <= 1 ? v : best_sweep;  // both sweeps timed with variants 0, 1
         double ms[7];

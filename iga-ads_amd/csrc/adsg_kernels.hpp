@@ -66,12 +66,17 @@ hipError_t launch_line_sweep_own(T* u, const line_geometry& g, const own_factors
                                  hipStream_t s);
 
 // Tables of one direction on the device: val/der [elements][p+1][p+1] (element, Gauss
-// point, local function) and wj [elements][p+1].
+// point, local function) and wj [elements][p+1]; bval/bder are val/der times the weight
+// wj of their Gauss point. val, der, bval and bder are zero-padded to a whole number of
+// kTablePad elements, so that a tile of elements can be copied without bounds checks.
+constexpr int kTablePad = 16;
 template <typename T>
 struct dev_tables {
     const T* val = nullptr;
     const T* der = nullptr;
     const T* wj = nullptr;
+    const T* bval = nullptr;
+    const T* bder = nullptr;
     int elements = 0;
     int dofs = 0;
 };

@@ -35,9 +35,17 @@ __device__ __forceinline__ int xcd_contiguous(int b, int n) {
     return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
+// A read of read-only, wave-uniform data through the constant address space: always a
+// scalar load, also after instructions the compiler must treat as memory writes.
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+    using CT = const __attribute__((address_space(4))) T*;
+    return *(CT)(p);
+}
+
 template <typename T, int P, int VAR>
 constexpr int rhs3_waves() {  // minimum waves per SIMD the register allocation must allow
-    return VAR == 0 ? 1 : VAR == 1 ? 2 : P == 3 ? 2 : sizeof(T) == sizeof(float) ? 4 : 3;
+    return VAR == 0 ? 1 : VAR == 1 || VAR == 3 ? 2 : P == 3 ? 2 : sizeof(T) == sizeof(float) ? 4 : 3;
 }
 
 // first element of tile t of the given parity colour (ntx2, nty2 tiles of that colour per
@@ -55,35 +63,52 @@ __device__ __forceinline__ tile3 rhs3_tile(int t, int colour, int ntx2, int nty2
     return {tx * TX, ty * TY, tz * TZ};
 }
 
-// the LDS of one tile: its coefficient block (and k / f blocks), the x and y tables of its
-// elements, and the waves' private DOF planes (wave lz owns planes lz*M .. lz*M+P, DOF
-// z = lz + c; FP64 in both precisions: the FP64 LDS add is the fast one on this part)
+// The LDS data of one tile: its coefficient block (and k / f blocks) and the x and y tables
+// of its elements: xt[FV], xt[FD] as tabulated, xt[BV], xt[BD] with the quadrature weight
+// (variants 0-2 also fold -dt*grad_x into xt[BD], -dt*grad_y into yt[BD]).
+enum { FV = 0, FD = 1, BV = 2, BD = 3 };
 template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC>
-struct rhs3_lds {
+struct rhs3_buf {
     static constexpr int M = P + 1, BX = TX + P, BY = TY + P, BZ = TZ + P;
     static constexpr int NB = BX * BY * BZ, PL = BX * BY;
     T blk[NB];
     T kb[KF ? NB : 1];
     T sb[SRC ? NB : 1];
-    T fvx[TX * M * M], fdx[TX * M * M], bvx[TX * M * M], bdx[TX * M * M];
-    T fvy[TY * M * M], fdy[TY * M * M], bvy[TY * M * M], bdy[TY * M * M];
-    double acc[TZ * M * PL];
+    T xt[4][TX * M * M];
+    T yt[4][TY * M * M];
+};
+// the waves' private DOF planes (wave lz owns planes lz*M .. lz*M+P, DOF z = lz + c; FP64 in
+// both precisions: the FP64 LDS add is the fast one on this part)
+template <int P, int TX, int TY, int TZ>
+constexpr int rhs3_acc_size() {
+    return TZ * (P + 1) * (TX + P) * (TY + P);
+}
+// The LDS of a workgroup with NBUF tile buffers and their planes, in one object: the
+// compiler then addresses every part from one base (separate __shared__ arrays cost
+// registers in the element loop).
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int NBUF>
+struct rhs3_lds {
+    rhs3_buf<T, P, TX, TY, TZ, KF, SRC> b[NBUF];
+    double acc[NBUF][rhs3_acc_size<P, TX, TY, TZ>()];
 };
 
 // One element (lane lx, ly of wave lz) of the tile whose first element is (ex0, ey0, ez0):
 // sum-factorised integrand over its Gauss points, contribution added to the wave's planes.
 template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX, int VAR>
-__device__ __forceinline__ void rhs3_element(rhs3_lds<T, P, TX, TY, TZ, KF, SRC>& L,
-                                             const dev_tables<T>& X, const dev_tables<T>& Y,
-                                             const dev_tables<T>& Z, const rhs_params<T>& prm,
-                                             int lx, int ly, int lz, int ex0, int ey0, int ez0) {
-    using LDS = rhs3_lds<T, P, TX, TY, TZ, KF, SRC>;
+__device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF, SRC>& L,
+                                             double* acc, const dev_tables<T>& X,
+                                             const dev_tables<T>& Y, const dev_tables<T>& Z,
+                                             const rhs_params<T>& prm, int lx, int ly, int lz,
+                                             int ex0, int ey0, int ez0) {
+    using LDS = rhs3_buf<T, P, TX, TY, TZ, KF, SRC>;
     constexpr int M = P + 1, BX = LDS::BX, BY = LDS::BY, PL = LDS::PL;
-    constexpr bool CREG = VAR == 0;   // coefficient block held in registers
-    constexpr bool FLUSH = VAR == 2;  // element contribution added to the planes per x point
+    constexpr bool CREG = VAR == 0;  // coefficient block held in registers
+    // element contribution added to the planes per x point (variant 3 at P = 3: registers)
+    constexpr bool FLUSH = VAR == 2 || (VAR == 3 && P == 3);
+    constexpr bool TW = VAR == 3;  // tables weighted only: -dt*grad applied as read
     const int ez = ez0 + lz;
     // the wave's planes: in one add instruction the 64 lanes hit 64 distinct entries
-    double* mine = L.acc + lz * M * PL;
+    double* mine = acc + lz * M * PL;
 
     T R[FLUSH ? 1 : M][FLUSH ? 1 : M][FLUSH ? 1 : M];  // the element's contribution, (a, b, c)
 #pragma unroll
@@ -101,14 +126,15 @@ __device__ __forceinline__ void rhs3_element(rhs3_lds<T, P, TX, TY, TZ, KF, SRC>
         const T* zd = Z.der + static_cast<long>(ez) * M * M;
 #pragma unroll
         for (int q = 0; q < M; ++q) {
-            wz[q] = Z.wj[static_cast<long>(ez) * M + q];
+            wz[q] = uniform_load(Z.wj + static_cast<long>(ez) * M + q);
 #pragma unroll
             for (int c = 0; c < M; ++c) {
-                Bz[q][c] = zv[q * M + c];
-                Dz[q][c] = zd[q * M + c];
+                Bz[q][c] = uniform_load(zv + q * M + c);
+                Dz[q][c] = uniform_load(zd + q * M + c);
             }
         }
         const T cz = -prm.dt * prm.gz;
+        const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy;  // used by TW only
         // per z point constants with the weight folded in (wave-uniform)
         T mwz[M], cwz[M], swz[M], rwz[M];
 #pragma unroll
@@ -140,8 +166,8 @@ __device__ __forceinline__ void rhs3_element(rhs3_lds<T, P, TX, TY, TZ, KF, SRC>
                 T bx[M], dx[M];
 #pragma unroll
                 for (int a = 0; a < M; ++a) {
-                    bx[a] = L.fvx[(lx * M + qx) * M + a];
-                    dx[a] = L.fdx[(lx * M + qx) * M + a];
+                    bx[a] = L.xt[FV][(lx * M + qx) * M + a];
+                    dx[a] = L.xt[FD][(lx * M + qx) * M + a];
                 }
                 // stage x: contract the element's first index
                 T A0[M][M], A1[M][M], KA[KF ? M : 1][KF ? M : 1], SA[SRC ? M : 1][SRC ? M : 1];
@@ -176,8 +202,8 @@ __device__ __forceinline__ void rhs3_element(rhs3_lds<T, P, TX, TY, TZ, KF, SRC>
                     T by[M], dy[M];
 #pragma unroll
                     for (int b = 0; b < M; ++b) {
-                        by[b] = L.fvy[(ly * M + qy) * M + b];
-                        dy[b] = L.fdy[(ly * M + qy) * M + b];
+                        by[b] = L.yt[FV][(ly * M + qy) * M + b];
+                        dy[b] = L.yt[FD][(ly * M + qy) * M + b];
                     }
                     // stage y for this Gauss point
                     T a00[M], a01[M], a10[M], ka[KF ? M : 1], sa[SRC ? M : 1];
@@ -239,7 +265,8 @@ __device__ __forceinline__ void rhs3_element(rhs3_lds<T, P, TX, TY, TZ, KF, SRC>
                     // stage y transposed (weights and -dt*grad_y folded into the tables)
 #pragma unroll
                     for (int b = 0; b < M; ++b) {
-                        const T wb = L.bvy[(ly * M + qy) * M + b], wd = L.bdy[(ly * M + qy) * M + b];
+                        const T wb = L.yt[BV][(ly * M + qy) * M + b];
+                        const T wd = TW ? L.yt[BD][(ly * M + qy) * M + b] * cy : L.yt[BD][(ly * M + qy) * M + b];
 #pragma unroll
                         for (int c = 0; c < M; ++c) {
                             K0[b][c] += wb * H0[c];
@@ -251,7 +278,8 @@ __device__ __forceinline__ void rhs3_element(rhs3_lds<T, P, TX, TY, TZ, KF, SRC>
                 // stage x transposed
 #pragma unroll
                 for (int a = 0; a < M; ++a) {
-                    const T wb = L.bvx[(lx * M + qx) * M + a], wd = L.bdx[(lx * M + qx) * M + a];
+                    const T wb = L.xt[BV][(lx * M + qx) * M + a];
+                    const T wd = TW ? L.xt[BD][(lx * M + qx) * M + a] * cx : L.xt[BD][(lx * M + qx) * M + a];
 #pragma unroll
                     for (int b = 0; b < M; ++b)
 #pragma unroll
@@ -282,83 +310,202 @@ __device__ __forceinline__ void rhs3_element(rhs3_lds<T, P, TX, TY, TZ, KF, SRC>
     }
 }
 
+// Stage the tile whose first element is o into B through registers: its coefficient block
+// (zero outside the mesh) and its x and y tables, the backward ones weighted and with
+// -dt*grad folded in.
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC>
+__device__ __forceinline__ void rhs3_stage(rhs3_buf<T, P, TX, TY, TZ, KF, SRC>& B, tile3 o,
+                                           const T* __restrict__ up, const dev_tables<T>& X,
+                                           const dev_tables<T>& Y, int nz, const rhs_params<T>& prm,
+                                           int tid) {
+    using BUF = rhs3_buf<T, P, TX, TY, TZ, KF, SRC>;
+    constexpr int M = P + 1, NT = TX * TY * TZ, BX = BUF::BX, BY = BUF::BY, NB = BUF::NB;
+    static_assert(TX * M * M <= NT && TY * M * M <= NT, "one table entry per thread");
+    const int nx = X.dofs, ny = Y.dofs;
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, r = i / BX, by = r % BY, bz = r / BY;
+        const int gx = o.ex0 + bx, gy = o.ey0 + by, gz = o.ez0 + bz;
+        const bool in = gx < nx && gy < ny && gz < nz;
+        const long g = gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz);
+        B.blk[i] = in ? up[g] : T(0);
+        if constexpr (KF) B.kb[i] = in ? prm.kfield[g] : T(0);
+        if constexpr (SRC) B.sb[i] = in ? prm.source[g] : T(0);
+    }
+    const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy;
+    if (tid < TX * M * M) {
+        const int e = o.ex0 + tid / (M * M), q = (tid / M) % M;
+        const bool in = e < X.elements;
+        const T v = in ? X.val[static_cast<long>(o.ex0) * M * M + tid] : T(0);
+        const T d = in ? X.der[static_cast<long>(o.ex0) * M * M + tid] : T(0);
+        const T w = in ? X.wj[static_cast<long>(e) * M + q] : T(0);
+        B.xt[FV][tid] = v;
+        B.xt[FD][tid] = d;
+        B.xt[BV][tid] = v * w;
+        B.xt[BD][tid] = cx * (d * w);
+    }
+    if (tid < TY * M * M) {
+        const int e = o.ey0 + tid / (M * M), q = (tid / M) % M;
+        const bool in = e < Y.elements;
+        const T v = in ? Y.val[static_cast<long>(o.ey0) * M * M + tid] : T(0);
+        const T d = in ? Y.der[static_cast<long>(o.ey0) * M * M + tid] : T(0);
+        const T w = in ? Y.wj[static_cast<long>(e) * M + q] : T(0);
+        B.yt[FV][tid] = v;
+        B.yt[FD][tid] = d;
+        B.yt[BV][tid] = v * w;
+        B.yt[BD][tid] = cy * (d * w);
+    }
+}
+
+// Copy n 32-bit words into LDS at dst (wave-uniform) from the global words src(k), k < n,
+// with global-to-LDS loads, which use no registers and complete in the background: wave
+// `wave` of NW takes the 64-word chunks wave, wave + NW, ...
+template <int NW, typename F>
+__device__ __forceinline__ void lds_fetch(uint32_t* dst, int n, int wave, int lane, F&& src) {
+    for (int j = wave; j * 64 < n; j += NW) {
+        const int k = j * 64 + lane;
+        if (k < n) __builtin_amdgcn_global_load_lds(src(k), dst + j * 64, 4, 0, 0);
+    }
+}
+
+// Start copying the tile whose first element is o into B: the coefficient block (indices
+// outside the mesh clamped to the last DOF: those entries are read by no element of the
+// mesh) and the tables as stored (zero-padded per tile; weighted, no -dt*grad).
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC>
+__device__ __forceinline__ void rhs3_fetch(rhs3_buf<T, P, TX, TY, TZ, KF, SRC>& B, tile3 o,
+                                           const T* __restrict__ up, const dev_tables<T>& X,
+                                           const dev_tables<T>& Y, int nz, const rhs_params<T>& prm,
+                                           int wave, int lane) {
+    using BUF = rhs3_buf<T, P, TX, TY, TZ, KF, SRC>;
+    constexpr int M = P + 1, NW = TZ, BX = BUF::BX, BY = BUF::BY, NB = BUF::NB;
+    constexpr int DW = sizeof(T) / 4;  // words per value
+    const int nx = X.dofs, ny = Y.dofs;
+    auto block_word = [&](const T* base) {
+        return [=](int k) {
+            const int i = k / DW, r = i / BX;
+            const int gx = min(o.ex0 + i % BX, nx - 1), gy = min(o.ey0 + r % BY, ny - 1);
+            const int gz = min(o.ez0 + r / BY, nz - 1);
+            const long g = gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz);
+            return reinterpret_cast<const uint32_t*>(base + g) + k % DW;
+        };
+    };
+    lds_fetch<NW>(reinterpret_cast<uint32_t*>(B.blk), NB * DW, wave, lane, block_word(up));
+    if constexpr (KF)
+        lds_fetch<NW>(reinterpret_cast<uint32_t*>(B.kb), NB * DW, wave, lane, block_word(prm.kfield));
+    if constexpr (SRC)
+        lds_fetch<NW>(reinterpret_cast<uint32_t*>(B.sb), NB * DW, wave, lane, block_word(prm.source));
+    // the tables: one copy per table, each from a wave-uniform base
+    const T* xs[4] = {X.val + static_cast<long>(o.ex0) * M * M, X.der + static_cast<long>(o.ex0) * M * M,
+                      X.bval + static_cast<long>(o.ex0) * M * M, X.bder + static_cast<long>(o.ex0) * M * M};
+    const T* ys[4] = {Y.val + static_cast<long>(o.ey0) * M * M, Y.der + static_cast<long>(o.ey0) * M * M,
+                      Y.bval + static_cast<long>(o.ey0) * M * M, Y.bder + static_cast<long>(o.ey0) * M * M};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t* xw = reinterpret_cast<const uint32_t*>(xs[t]);
+        const uint32_t* yw = reinterpret_cast<const uint32_t*>(ys[t]);
+        lds_fetch<NW>(reinterpret_cast<uint32_t*>(B.xt[t]), TX * M * M * DW, wave, lane,
+                      [=](int k) { return xw + k; });
+        lds_fetch<NW>(reinterpret_cast<uint32_t*>(B.yt[t]), TY * M * M * DW, wave, lane,
+                      [=](int k) { return yw + k; });
+    }
+}
+
+// DOF plane k of the tile's block collects plane k - lz of the waves lz = k-P .. k; the sum
+// is added to the rhs with a return-less global add (the tiles of one colour share no DOF,
+// so each DOF has one writer per launch and the result does not depend on timing). With
+// `clear`, the planes are left zero for the next tile.
+template <typename T, int P, int TX, int TY, int TZ>
+__device__ __forceinline__ void rhs3_add_block(double* acc, tile3 o, T* __restrict__ rhs, int nx,
+                                               int ny, int nz, int tid, bool clear) {
+    constexpr int M = P + 1, BX = TX + P, BY = TY + P, PL = BX * BY, NB = PL * (TZ + P);
+    constexpr int NT = TX * TY * TZ;
+    for (int i = tid; i < NB; i += NT) {
+        const int bxy = i % PL, bz = i / PL;
+        const int gx = o.ex0 + bxy % BX, gy = o.ey0 + bxy / BX, gz = o.ez0 + bz;
+        double sum = 0;
+#pragma unroll
+        for (int d = 0; d < M; ++d) {
+            const int w = bz - d;
+            if (w >= 0 && w < TZ) {
+                sum += acc[(w * M + d) * PL + bxy];
+                if (clear) acc[(w * M + d) * PL + bxy] = 0.0;
+            }
+        }
+        if (gx < nx && gy < ny && gz < nz)
+            unsafeAtomicAdd(rhs + (gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz)),
+                            static_cast<T>(sum));
+    }
+}
+
 // One workgroup per tile (block b -> tile by XCD-contiguous numbering).
 template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX, int VAR>
 __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
     dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
-    using LDS = rhs3_lds<T, P, TX, TY, TZ, KF, SRC>;
-    constexpr int M = P + 1, NT = TX * TY * TZ;
-    constexpr int BX = LDS::BX, BY = LDS::BY, NB = LDS::NB, PL = LDS::PL;
+    constexpr int NT = TX * TY * TZ, NA = rhs3_acc_size<P, TX, TY, TZ>();
     static_assert(TX * TY == 64, "a wave must cover exactly one z-layer of the tile");
-    static_assert(TX * M * M <= NT && TY * M * M <= NT, "one table entry per thread");
-    __shared__ LDS L;
+    __shared__ rhs3_lds<T, P, TX, TY, TZ, KF, SRC, 1> S;
+    auto& L = S.b[0];
+    double* acc = S.acc[0];
 
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = (tid / TX) % TY;
     const int lz = __builtin_amdgcn_readfirstlane(tid / (TX * TY));
-    const int nx = X.dofs, ny = Y.dofs, nz = Z.dofs;
-    const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy;
+    // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD gets a
+    // contiguous run of tiles (one slab of the domain), whose halos the next colour's launch
+    // then finds in that XCD's L2.
+    const tile3 o = rhs3_tile<TX, TY, TZ>(xcd_contiguous(blockIdx.x, gridDim.x), colour, ntx2, nty2);
+    // the tile's DOF block (first(e) = e for these knot vectors: dof origin = element origin)
+    rhs3_stage(L, o, up, X, Y, Z.dofs, prm, tid);
+    for (int i = tid; i < NA; i += NT) acc[i] = 0.0;
+    __syncthreads();
+    rhs3_element<T, P, TX, TY, TZ, KF, SRC, RX, VAR>(L, acc, X, Y, Z, prm, lx, ly, lz, o.ex0,
+                                                      o.ey0, o.ez0);
+    __syncthreads();
+    rhs3_add_block<T, P, TX, TY, TZ>(acc, o, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
+}
 
-    {
-        // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD
-        // gets a contiguous run of tiles (one slab of the domain), whose halos the next
-        // colour's launch then finds in that XCD's L2.
-        const tile3 o = rhs3_tile<TX, TY, TZ>(xcd_contiguous(blockIdx.x, gridDim.x), colour,
-                                              ntx2, nty2);
-        // the tile's DOF block (first(e) = e for these knot vectors: dof origin = element origin)
-        for (int i = tid; i < NB; i += NT) {
-            const int bx = i % BX, r = i / BX, by = r % BY, bz = r / BY;
-            const int gx = o.ex0 + bx, gy = o.ey0 + by, gz = o.ez0 + bz;
-            const bool in = gx < nx && gy < ny && gz < nz;
-            const long g = gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz);
-            L.blk[i] = in ? up[g] : T(0);
-            if constexpr (KF) L.kb[i] = in ? prm.kfield[g] : T(0);
-            if constexpr (SRC) L.sb[i] = in ? prm.source[g] : T(0);
-        }
-        if (tid < TX * M * M) {
-            const int e = o.ex0 + tid / (M * M), q = (tid / M) % M;
-            const bool in = e < X.elements;
-            const T v = in ? X.val[static_cast<long>(o.ex0) * M * M + tid] : T(0);
-            const T d = in ? X.der[static_cast<long>(o.ex0) * M * M + tid] : T(0);
-            const T w = in ? X.wj[static_cast<long>(e) * M + q] : T(0);
-            L.fvx[tid] = v;
-            L.fdx[tid] = d;
-            L.bvx[tid] = v * w;
-            L.bdx[tid] = cx * (d * w);
-        }
-        if (tid < TY * M * M) {
-            const int e = o.ey0 + tid / (M * M), q = (tid / M) % M;
-            const bool in = e < Y.elements;
-            const T v = in ? Y.val[static_cast<long>(o.ey0) * M * M + tid] : T(0);
-            const T d = in ? Y.der[static_cast<long>(o.ey0) * M * M + tid] : T(0);
-            const T w = in ? Y.wj[static_cast<long>(e) * M + q] : T(0);
-            L.fvy[tid] = v;
-            L.fdy[tid] = d;
-            L.bvy[tid] = v * w;
-            L.bdy[tid] = cy * (d * w);
-        }
-        for (int i = tid; i < TZ * M * PL; i += NT) L.acc[i] = 0.0;
+// Variant 3: two tiles per workgroup (tiles 2w, 2w+1 of the colour for workgroup w after
+// XCD-contiguous renumbering). The second tile's data is copied into its own LDS buffer by
+// global-to-LDS loads issued before the first tile's integration, so that its HBM/L2
+// latency is hidden behind that work instead of stalling every workgroup at its start.
+// Each tile has its own DOF planes and both are added to the rhs at the end: no global
+// write precedes the integrations, so their z tables stay in scalar registers.
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX>
+__global__ __launch_bounds__(TX* TY* TZ, 2) void k_rhs3_pipe(
+    const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
+    dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2, int ntiles) {
+    constexpr int NT = TX * TY * TZ, NA = rhs3_acc_size<P, TX, TY, TZ>();
+    static_assert(TX * TY == 64, "a wave must cover exactly one z-layer of the tile");
+    __shared__ rhs3_lds<T, P, TX, TY, TZ, KF, SRC, 2> S;
+    auto& B0 = S.b[0];
+    auto& B1 = S.b[1];
+    double* acc0 = S.acc[0];
+    double* acc1 = S.acc[1];
+
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int lx = tid % TX, ly = (tid / TX) % TY;
+    const int lz = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int w = xcd_contiguous(blockIdx.x, gridDim.x);
+    const bool two = 2 * w + 1 < ntiles;  // uniform over the workgroup
+    const tile3 o0 = rhs3_tile<TX, TY, TZ>(2 * w, colour, ntx2, nty2);
+    const tile3 o1 = rhs3_tile<TX, TY, TZ>(two ? 2 * w + 1 : 2 * w, colour, ntx2, nty2);
+
+    rhs3_fetch(B0, o0, up, X, Y, Z.dofs, prm, lz, lane);
+    for (int i = tid; i < NA; i += NT) acc0[i] = acc1[i] = 0.0;
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies into B0 have landed
+    __syncthreads();
+    if (two) rhs3_fetch(B1, o1, up, X, Y, Z.dofs, prm, lz, lane);
+    rhs3_element<T, P, TX, TY, TZ, KF, SRC, RX, 3>(B0, acc0, X, Y, Z, prm, lx, ly, lz, o0.ex0,
+                                                    o0.ey0, o0.ez0);
+    if (two) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // the copies into B1
         __syncthreads();
-
-        rhs3_element<T, P, TX, TY, TZ, KF, SRC, RX, VAR>(L, X, Y, Z, prm, lx, ly, lz, o.ex0,
-                                                          o.ey0, o.ez0);
-        __syncthreads();
-
-        // DOF plane k of the block collects plane k - lz of the waves lz = k-P .. k
-        for (int i = tid; i < NB; i += NT) {
-            const int bxy = i % PL, bz = i / PL;
-            const int gx = o.ex0 + bxy % BX, gy = o.ey0 + bxy / BX, gz = o.ez0 + bz;
-            if (gx >= nx || gy >= ny || gz >= nz) continue;
-            double sum = 0;
-#pragma unroll
-            for (int d = 0; d < M; ++d) {
-                const int w = bz - d;
-                if (w >= 0 && w < TZ) sum += L.acc[(w * M + d) * PL + bxy];
-            }
-            rhs[gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz)] += static_cast<T>(sum);
-        }
+        rhs3_element<T, P, TX, TY, TZ, KF, SRC, RX, 3>(B1, acc1, X, Y, Z, prm, lx, ly, lz, o1.ex0,
+                                                        o1.ey0, o1.ez0);
     }
+    __syncthreads();
+    rhs3_add_block<T, P, TX, TY, TZ>(acc0, o0, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
+    if (two) rhs3_add_block<T, P, TX, TY, TZ>(acc1, o1, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
 }
 
 // ------------------------------------------------------------------------------------
@@ -587,14 +734,19 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
     const int nty = (d[1].elements + TL::Y - 1) / TL::Y;
     const int ntz = (d[2].elements + TL::Z - 1) / TL::Z;
     constexpr int NT = TL::X * TL::Y * TL::Z;
-    const auto kernel = k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, RX, VAR>;
     for (int c = 0; c < 8; ++c) {
         const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
         const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2, nz2 = (ntz - cz + 1) / 2;
-        const long blocks = static_cast<long>(nx2) * ny2 * nz2;
-        if (blocks == 0) continue;
-        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm, c,
-                           nx2, ny2);
+        const long tiles = static_cast<long>(nx2) * ny2 * nz2;
+        if (tiles == 0) continue;
+        if constexpr (VAR == 3) {
+            hipLaunchKernelGGL((k_rhs3_pipe<T, P, TL::X, TL::Y, TL::Z, KF, SRC, RX>),
+                               dim3((tiles + 1) / 2), dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm,
+                               c, nx2, ny2, static_cast<int>(tiles));
+        } else {
+            hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, RX, VAR>), dim3(tiles),
+                               dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm, c, nx2, ny2);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -644,6 +796,7 @@ hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
     if (dim == 3) {
         if (variant == 1) return rhs3_fields<T, P, 1>(up, rhs, d, prm, s);
         if (variant == 2) return rhs3_fields<T, P, 2>(up, rhs, d, prm, s);
+        if (variant == 3) return rhs3_fields<T, P, 3>(up, rhs, d, prm, s);
         return rhs3_fields<T, P, 0>(up, rhs, d, prm, s);
     }
     if (kf && src) return rhs2_colours<T, P, true, true>(up, rhs, d, prm, s);

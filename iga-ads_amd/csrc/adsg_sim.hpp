@@ -38,9 +38,11 @@ struct dev_slot {
 struct dev_dir {
     adsg::direction tab;
     std::vector<dev_slot> slots;
-    void* val = nullptr;  // [elements][nq][p+1] in the step precision
+    void* val = nullptr;  // [elements][nq][p+1] in the step precision (padded, dev_tables)
     void* der = nullptr;
     void* wj = nullptr;
+    void* bval = nullptr;  // val, der times the Gauss weight (padded like val)
+    void* bder = nullptr;
     void* rows = nullptr;  // FP32 mode: slot-0 matrix as n rows of 2p+1 entries
     // the 1D Gram and stiffness matrices as rows of 2p+1 entries (step precision), for the
     // separable-integrand rhs (launch_rhs_kron)

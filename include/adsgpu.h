@@ -261,7 +261,10 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
 /* Kernel tuning (no counterpart in the reference, whose CPU loops have no launch shape):
  * ADSG_TUNE_RHS_VARIANT selects the 3D element-kernel variant (0: element coefficients held
  * in registers, 1: re-read from LDS, 2: as 1 with the element's contribution added to LDS
- * per x Gauss point); they compute the same sums (variant 2 adds them in another order).
+ * per x Gauss point, 3: two tiles per workgroup, the second tile's data copied to LDS while
+ * the first is integrated, integrated like 1 (like 2 at p = 3)); they compute the same sums
+ * (variant 2 adds them in another order; in FP64 variant 3 equals 1, or 2 at p = 3, bit for
+ * bit).
  * ADSG_TUNE_SWEEP_VARIANT selects the line sweep (0: lines cut into segments solved by
  * separate waves and joined through FP64-computed influence coefficients, where the factors
  * are unpivoted and the lines are 2..16 segments long; 1: one lane per whole line).

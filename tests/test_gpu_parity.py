@@ -108,7 +108,7 @@ def test_rhs_heat_matches_oracle(dim, p, ne):
     assert po.rel_l2(got, expect) < 1e-13
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("dim,p,ne", [(2, 2, (20, 17)), (3, 2, (9, 8, 10)), (2, 3, (11, 12)), (3, 1, (8, 9, 7)),
                                       (3, 3, (9, 7, 8))])
 def test_rhs_general_forms_match_oracle(dim, p, ne, variant):
@@ -293,7 +293,8 @@ def test_fp32_step_error_against_fp64_oracle():
 
 
 # ------------------------------------------------------------------ tuning variants
-@pytest.mark.parametrize("rhs_variant,sweep_variant", [(0, 0), (1, 1), (0, 1), (1, 0), (2, 0), (2, 1)])
+@pytest.mark.parametrize("rhs_variant,sweep_variant", [(0, 0), (1, 1), (0, 1), (1, 0), (2, 0), (2, 1), (3, 0),
+                                                       (3, 1)])
 @pytest.mark.parametrize("dim,p,ne", [(3, 2, (20, 17, 19)), (3, 3, (13, 11, 12)), (2, 2, (70, 45)),
                                       (3, 1, (30, 9, 8)), (2, 3, (129, 33))])
 def test_every_kernel_variant_matches_oracle(dim, p, ne, rhs_variant, sweep_variant):
@@ -314,6 +315,28 @@ def test_every_kernel_variant_matches_oracle(dim, p, ne, rhs_variant, sweep_vari
             u = o.solve(rhs)
         g.step(3, ag.rhs_form(1e-4), flags)
         assert po.rel_l2(g.get(ag.U), u) <= FP64_STEP_TOL
+
+
+@pytest.mark.parametrize("p,ne,same_as", [(2, (40, 33, 37), 1), (3, (26, 19, 23), 2), (1, (21, 30, 9), 1)])
+def test_pipelined_rhs_variant_is_bitwise_its_single_tile_variant(p, ne, same_as):
+    """Variant 3 (two tiles per workgroup, the second copied to LDS during the first's
+    integration) adds the same products in the same order as the single-tile variant with
+    the same register scheme, so in FP64 the two rhs agree bit for bit, for every form;
+    odd tile counts per colour leave workgroups with one tile."""
+    _log(f"case: pipelined rhs p={p} ne={ne}")
+    g, _ = _pair(3, p, ne)
+    rng = np.random.default_rng(5)
+    g.set(ag.U_PREV, rng.uniform(0, 1, g.size))
+    g.set(ag.KFIELD, np.exp(rng.uniform(-0.5, 0.5, g.size)))
+    g.set(ag.SOURCE, rng.standard_normal(g.size))
+    for form in (ag.rhs_form(1e-3), ag.rhs_form(2e-3, grad=(0.5, 1.0, 2.0), reaction=0.5),
+                 ag.rhs_form(1e-3, kfield=True, source=True)):
+        out = []
+        for v in (same_as, 3):
+            g.set_tuning(ag.TUNE_RHS_VARIANT, v)
+            g.compute_rhs(form)
+            out.append(g.get(ag.U))
+        assert np.array_equal(out[0], out[1])
 
 
 def test_segmented_sweep_on_implicit_matrices():
@@ -339,7 +362,7 @@ def test_autotune_keeps_state_and_picks_a_variant():
     u = o.init_gaussians(seed=2, count=2, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
     g.set(ag.U, u)
     v = g.autotune(ag.rhs_form(1e-4))
-    assert v in (0, 1, 2) and g.get_tuning(ag.TUNE_SWEEP_VARIANT) in (0, 1)
+    assert v in (0, 1, 2, 3) and g.get_tuning(ag.TUNE_SWEEP_VARIANT) in (0, 1)
     assert np.array_equal(g.get(ag.U), u)
     g.step(2, ag.rhs_form(1e-4))
     ref = o.solve(o.compute_rhs(u, po.heat_form(1e-4, 3), threads=8))
