@@ -300,13 +300,15 @@ def copy_calibration(mib, reps):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_rhs3_heat3d_p2_128.md")
 
 
-def launch_traffic():
-    """HBM bytes per k_rhs3 launch at 128^3 from the committed counter summary (the line
-    `traffic_bytes_per_launch: <bytes>`), or None when there is none."""
+def launch_traffic(kernel):
+    """HBM bytes per launch of the element kernel `kernel` (k_rhs3 or k_rhs3_pipe) at 128^3
+    from the committed counter summary (the line `traffic_bytes_per_launch <kernel>: <bytes>`),
+    or None when there is none."""
+    key = f"traffic_bytes_per_launch {kernel}:"
     try:
         with open(TRAFFIC_FILE) as fh:
             for line in fh:
-                if line.strip().startswith("traffic_bytes_per_launch:"):
+                if line.strip().startswith(key):
                     return float(line.split(":", 1)[1].split()[0])
     except OSError:
         pass
@@ -380,8 +382,10 @@ def main():
     value = N * args.steps / t64
     ms_step = 1e3 * t64 / args.steps
 
-    # dominant kernel: k_rhs3, launched once per parity colour (8 launches per step, each
-    # over E/8 elements); HIP events on the handle's stream bracket the 8 launches
+    # dominant kernel: the element kernel (k_rhs3, or k_rhs3_pipe for variant 3), launched
+    # once per parity colour (8 launches per step, each over E/8 elements); HIP events on the
+    # handle's stream bracket the 8 launches
+    rhs_kernel = "k_rhs3_pipe" if var64 == 3 else "k_rhs3"
     launches = 8
     launch_flops = f_elem(3, 2) * E / launches
     launch_ms = ph64["rhs"] / launches
@@ -409,13 +413,14 @@ def main():
             "dim": 3, "p": 2, "elements": ne, "dofs": N, "dt": dt,
         },
         "roofline": {
-            "kernel": "k_rhs3 (element-loop RHS assembly; one launch per parity colour)",
+            "kernel": f"{rhs_kernel} (element-loop RHS assembly, variant {var64}; one launch "
+                      "per parity colour)",
             "bound": "mfma",
             "achieved": rhs_tflops,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": rhs_tflops / PEAK_FP64_TFLOPS,
-            "traffic": launch_traffic(),
+            "traffic": launch_traffic(rhs_kernel),
             "launch_flops": launch_flops,
             "launch_ms": launch_ms,
             "note": ("FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 matrix peak = "
