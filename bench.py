@@ -59,6 +59,18 @@ def run_form(ag, precision, ne, steps, warmup, u0, dt, variant=None, kron=False)
     return (t1 - t0), phases, out, variant
 
 
+def func_rel_l2(ag, p, ne, got, ref):
+    """Relative error of the 3D spline with coefficients `got` against `ref` in the function
+    L2 norm (the M-weighted norm sqrt(e^T M e), by Gauss quadrature on the device)."""
+    s = ag.Simulation(3, [p] * 3, [ne] * 3, precision=ag.FP64)
+    s.set(ag.U, got - ref)
+    e = s.norm(0)
+    s.set(ag.U, ref)
+    r = s.norm(0)
+    s.close()
+    return e / r
+
+
 def run_implicit(ag, precision, ne, steps, warmup, u0, f, dt, flags, kron=False):
     """Implicit delta-form heat step with source (p=3): u += A^-1 dt (f - K u),
     A = (x)_d (M_d + dt/2 K_d)."""
@@ -110,7 +122,8 @@ def implicit_block(ag, np, ne, steps, warmup):
         res[name] = {"value": N * steps / t32, "ms_per_step": 1e3 * t32 / steps,
                      "speedup_vs_fp64": t64 / t32, "phases_ms": ph32, "rhs_variant": v32,
                      f"rel_l2_vs_fp64_after_{total}_steps":
-                         float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))}
+                         float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64)),
+                     f"function_rel_l2_vs_fp64_after_{total}_steps": func_rel_l2(ag, 3, ne, x32, x64)}
     # the same steps with the rhs of this separable integrand through the 1D matrices
     kr = {}
     form = ag.rhs_form(dt, mass=0.0, source=True)
@@ -120,7 +133,9 @@ def implicit_block(ag, np, ne, steps, warmup):
         kr[name] = {"value": N * steps / tk, "ms_per_step": 1e3 * tk / steps, "phases_ms": phk,
                     "speedup_vs_element_fp64": t64 / tk,
                     f"rel_l2_vs_element_fp64_after_{total}_steps":
-                        float(np.linalg.norm(xk - x64) / np.linalg.norm(x64))}
+                        float(np.linalg.norm(xk - x64) / np.linalg.norm(x64)),
+                    f"function_rel_l2_vs_element_fp64_after_{total}_steps":
+                        func_rel_l2(ag, 3, ne, xk, x64)}
         if prec == ag.FP64:  # the adjoint of ||u(T)||^2 with the same rhs form
             sk.set(ag.U, 2.0 * xk)
             sk.adjoint_step_delta(1, form, slots)
@@ -450,6 +465,7 @@ def main():
                 "ms_per_step": 1e3 * t32 / args.steps,
                 "speedup_vs_fp64": t64 / t32,
                 "rel_l2_vs_fp64_after_all_steps": err,
+                "function_rel_l2_vs_fp64_after_all_steps": func_rel_l2(ag, 2, ne, x32, x64),
                 "rhs_variant": var32,
                 "phases_ms": ph32,
             },
@@ -463,7 +479,9 @@ def main():
             kron[name] = {"value": N * args.steps / tk, "ms_per_step": 1e3 * tk / args.steps,
                           "speedup_vs_element_fp64": t64 / tk, "phases_ms": phk,
                           "rel_l2_vs_element_fp64_after_all_steps":
-                              float(np.linalg.norm(xk - x64) / np.linalg.norm(x64))}
+                              float(np.linalg.norm(xk - x64) / np.linalg.norm(x64)),
+                          "function_rel_l2_vs_element_fp64_after_all_steps":
+                              func_rel_l2(ag, 2, ne, xk, x64)}
             # the rhs kernel reads u and writes rhs once: 2 N s bytes per step
             nbytes = 2 * N * (8 if prec == ag.FP64 else 4)
             kron[name]["rhs_roofline"] = {

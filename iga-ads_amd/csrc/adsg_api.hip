@@ -103,17 +103,16 @@ int upload_tables(adsg_sim* h, dev_dir& d) {
         if ((rc = upload<float>(h, &d.bval, bval))) return rc;
         if ((rc = upload<float>(h, &d.bder, bder))) return rc;
         if ((rc = upload<float>(h, &d.wj, d.tab.wj))) return rc;
-        if ((rc = upload<float>(h, &d.kM, M))) return rc;
-        if ((rc = upload<float>(h, &d.kK, K))) return rc;
     } else {
         if ((rc = upload<double>(h, &d.val, val))) return rc;
         if ((rc = upload<double>(h, &d.der, der))) return rc;
         if ((rc = upload<double>(h, &d.bval, bval))) return rc;
         if ((rc = upload<double>(h, &d.bder, bder))) return rc;
         if ((rc = upload<double>(h, &d.wj, d.tab.wj))) return rc;
-        if ((rc = upload<double>(h, &d.kM, M))) return rc;
-        if ((rc = upload<double>(h, &d.kK, K))) return rc;
     }
+    // the 1D matrices in FP64 in both precisions (the separable rhs contracts in FP64)
+    if ((rc = upload<double>(h, &d.kM, M))) return rc;
+    if ((rc = upload<double>(h, &d.kK, K))) return rc;
     return ADSG_OK;
 }
 
@@ -323,14 +322,15 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
     if (e != hipSuccess) return hip_fail(h, e, "rhs start");
     mark(h, 1);
     if (kron) {
-        adsg::kron_dir<T> kd[3];
+        adsg::kron_dir kd[3];
         for (int d = 0; d < 3; ++d) {
-            kd[d].M = static_cast<const T*>(h->dirs[d].kM);
-            kd[d].K = static_cast<const T*>(h->dirs[d].kK);
+            kd[d].M = static_cast<const double*>(h->dirs[d].kM);
+            kd[d].K = static_cast<const double*>(h->dirs[d].kK);
             kd[d].n = h->dirs[d].tab.dofs;
         }
-        const T grad[3] = {prm.gx, prm.gy, prm.gz};
-        e = adsg::launch_rhs_kron<T>(h->dim, h->dirs[0].tab.p, up, rhs, kd, prm.mass, prm.dt, grad,
+        // the form's constants as given (FP64), the FP32 increment form's mass - 1 included
+        const double mass = sizeof(T) == sizeof(float) && !delta ? form->mass - 1.0 : form->mass;
+        e = adsg::launch_rhs_kron<T>(h->dim, h->dirs[0].tab.p, up, rhs, kd, mass, form->dt, form->grad,
                                      form->use_source, h->stream);
     } else {
         e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->rhs_variant, h->stream);
@@ -470,7 +470,7 @@ int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, d
     hipError_t e;
     if (h->rhs_kron && h->dim == 3 && !form->use_kfield) {  // separable: the 1D-matrix form
         mark(h, 1);
-        adsg::kron_dir<double> kd[3];
+        adsg::kron_dir kd[3];
         for (int d = 0; d < 3; ++d) {
             kd[d].M = static_cast<const double*>(h->dirs[d].kM);
             kd[d].K = static_cast<const double*>(h->dirs[d].kK);
@@ -496,7 +496,7 @@ int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, d
 int gram_apply(adsg_sim* h, const double* v, double* out) {
     hipError_t e;
     if (h->rhs_kron && h->dim == 3) {
-        adsg::kron_dir<double> kd[3];
+        adsg::kron_dir kd[3];
         for (int d = 0; d < 3; ++d) {
             kd[d].M = static_cast<const double*>(h->dirs[d].kM);
             kd[d].K = static_cast<const double*>(h->dirs[d].kK);

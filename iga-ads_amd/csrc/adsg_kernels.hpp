@@ -157,18 +157,19 @@ hipError_t launch_norm(int dim, int p, const double* u, const double* sq, const 
                        int kind, double* partial, int blocks, double* out, hipStream_t s);
 
 // One direction's 1D Gram (M) and stiffness (K) matrices as rows of 2p+1 entries,
-// M[i*(2p+1) + b] = M(i, i+b-p) (0 outside the matrix), n rows.
-template <typename T>
+// M[i*(2p+1) + b] = M(i, i+b-p) (0 outside the matrix), n rows, in FP64.
 struct kron_dir {
-    const T* M = nullptr;
-    const T* K = nullptr;
+    const double* M = nullptr;
+    const double* K = nullptr;
     int n = 0;
 };
 // rhs (=, or += with accumulate) (mass Mz(x)My(x)Mx - dt sum_d grad_d K_d (x) M (x) M) u: the
-// element-loop rhs of separable integrands (3D)
+// element-loop rhs of separable integrands (3D). u and rhs in T; the contractions in FP64
+// in both precisions (the stiffness rows sum to zero, and FP32 sums of their products lose
+// the digits that the element kernels keep by adding in FP64).
 template <typename T>
-hipError_t launch_rhs_kron(int dim, int p, const T* u, T* rhs, const kron_dir<T>* dirs, T mass, T dt,
-                           const T* grad, bool accumulate, hipStream_t s);
+hipError_t launch_rhs_kron(int dim, int p, const T* u, T* rhs, const kron_dir* dirs, double mass,
+                           double dt, const double* grad, bool accumulate, hipStream_t s);
 
 // Load vector of the L2 projection of f (given at the Gauss points, fq[ix + NX*(iy +
 // NY*iz)], ix = e_x*(p+1) + q_x, NX = elements_x*(p+1)), added into out (FP64)

@@ -8,7 +8,8 @@
 // the same sums in another order. Applied as three banded contractions: a workgroup owns a
 // 32 x 8 column of DOFs and streams through a range of z-planes, staging each u plane with
 // its halo in LDS (the next plane's loads in flight meanwhile), contracting x, then y, and
-// keeping the last 2p+1 planes' y results in registers for the z contraction.
+// keeping the last 2p+1 planes' y results in registers for the z contraction. u and rhs are
+// stored in T; the matrices, the staged contractions and the sums are FP64.
 #include <hip/hip_runtime.h>
 
 #include "adsg_kernels.hpp"
@@ -19,25 +20,26 @@ namespace {
 
 template <typename T, int P, bool ACC>
 __global__ __launch_bounds__(256) void k_rhs_kron3(const T* __restrict__ u, T* __restrict__ rhs,
-                                                   kron_dir<T> X, kron_dir<T> Y, kron_dir<T> Z, T cm,
-                                                   T cx, T cy, T cz, int zchunk) {
+                                                   kron_dir X, kron_dir Y, kron_dir Z, double cm,
+                                                   double cx, double cy, double cz, int zchunk) {
+    using C = double;  // contraction precision
     constexpr int B = 2 * P + 1, TX = 32, TY = 8, UX = TX + 2 * P, UY = TY + 2 * P;
     constexpr int NU = (UX * UY + 255) / 256;  // staged plane entries per thread
     __shared__ T U[UY * UX];
-    __shared__ T XM[UY * TX], XK[UY * TX];
+    __shared__ C XM[UY * TX], XK[UY * TX];
     const int tid = threadIdx.x, i = tid % TX, j = tid / TX;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * zchunk;
     const int nx = X.n, ny = Y.n, nz = Z.n;
     const int z1 = z0 + zchunk < nz ? z0 + zchunk : nz;
     const int gx = x0 + i, gy = y0 + j;
     // this thread's rows of the 1D matrices: A(g, g + b - P), b = 0..2P
-    T mx[B], kx[B], my[B], ky[B];
+    C mx[B], kx[B], my[B], ky[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        mx[b] = gx < nx ? X.M[static_cast<long>(gx) * B + b] : T(0);
-        kx[b] = gx < nx ? X.K[static_cast<long>(gx) * B + b] : T(0);
-        my[b] = gy < ny ? Y.M[static_cast<long>(gy) * B + b] : T(0);
-        ky[b] = gy < ny ? Y.K[static_cast<long>(gy) * B + b] : T(0);
+        mx[b] = gx < nx ? X.M[static_cast<long>(gx) * B + b] : C(0);
+        kx[b] = gx < nx ? X.K[static_cast<long>(gx) * B + b] : C(0);
+        my[b] = gy < ny ? Y.M[static_cast<long>(gy) * B + b] : C(0);
+        ky[b] = gy < ny ? Y.K[static_cast<long>(gy) * B + b] : C(0);
     }
     T pre[NU];
     auto fetch = [&](int kz) {
@@ -51,9 +53,9 @@ __global__ __launch_bounds__(256) void k_rhs_kron3(const T* __restrict__ u, T* _
         }
     };
     // y-stage results of the last 2P+1 planes (q[2P] the newest)
-    T qmm[B], qkm[B], qmk[B];
+    C qmm[B], qkm[B], qmk[B];
 #pragma unroll
-    for (int c = 0; c < B; ++c) qmm[c] = qkm[c] = qmk[c] = T(0);
+    for (int c = 0; c < B; ++c) qmm[c] = qkm[c] = qmk[c] = C(0);
     fetch(z0 - P);
     for (int kz = z0 - P; kz < z1 + P; ++kz) {
 #pragma unroll
@@ -68,10 +70,10 @@ __global__ __launch_bounds__(256) void k_rhs_kron3(const T* __restrict__ u, T* _
         for (int h = 0; h < 2; ++h) {
             const int jj = j + h * TY;
             if (jj < UY) {
-                T sm = T(0), sk = T(0);
+                C sm = C(0), sk = C(0);
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
-                    const T v = U[jj * UX + i + b];
+                    const C v = U[jj * UX + i + b];
                     sm += mx[b] * v;
                     sk += kx[b] * v;
                 }
@@ -81,10 +83,10 @@ __global__ __launch_bounds__(256) void k_rhs_kron3(const T* __restrict__ u, T* _
         }
         __syncthreads();
         // y-stage
-        T ymm = T(0), ykm = T(0), ymk = T(0);
+        C ymm = C(0), ykm = C(0), ymk = C(0);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            const T am = XM[(j + b) * TX + i], ak = XK[(j + b) * TX + i];
+            const C am = XM[(j + b) * TX + i], ak = XK[(j + b) * TX + i];
             ymm += my[b] * am;
             ykm += ky[b] * am;
             ymk += my[b] * ak;
@@ -101,9 +103,9 @@ __global__ __launch_bounds__(256) void k_rhs_kron3(const T* __restrict__ u, T* _
         // z-stage for plane ko = kz - P (its planes ko-P .. ko+P are in the queue)
         const int ko = kz - P;
         if (ko >= z0 && ko < z1 && gx < nx && gy < ny) {
-            const T* mz = Z.M + static_cast<long>(ko) * B;
-            const T* kzr = Z.K + static_cast<long>(ko) * B;
-            T smm = T(0), smk = T(0), skm = T(0), szz = T(0);
+            const C* mz = Z.M + static_cast<long>(ko) * B;
+            const C* kzr = Z.K + static_cast<long>(ko) * B;
+            C smm = C(0), smk = C(0), skm = C(0), szz = C(0);
 #pragma unroll
             for (int c = 0; c < B; ++c) {
                 smm += mz[c] * qmm[c];
@@ -111,18 +113,19 @@ __global__ __launch_bounds__(256) void k_rhs_kron3(const T* __restrict__ u, T* _
                 skm += mz[c] * qkm[c];
                 szz += kzr[c] * qmm[c];
             }
-            T val = cm * smm;
+            C val = cm * smm;
             val += cx * smk;
             val += cy * skm;
             val += cz * szz;
             const long g = gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * ko);
-            rhs[g] = ACC ? rhs[g] + val : val;
+            rhs[g] = static_cast<T>(ACC ? static_cast<C>(rhs[g]) + val : val);
         }
     }
 }
 
 template <typename T, int P, bool ACC>
-hipError_t kron3(const T* u, T* rhs, const kron_dir<T>* d, T cm, T cx, T cy, T cz, hipStream_t s) {
+hipError_t kron3(const T* u, T* rhs, const kron_dir* d, double cm, double cx, double cy, double cz,
+                 hipStream_t s) {
     constexpr int zchunk = 16;
     const dim3 grid((d[0].n + 31) / 32, (d[1].n + 7) / 8, (d[2].n + zchunk - 1) / zchunk);
     hipLaunchKernelGGL((k_rhs_kron3<T, P, ACC>), grid, dim3(256), 0, s, u, rhs, d[0], d[1], d[2], cm,
@@ -133,10 +136,10 @@ hipError_t kron3(const T* u, T* rhs, const kron_dir<T>* d, T cm, T cx, T cy, T c
 }  // namespace
 
 template <typename T>
-hipError_t launch_rhs_kron(int dim, int p, const T* u, T* rhs, const kron_dir<T>* dirs, T mass, T dt,
-                           const T* grad, bool accumulate, hipStream_t s) {
+hipError_t launch_rhs_kron(int dim, int p, const T* u, T* rhs, const kron_dir* dirs, double mass,
+                           double dt, const double* grad, bool accumulate, hipStream_t s) {
     if (dim != 3) return hipErrorInvalidValue;
-    const T cx = -dt * grad[0], cy = -dt * grad[1], cz = -dt * grad[2];
+    const double cx = -dt * grad[0], cy = -dt * grad[1], cz = -dt * grad[2];
 #define ADSG_KRON(P_)                                                                        \
     if (p == P_) return accumulate ? kron3<T, P_, true>(u, rhs, dirs, mass, cx, cy, cz, s)    \
                                    : kron3<T, P_, false>(u, rhs, dirs, mass, cx, cy, cz, s);
@@ -147,9 +150,9 @@ hipError_t launch_rhs_kron(int dim, int p, const T* u, T* rhs, const kron_dir<T>
     return hipErrorInvalidValue;
 }
 
-template hipError_t launch_rhs_kron<double>(int, int, const double*, double*, const kron_dir<double>*,
-                                            double, double, const double*, bool, hipStream_t);
-template hipError_t launch_rhs_kron<float>(int, int, const float*, float*, const kron_dir<float>*, float,
-                                           float, const float*, bool, hipStream_t);
+template hipError_t launch_rhs_kron<double>(int, int, const double*, double*, const kron_dir*, double,
+                                            double, const double*, bool, hipStream_t);
+template hipError_t launch_rhs_kron<float>(int, int, const float*, float*, const kron_dir*, double,
+                                           double, const double*, bool, hipStream_t);
 
 }  // namespace adsg

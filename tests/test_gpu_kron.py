@@ -106,6 +106,37 @@ def test_kronecker_implicit_delta_step_with_source():
     g.close()
 
 
+def test_kronecker_implicit_fp32_refined_within_1e6_of_fp64():
+    """The c5 regime at 96^3 (dt = 1e-4, p = 3, smooth state and source, 10 steps): FP32 +
+    refinement through the separable rhs and through the element kernels both stay within
+    1e-6 of the FP64 element-loop steps. The stiffness rows sum to zero, so an rhs contracted
+    in FP32 loses digits to cancellation (4e-6 after 22 steps at 192^3); the separable path
+    contracts in FP64 from the FP32 shadow of u."""
+    from adsgpu import workloads as wl
+
+    p, ne, dt, steps = 3, [96] * 3, 1e-4, 10
+    print("case: kron implicit fp32 refined 96^3", flush=True)
+    u0 = wl.gaussians([p] * 3, ne, seed=2025, count=8, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+    f = wl.sine_source([p] * 3, ne, seed=2026)
+    form = ag.rhs_form(dt, mass=0.0, source=True)
+    out = {}
+    for name, prec, kron, flags in (("fp64", ag.FP64, 0, 0), ("kron32", ag.FP32, 1, ag.STEP_REFINE),
+                                    ("elem32", ag.FP32, 0, ag.STEP_REFINE)):
+        g = ag.Simulation(3, [p] * 3, ne, precision=prec)
+        for d in range(3):
+            g.set_matrix(d, 1, 1.0, 0.5 * dt)
+        g.prepare_matrices()
+        g.set_tuning(ag.TUNE_RHS_KRONECKER, kron)
+        g.set(ag.U, u0)
+        g.set(ag.SOURCE, f)
+        g.step_delta(steps, form, [1, 1, 1], flags)
+        out[name] = g.get(ag.U)
+        g.close()
+    errs = {k: po.rel_l2(out[k], out["fp64"]) for k in ("kron32", "elem32")}
+    print(f"  {errs}", flush=True)
+    assert max(errs.values()) <= 1e-6
+
+
 def test_kronecker_adjoint_delta_step_equals_element_adjoint():
     """adsg_adjoint_step_delta applies R^T = R (the linear form without source) to the
     transposed solve's result: the 1D-matrix form gives the element kernels' result."""
