@@ -94,7 +94,7 @@ struct rhs3_lds {
 
 // One element (lane lx, ly of wave lz) of the tile whose first element is (ex0, ey0, ez0):
 // sum-factorised integrand over its Gauss points, contribution added to the wave's planes.
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX, int VAR>
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM, int VAR>
 __device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF, SRC>& L,
                                              double* acc, const dev_tables<T>& X,
                                              const dev_tables<T>& Y, const dev_tables<T>& Z,
@@ -106,6 +106,10 @@ __device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF
     // element contribution added to the planes per x point (variant 3 at P = 3: registers)
     constexpr bool FLUSH = VAR == 2 || (VAR == 3 && P == 3);
     constexpr bool TW = VAR == 3;  // tables weighted only: -dt*grad applied as read
+    // FM: 0 the heat form (mass term), 1 with the reaction term, 2 no mass term (mass = 0, no
+    // reaction, no source: the delta and FP32 increment forms), where u itself is not needed
+    constexpr bool RX = FM == 1, MS = FM != 2;
+    static_assert(MS || (!SRC && !RX), "form 2 has no term in v other than the gradient terms");
     const int ez = ez0 + lz;
     // the wave's planes: in one add instruction the 64 lanes hit 64 distinct entries
     double* mine = acc + lz * M * PL;
@@ -233,14 +237,14 @@ __device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF
                         T u = 0, uz = 0, uy = 0, ux = 0, kv = 0, fv = 0;
 #pragma unroll
                         for (int c = 0; c < M; ++c) {
-                            u += Bz[qz][c] * a00[c];
+                            if constexpr (MS) u += Bz[qz][c] * a00[c];
                             uz += Dz[qz][c] * a00[c];
                             uy += Bz[qz][c] * a01[c];
                             ux += Bz[qz][c] * a10[c];
                             if constexpr (KF) kv += Bz[qz][c] * ka[c];
                             if constexpr (SRC) fv += Bz[qz][c] * sa[c];
                         }
-                        T g0 = mwz[qz] * u;
+                        T g0 = MS ? mwz[qz] * u : T(0);
                         if constexpr (RX) g0 += rwz[qz] * u * (T(1) - u);
                         if constexpr (SRC) g0 += swz[qz] * fv;
                         T gx, gy, gz;
@@ -256,7 +260,7 @@ __device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF
                         }
 #pragma unroll
                         for (int c = 0; c < M; ++c) {  // separate updates: two FMAs, no add
-                            H0[c] += Bz[qz][c] * g0;
+                            if constexpr (MS) H0[c] += Bz[qz][c] * g0;
                             H0[c] += Dz[qz][c] * gz;
                             Hx[c] += Bz[qz][c] * gx;
                             Hy[c] += Bz[qz][c] * gy;
@@ -437,7 +441,7 @@ __device__ __forceinline__ void rhs3_add_block(double* acc, tile3 o, T* __restri
 }
 
 // One workgroup per tile (block b -> tile by XCD-contiguous numbering).
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX, int VAR>
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM, int VAR>
 __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
     dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
@@ -458,7 +462,7 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     rhs3_stage(L, o, up, X, Y, Z.dofs, prm, tid);
     for (int i = tid; i < NA; i += NT) acc[i] = 0.0;
     __syncthreads();
-    rhs3_element<T, P, TX, TY, TZ, KF, SRC, RX, VAR>(L, acc, X, Y, Z, prm, lx, ly, lz, o.ex0,
+    rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, VAR>(L, acc, X, Y, Z, prm, lx, ly, lz, o.ex0,
                                                       o.ey0, o.ez0);
     __syncthreads();
     rhs3_add_block<T, P, TX, TY, TZ>(acc, o, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
@@ -470,7 +474,7 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
 // latency is hidden behind that work instead of stalling every workgroup at its start.
 // Each tile has its own DOF planes and both are added to the rhs at the end: no global
 // write precedes the integrations, so their z tables stay in scalar registers.
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, bool RX>
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM>
 __global__ __launch_bounds__(TX* TY* TZ, 2) void k_rhs3_pipe(
     const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
     dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2, int ntiles) {
@@ -495,12 +499,12 @@ __global__ __launch_bounds__(TX* TY* TZ, 2) void k_rhs3_pipe(
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies into B0 have landed
     __syncthreads();
     if (two) rhs3_fetch(B1, o1, up, X, Y, Z.dofs, prm, lz, lane);
-    rhs3_element<T, P, TX, TY, TZ, KF, SRC, RX, 3>(B0, acc0, X, Y, Z, prm, lx, ly, lz, o0.ex0,
+    rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, 3>(B0, acc0, X, Y, Z, prm, lx, ly, lz, o0.ex0,
                                                     o0.ey0, o0.ez0);
     if (two) {
         __builtin_amdgcn_s_waitcnt(0x0F70);  // the copies into B1
         __syncthreads();
-        rhs3_element<T, P, TX, TY, TZ, KF, SRC, RX, 3>(B1, acc1, X, Y, Z, prm, lx, ly, lz, o1.ex0,
+        rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, 3>(B1, acc1, X, Y, Z, prm, lx, ly, lz, o1.ex0,
                                                         o1.ey0, o1.ez0);
     }
     __syncthreads();
@@ -726,7 +730,7 @@ struct tile<3, 3> {
     static constexpr int X = 8, Y = 8, Z = 3;
 };
 
-template <typename T, int P, bool KF, bool SRC, bool RX, int VAR>
+template <typename T, int P, bool KF, bool SRC, int FM, int VAR>
 hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                         hipStream_t s) {
     using TL = tile<3, P>;
@@ -740,11 +744,11 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
         const long tiles = static_cast<long>(nx2) * ny2 * nz2;
         if (tiles == 0) continue;
         if constexpr (VAR == 3) {
-            hipLaunchKernelGGL((k_rhs3_pipe<T, P, TL::X, TL::Y, TL::Z, KF, SRC, RX>),
+            hipLaunchKernelGGL((k_rhs3_pipe<T, P, TL::X, TL::Y, TL::Z, KF, SRC, FM>),
                                dim3((tiles + 1) / 2), dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm,
                                c, nx2, ny2, static_cast<int>(tiles));
         } else {
-            hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, RX, VAR>), dim3(tiles),
+            hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, FM, VAR>), dim3(tiles),
                                dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm, c, nx2, ny2);
         }
         hipError_t e = hipGetLastError();
@@ -772,21 +776,23 @@ hipError_t rhs2_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
     return hipSuccess;
 }
 
-template <typename T, int P, bool RX, int VAR>
+template <typename T, int P, int FM, int VAR>
 hipError_t rhs3_forms(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                       hipStream_t s) {
     const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
-    if (kf && src) return rhs3_colours<T, P, true, true, RX, VAR>(up, rhs, d, prm, s);
-    if (kf) return rhs3_colours<T, P, true, false, RX, VAR>(up, rhs, d, prm, s);
-    if (src) return rhs3_colours<T, P, false, true, RX, VAR>(up, rhs, d, prm, s);
-    return rhs3_colours<T, P, false, false, RX, VAR>(up, rhs, d, prm, s);
+    if (kf && src) return rhs3_colours<T, P, true, true, FM, VAR>(up, rhs, d, prm, s);
+    if (kf) return rhs3_colours<T, P, true, false, FM, VAR>(up, rhs, d, prm, s);
+    if (src) return rhs3_colours<T, P, false, true, FM, VAR>(up, rhs, d, prm, s);
+    return rhs3_colours<T, P, false, false, FM, VAR>(up, rhs, d, prm, s);
 }
 
 template <typename T, int P, int VAR>
 hipError_t rhs3_fields(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                        hipStream_t s) {
-    if (prm.reaction != T(0)) return rhs3_forms<T, P, true, VAR>(up, rhs, d, prm, s);
-    return rhs3_forms<T, P, false, VAR>(up, rhs, d, prm, s);
+    if (prm.reaction != T(0)) return rhs3_forms<T, P, 1, VAR>(up, rhs, d, prm, s);
+    if (prm.mass == T(0) && !prm.kfield && !prm.source)
+        return rhs3_colours<T, P, false, false, 2, VAR>(up, rhs, d, prm, s);
+    return rhs3_forms<T, P, 0, VAR>(up, rhs, d, prm, s);
 }
 
 template <typename T, int P>

@@ -125,6 +125,7 @@ def test_rhs_general_forms_match_oracle(dim, p, ne, variant):
     dt = 2e-3
     cases = [
         (dict(grad=(0.0, 1.0, 0.0)), dict(grad=(0.0, 1.0, 0.0))),
+        (dict(mass=0.0, grad=(1.0, 0.5, 2.0)), dict(mass=0.0, grad=(1.0, 0.5, 2.0))),
         (dict(kfield=True), dict(kfield=kf)),
         (dict(reaction=1.0), dict(reaction=1.0)),
         (dict(source=True), dict(source=src)),
