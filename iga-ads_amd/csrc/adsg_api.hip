@@ -776,6 +776,16 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
         s0.ab = adsg::band_form(h->dirs[d].tab, 1.0, 0.0);  // gram_matrix_1d (dimension.cpp:17)
         h->dirs[d].slots.push_back(std::move(s0));
     }
+    // default element-kernel variant (adsg_autotune refines it): the one measured fastest for
+    // the degree and precision. p = 3: 2 (variants 0 and 1 need more registers than allow
+    // two waves per SIMD); FP64 p <= 2: 3 when the mesh has enough tiles of 256 elements for
+    // two workgroups per CU in every colour launch (2^21 elements), else 1; FP32: 1.
+    {
+        long E = 1;
+        for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
+        const int p = dc[0]->p;
+        h->rhs_variant = p == 3 ? 2 : h->precision == ADSG_FP64 && E >= (1L << 21) ? 3 : 1;
+    }
     if (shared) {
         h->stream = shared;
         h->owns_stream = false;

@@ -71,7 +71,7 @@ struct adsg_sim {
     bool pending = false;            // FP32: buffer holds an increment based on u_prev
     bool pending_dirichlet = false;  // FP32: boundary rows of the increment were fixed
     bool prepared = false;
-    int rhs_variant = 0;    // element-kernel tuning choice (adsg_set_tuning)
+    int rhs_variant = 1;    // element-kernel tuning choice (set per degree at creation)
     int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
     int rhs_kron = 0;       // 1: separable integrands through the 1D matrices (adsg_set_tuning)
     std::string err;

@@ -247,6 +247,16 @@ def test_config2_full_size_one_step_and_conservation():
     g.set(ag.U, u0)
     g.step(6, form)
     assert np.array_equal(g.get(ag.U), x6)
+    # the pipelined element variant (the one the autotuner picks at this size) gives the same
+    # bits as variant 1 at full size
+    for v in (1, 3):
+        g.set_tuning(ag.TUNE_RHS_VARIANT, v)
+        g.set(ag.U, u0)
+        g.step(1, form)
+        if v == 1:
+            xv1 = g.get(ag.U)
+    assert np.array_equal(g.get(ag.U), xv1)
+    assert po.rel_l2(xv1, y1) <= FP64_STEP_TOL
 
 
 def test_fp32_dirichlet_2d_against_fp64_oracle():
