@@ -403,6 +403,14 @@ def heat_k(sim, k, u0, steps, every, dt, dirichlet=True):
     return adsg_torch.heat_k(k, u0, sim.h.value, steps, every, dt, dirichlet)
 
 
+def heat_kf(sim, k, f, u0, steps, every, dt, dirichlet=True):
+    """As heat_k with a source field f as well (the form {mass 1, dt, k field, source f}),
+    differentiable in k, f and u0."""
+    from . import adsg_torch
+
+    return adsg_torch.heat_kf(k, f, u0, sim.h.value, steps, every, dt, dirichlet)
+
+
 class Batch:
     """Many independent 2D problems (degree 1..3, natural boundaries) advanced together with
     grouped launches and one captured graph per step count (adsg_batch_*)."""

@@ -1,9 +1,10 @@
-// libtorch autograd Function over the ADS heat step with a B-spline diffusion field k:
-// u(T) = step^N(u0; k), each step the explicit step of adsg_step with the form
-// {mass 1, dt, k field} (validation.hpp:95-119 with k as in flow.hpp:73-100). The backward
-// pass is the hand-written discrete adjoint run over the same steps in reverse, with the
-// forward states recomputed from checkpoints kept every `every` steps
-// (adsg_forward_checkpointed / adsg_adjoint_checkpointed).
+// libtorch autograd Functions over the ADS heat step with a B-spline diffusion field k
+// (HeatK), and with k and a B-spline source field f (HeatKF): u(T) = step^N(u0; k[, f]), each
+// step the explicit step of adsg_step with the form {mass 1, dt, k field[, source f]}
+// (validation.hpp:95-119 with k as in flow.hpp:73-100). The backward pass is the
+// hand-written discrete adjoint run over the same steps in reverse, with the forward states
+// recomputed from checkpoints kept every `every` steps (adsg_forward_checkpointed /
+// adsg_adjoint_checkpointed); it returns dL/dk, dL/df (dt M w per step) and dL/du0.
 //
 // Host code only (plain C++ over the C ABI); tensors are FP64 on the GPU and are exchanged
 // with the handle by device-to-device copies.
@@ -32,12 +33,13 @@ void device_sync() {
     if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("hipDeviceSynchronize failed");
 }
 
-adsg_rhs_form make_form(double dt) {
+adsg_rhs_form make_form(double dt, bool source = false) {
     adsg_rhs_form f{};
     f.mass = 1.0;
     f.dt = dt;
     f.grad[0] = f.grad[1] = f.grad[2] = 1.0;
     f.use_kfield = 1;
+    f.use_source = source ? 1 : 0;
     return f;
 }
 
@@ -100,9 +102,71 @@ struct HeatK : public torch::autograd::Function<HeatK> {
     }
 };
 
+struct HeatKF : public torch::autograd::Function<HeatKF> {
+    static Tensor forward(AutogradContext* ctx, Tensor k, Tensor f, Tensor u0, int64_t handle,
+                          int64_t steps, int64_t every, double dt, bool dirichlet) {
+        auto* h = reinterpret_cast<adsg_sim*>(handle);
+        const long n = adsg_size(h);
+        TORCH_CHECK(h && n > 0, "bad handle");
+        TORCH_CHECK(steps >= 1 && every >= 1, "steps and every must be positive");
+        check_vector(k, n, "k");
+        check_vector(f, n, "f");
+        check_vector(u0, n, "u0");
+        const int flags = dirichlet ? ADSG_STEP_DIRICHLET : 0;
+        const adsg_rhs_form form = make_form(dt, true);
+        device_sync();
+        check(h, adsg_set_vector_device(h, ADSG_KFIELD, k.data_ptr<double>()), "set k");
+        check(h, adsg_set_vector_device(h, ADSG_SOURCE, f.data_ptr<double>()), "set f");
+        check(h, adsg_set_vector_device(h, ADSG_U, u0.data_ptr<double>()), "set u0");
+        check(h, adsg_step(h, static_cast<int>(steps), &form, flags), "steps");
+        Tensor uT = torch::empty_like(u0);
+        check(h, adsg_get_vector_device(h, ADSG_U, uT.data_ptr<double>()), "get u(T)");
+        ctx->save_for_backward({k, f, u0});
+        ctx->saved_data["handle"] = handle;
+        ctx->saved_data["steps"] = steps;
+        ctx->saved_data["every"] = every;
+        ctx->saved_data["dt"] = dt;
+        ctx->saved_data["dirichlet"] = dirichlet;
+        return uT;
+    }
+
+    static tensor_list backward(AutogradContext* ctx, tensor_list grads) {
+        auto saved = ctx->get_saved_variables();
+        Tensor k = saved[0], f = saved[1], u0 = saved[2];
+        auto* h = reinterpret_cast<adsg_sim*>(ctx->saved_data["handle"].toInt());
+        const int steps = static_cast<int>(ctx->saved_data["steps"].toInt());
+        const int every = static_cast<int>(ctx->saved_data["every"].toInt());
+        const double dt = ctx->saved_data["dt"].toDouble();
+        const int flags = ctx->saved_data["dirichlet"].toBool() ? ADSG_STEP_DIRICHLET : 0;
+        const adsg_rhs_form form = make_form(dt, true);
+        Tensor lam = grads[0].to(torch::kFloat64).contiguous();
+        Tensor dk = torch::zeros_like(k);
+        Tensor df = torch::zeros_like(f);
+        Tensor du0 = torch::empty_like(u0);
+        device_sync();
+        check(h, adsg_set_vector_device(h, ADSG_KFIELD, k.data_ptr<double>()), "set k");
+        check(h, adsg_set_vector_device(h, ADSG_SOURCE, f.data_ptr<double>()), "set f");
+        check(h, adsg_set_vector_device(h, ADSG_U, u0.data_ptr<double>()), "set u0");
+        check(h, adsg_forward_checkpointed(h, steps, every, &form, flags), "checkpointed forward");
+        check(h, adsg_set_vector_device(h, ADSG_GRAD_K, dk.data_ptr<double>()), "clear dL/dk");
+        check(h, adsg_set_vector_device(h, ADSG_GRAD_F, df.data_ptr<double>()), "clear dL/df");
+        check(h, adsg_set_vector_device(h, ADSG_U, lam.data_ptr<double>()), "dL/du(T)");
+        check(h, adsg_adjoint_checkpointed(h, &form, flags), "adjoint");
+        check(h, adsg_get_vector_device(h, ADSG_GRAD_K, dk.data_ptr<double>()), "dL/dk");
+        check(h, adsg_get_vector_device(h, ADSG_GRAD_F, df.data_ptr<double>()), "dL/df");
+        check(h, adsg_get_vector_device(h, ADSG_U, du0.data_ptr<double>()), "dL/du0");
+        return {dk, df, du0, Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+    }
+};
+
 Tensor heat_k(Tensor k, Tensor u0, int64_t handle, int64_t steps, int64_t every, double dt,
               bool dirichlet) {
     return HeatK::apply(k, u0, handle, steps, every, dt, dirichlet);
+}
+
+Tensor heat_kf(Tensor k, Tensor f, Tensor u0, int64_t handle, int64_t steps, int64_t every,
+               double dt, bool dirichlet) {
+    return HeatKF::apply(k, f, u0, handle, steps, every, dt, dirichlet);
 }
 
 }  // namespace
@@ -113,4 +177,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           "hand-written adjoint with checkpoints every `every` steps)",
           py::arg("k"), py::arg("u0"), py::arg("handle"), py::arg("steps"), py::arg("every"),
           py::arg("dt"), py::arg("dirichlet"));
+    m.def("heat_kf", &heat_kf,
+          "u(T) after `steps` explicit ADS heat steps with diffusion field k and source field f "
+          "(autograd in k, f and u0: hand-written adjoint with checkpoints every `every` steps)",
+          py::arg("k"), py::arg("f"), py::arg("u0"), py::arg("handle"), py::arg("steps"),
+          py::arg("every"), py::arg("dt"), py::arg("dirichlet"));
 }
