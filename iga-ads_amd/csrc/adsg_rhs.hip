@@ -360,6 +360,10 @@ __device__ __forceinline__ void rhs3_stage(rhs3_buf<T, P, TX, TY, TZ, KF, SRC>& 
     }
 }
 
+// s_waitcnt operand (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8) that
+// waits for every vector memory operation of the wave and for nothing else: vmcnt(0)
+constexpr int kWaitVmem = 0x0F70;
+
 // Copy n 32-bit words into LDS at dst (wave-uniform) from the global words src(k), k < n,
 // with global-to-LDS loads, which use no registers and complete in the background: wave
 // `wave` of NW takes the 64-word chunks wave, wave + NW, ...
@@ -496,13 +500,13 @@ __global__ __launch_bounds__(TX* TY* TZ, 2) void k_rhs3_pipe(
 
     rhs3_fetch(B0, o0, up, X, Y, Z.dofs, prm, lz, lane);
     for (int i = tid; i < NA; i += NT) acc0[i] = acc1[i] = 0.0;
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies into B0 have landed
+    __builtin_amdgcn_s_waitcnt(kWaitVmem);  // this wave's copies into B0 have landed
     __syncthreads();
     if (two) rhs3_fetch(B1, o1, up, X, Y, Z.dofs, prm, lz, lane);
     rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, 3>(B0, acc0, X, Y, Z, prm, lx, ly, lz, o0.ex0,
                                                     o0.ey0, o0.ez0);
     if (two) {
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // the copies into B1
+        __builtin_amdgcn_s_waitcnt(kWaitVmem);  // the copies into B1
         __syncthreads();
         rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, 3>(B1, acc1, X, Y, Z, prm, lx, ly, lz, o1.ex0,
                                                         o1.ey0, o1.ez0);
