@@ -201,6 +201,7 @@ def ensemble_block(ag, np, count, steps, dt):
         for i, u in enumerate(u0):
             b.set(i, u)
         tile, tile_ms = b.autotune()
+        lines = ["segmented", "one lane per line"][b.get_tuning(ag.TUNE_SWEEP_VARIANT)]
         b.step(steps)  # capture, instantiate and warm the graph of `steps` steps
         b.synchronize()
         for i, u in enumerate(u0):
@@ -212,7 +213,7 @@ def ensemble_block(ag, np, count, steps, dt):
         out[name] = [b.get(i) for i in range(len(probs))]
         b.close()
         res[name] = {"value": dofs * steps / t, "ms_per_step": 1e3 * t / steps,
-                     "tile_shape": ["16x16", "8x8"][tile],
+                     "tile_shape": ["16x16", "8x8"][tile], "line_sweeps": lines,
                      "autotune_ms_per_step": list(map(float, tile_ms)),
                      "element_tflops": flops * steps / t / 1e12}
     err = max(float(np.linalg.norm(a - c) / np.linalg.norm(c)) for a, c in zip(out["fp32"], out["fp64"]))

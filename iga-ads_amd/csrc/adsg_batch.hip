@@ -539,23 +539,32 @@ int adsg_batch_autotune(adsg_batch* b, double* ms_out) {
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    int best = b->shape, rc = ADSG_OK;
+    // every (tile shape, line grouping) pair: 16 x 16 or 8 x 8 element tiles; lines either
+    // segmented (64 lines per workgroup of nine waves) or one lane per line (64 lines per
+    // one-wave workgroup)
+    int best = b->shape, best_sweep = b->sweep, rc = ADSG_OK;
     float best_ms = 1e30f;
     for (int shape = 0; shape <= 1 && rc == ADSG_OK; ++shape) {
-        b->shape = shape;
-        if ((rc = build(b))) break;
-        rc = batch_step(b);  // code load
-        (void)hipEventRecord(e0, b->stream);
-        for (int s = 0; s < 2 && rc == ADSG_OK; ++s) rc = batch_step(b);
-        (void)hipEventRecord(e1, b->stream);
-        (void)hipEventSynchronize(e1);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms_out) ms_out[shape] = ms / 2;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = shape;
+        float shape_ms = 1e30f;
+        for (int sweep = 0; sweep <= 1 && rc == ADSG_OK; ++sweep) {
+            b->shape = shape;
+            b->sweep = sweep;
+            if ((rc = build(b))) break;
+            rc = batch_step(b);  // code load
+            (void)hipEventRecord(e0, b->stream);
+            for (int s = 0; s < 2 && rc == ADSG_OK; ++s) rc = batch_step(b);
+            (void)hipEventRecord(e1, b->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            shape_ms = ms < shape_ms ? ms : shape_ms;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = shape;
+                best_sweep = sweep;
+            }
         }
+        if (ms_out) ms_out[shape] = shape_ms / 2;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -565,6 +574,8 @@ int adsg_batch_autotune(adsg_batch* b, double* ms_out) {
     for (void* q : saved) (void)hipFree(q);
     b->phase = phase0;
     b->shape = best;
+    b->sweep = best_sweep;
+    drop_batch_graphs(b);
     int rc2 = build(b);
     return rc ? rc : rc2;
 }

@@ -313,8 +313,9 @@ int adsg_batch_synchronize(adsg_batch* b);
 /* ADSG_TUNE_BATCH_TILE: element tile of the batched kernels, 0 = 16 x 16, 1 = 8 x 8.
  * ADSG_TUNE_SWEEP_VARIANT: 0 = segmented line sweeps (9 waves per 64 lines; used when every
  * problem's lines split into segments), 1 = one lane per line.
- * adsg_batch_autotune times one step with each tile shape (ms_out[0..1], may be NULL),
- * keeps the faster and restores the state. */
+ * adsg_batch_autotune times one step with each tile shape and each line grouping (the two
+ * sweep variants), keeps the fastest pair and restores the state; ms_out[0..1] (may be
+ * NULL) receives each tile shape's time with its better sweep variant. */
 #define ADSG_TUNE_BATCH_TILE 2
 int adsg_batch_set_tuning(adsg_batch* b, int key, int value);
 int adsg_batch_get_tuning(const adsg_batch* b, int key);

@@ -106,8 +106,11 @@ def test_batch_graph_replay_and_autotune_keep_results():
         b = _batch(ag.FP32, 0)
         if tune:
             shape, ms = b.autotune()
-            _log(f"  autotune: shape {shape}, ms per step {ms}")
+            sweep = b.get_tuning(ag.TUNE_SWEEP_VARIANT)
+            _log(f"  autotune: shape {shape}, sweep variant {sweep}, ms per step {ms}")
+            assert shape in (0, 1) and sweep in (0, 1)
             b.set_tuning(0)
+            b.set_tuning(0, ag.TUNE_SWEEP_VARIANT)
         for n in chunks:
             b.step(n)
         runs.append([b.get(i) for i in range(len(PROBLEMS))])
