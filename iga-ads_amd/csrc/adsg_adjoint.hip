@@ -153,24 +153,57 @@ __global__ __launch_bounds__(256) void k_norm(const double* __restrict__ u, cons
         const TT* dy = ty.der + static_cast<long>(ey) * n * n;
         const TT* vz = DIM == 3 ? tz.val + static_cast<long>(ez) * n * n : nullptr;
         const TT* dz = DIM == 3 ? tz.der + static_cast<long>(ez) * n * n : nullptr;
-        for (int qz = 0; qz < NZ; ++qz)
-            for (int qy = 0; qy < n; ++qy)
-                for (int qx = 0; qx < n; ++qx) {
+        // sum factorisation: contract x (values and derivatives), then y, then z, per Gauss
+        // point of the outer directions (9 n^4 products per element instead of 4 n^6)
+#pragma unroll 1
+        for (int qx = 0; qx < n; ++qx) {
+            double A0[NZ][n], A1[NZ][n];
+#pragma unroll
+            for (int c = 0; c < NZ; ++c)
+#pragma unroll
+                for (int b = 0; b < n; ++b) {
+                    double s0 = 0, s1 = 0;
+#pragma unroll
+                    for (int a = 0; a < n; ++a) {
+                        s0 += double(vx[qx * n + a]) * U[c][b][a];
+                        s1 += double(dx[qx * n + a]) * U[c][b][a];
+                    }
+                    A0[c][b] = s0;
+                    A1[c][b] = s1;
+                }
+#pragma unroll 1
+            for (int qy = 0; qy < n; ++qy) {
+                double a00[NZ], a01[NZ], a10[NZ];
+#pragma unroll
+                for (int c = 0; c < NZ; ++c) {
+                    double s00 = 0, s01 = 0, s10 = 0;
+#pragma unroll
+                    for (int b = 0; b < n; ++b) {
+                        const double by = vy[qy * n + b], bdy = dy[qy * n + b];
+                        s00 += by * A0[c][b];
+                        s01 += bdy * A0[c][b];
+                        s10 += by * A1[c][b];
+                    }
+                    a00[c] = s00;
+                    a01[c] = s01;
+                    a10[c] = s10;
+                }
+#pragma unroll
+                for (int qz = 0; qz < NZ; ++qz) {
                     double v = 0, gx = 0, gy = 0, gz = 0;
-                    for (int c = 0; c < NZ; ++c) {
-                        const double bz = DIM == 3 ? double(vz[qz * n + c]) : 1.0;
-                        const double bdz = DIM == 3 ? double(dz[qz * n + c]) : 0.0;
-                        for (int b = 0; b < n; ++b) {
-                            const double by = vy[qy * n + b], bdy = dy[qy * n + b];
-                            for (int a = 0; a < n; ++a) {
-                                const double bx = vx[qx * n + a], bdx = dx[qx * n + a];
-                                const double cu = U[c][b][a];
-                                v += cu * (bx * by * bz);
-                                gx += cu * (bdx * by * bz);
-                                gy += cu * (bx * bdy * bz);
-                                if (DIM == 3) gz += cu * (bx * by * bdz);
-                            }
+                    if constexpr (DIM == 3) {
+#pragma unroll
+                        for (int c = 0; c < NZ; ++c) {
+                            const double bz = vz[qz * n + c], bdz = dz[qz * n + c];
+                            v += bz * a00[c];
+                            gz += bdz * a00[c];
+                            gy += bz * a01[c];
+                            gx += bz * a10[c];
                         }
+                    } else {
+                        v = a00[0];
+                        gy = a01[0];
+                        gx = a10[0];
                     }
                     if (sq) {
                         const long pt = (static_cast<long>(ex) * n + qx) +
@@ -187,6 +220,8 @@ __global__ __launch_bounds__(256) void k_norm(const double* __restrict__ u, cons
                     const double g2 = gx * gx + gy * gy + gz * gz;
                     acc += w * (kind == 0 ? v * v : kind == 1 ? g2 : v * v + g2);
                 }
+            }
+        }
     }
     red[threadIdx.x] = acc;
     __syncthreads();
