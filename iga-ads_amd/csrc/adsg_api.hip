@@ -1512,7 +1512,8 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
         h->sweep_variant = v <= 1 ? v : best_sweep;  // both sweeps timed with variants 0, 1
         double ms[7];
         rc = adsg_time_phases(h, 1, form, slots, flags, ms);  // warm-up (code load, caches)
-        if (!rc) rc = adsg_time_phases(h, 3, form, slots, flags, ms);
+        // 8 timed steps: variants 1 and 3 are a few per cent apart at full-GPU sizes
+        if (!rc) rc = adsg_time_phases(h, 8, form, slots, flags, ms);
         if (!rc && ms[0] < best_ms) {
             best_ms = ms[0];
             best = v;
