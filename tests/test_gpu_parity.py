@@ -283,6 +283,27 @@ def test_fp32_dirichlet_2d_against_fp64_oracle():
     _log(f"FP32 2D Dirichlet worst per-step rel L2 vs FP64 oracle = {worst:.3e}")
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("p,ne,dt", [(2, (20, 17, 19), 1e-5), (3, (13, 11, 12), 1e-5), (1, (30, 9, 8), 1e-4)])
+def test_fp32_every_element_variant_against_fp64_oracle(p, ne, dt, variant):
+    """FP32 increment steps with each element-kernel variant (variant 3 reads tables weighted
+    on the host in FP64 and rounded, the others weight them on the device in FP32), with
+    fixed boundary rows: <= 1e-6 against the FP64 oracle after every one of 5 steps."""
+    _log(f"case: FP32 variant {variant} 3D p={p} ne={ne}")
+    g, o = _pair(3, p, ne, precision=ag.FP32, dirichlet=True)
+    g.set_tuning(ag.TUNE_RHS_VARIANT, variant)
+    u = o.init_gaussians(seed=31, count=3, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+    o.zero_boundary(u)
+    g.set(ag.U, u)
+    for _ in range(5):
+        rhs = o.compute_rhs(u, po.heat_form(dt, 3), threads=8)
+        o.zero_boundary(rhs)
+        u = o.solve(rhs)
+        g.step(1, ag.rhs_form(dt), ag.STEP_DIRICHLET)
+        assert po.rel_l2(g.get(ag.U), u) <= 1e-6
+    g.close()
+
+
 def test_fp32_step_error_against_fp64_oracle():
     """FP32 increment step (FP32 assembly of rhs - M u_prev, FP32 substitution on
     FP64-computed factors, FP64 state update), 3D p=2 32^3, 20 steps: relative L2 error
