@@ -778,13 +778,13 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
     }
     // default element-kernel variant (adsg_autotune refines it): the one measured fastest for
     // the degree and precision. p = 3: 2 (variants 0 and 1 need more registers than allow
-    // two waves per SIMD); FP64 p <= 2: 3 when the mesh has enough tiles of 256 elements for
+    // two waves per SIMD); FP64 p <= 2: 4 when the mesh has enough tiles of 512 elements for
     // two workgroups per CU in every colour launch (2^21 elements), else 1; FP32: 1.
     {
         long E = 1;
         for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
         const int p = dc[0]->p;
-        h->rhs_variant = p == 3 ? 2 : h->precision == ADSG_FP64 && E >= (1L << 21) ? 3 : 1;
+        h->rhs_variant = p == 3 ? 2 : h->precision == ADSG_FP64 && E >= (1L << 21) ? 4 : 1;
     }
     if (shared) {
         h->stream = shared;
@@ -1468,7 +1468,7 @@ int adsg_set_tuning(adsg_sim* h, int key, int value) {
                 : key == ADSG_TUNE_SWEEP_VARIANT ? &h->sweep_variant
                 : key == ADSG_TUNE_RHS_KRONECKER ? &h->rhs_kron
                                                  : nullptr;
-    const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 3 : 1;
+    const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 4 : 1;
     if (!slot || value < 0 || value > max_value) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
     if (*slot != value) drop_graphs(h);
     *slot = value;
@@ -1507,7 +1507,7 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
     int best = h->rhs_variant, best_sweep = h->sweep_variant;
     double best_ms = 1e30, best_sweep_ms = 1e30;
     int rc = ADSG_OK;
-    for (int v = 0; v <= 3 && rc == ADSG_OK; ++v) {
+    for (int v = 0; v <= 4 && rc == ADSG_OK; ++v) {
         h->rhs_variant = v;
         h->sweep_variant = v <= 1 ? v : best_sweep;  // both sweeps timed with variants 0, 1
         double ms[7];

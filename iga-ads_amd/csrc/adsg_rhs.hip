@@ -45,7 +45,7 @@ __device__ __forceinline__ T uniform_load(const T* p) {
 
 template <typename T, int P, int VAR>
 constexpr int rhs3_waves() {  // minimum waves per SIMD the register allocation must allow
-    return VAR == 0 ? 1 : VAR == 1 || VAR == 3 ? 2 : P == 3 ? 2 : sizeof(T) == sizeof(float) ? 4 : 3;
+    return VAR == 0 ? 1 : VAR == 1 || VAR == 3 || VAR == 4 ? 2 : P == 3 ? 2 : sizeof(T) == sizeof(float) ? 4 : 3;
 }
 
 // first element of tile t of the given parity colour (ntx2, nty2 tiles of that colour per
@@ -93,13 +93,16 @@ struct rhs3_lds {
 };
 
 // One element (lane lx, ly of wave lz) of the tile whose first element is (ex0, ey0, ez0):
-// sum-factorised integrand over its Gauss points, contribution added to the wave's planes.
+// sum-factorised integrand over its Gauss points, its contribution accumulated in R (owned
+// and zeroed by the caller) and added to the wave's planes, or (variant 4) left in R for
+// the caller, which carries the planes shared with the wave's next element layer.
 template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM, int VAR>
 __device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF, SRC>& L,
                                              double* acc, const dev_tables<T>& X,
                                              const dev_tables<T>& Y, const dev_tables<T>& Z,
                                              const rhs_params<T>& prm, int lx, int ly, int lz,
-                                             int ex0, int ey0, int ez0) {
+                                             int ex0, int ey0, int ez0,
+                                             T (&R)[P + 1][P + 1][P + 1]) {
     using LDS = rhs3_buf<T, P, TX, TY, TZ, KF, SRC>;
     constexpr int M = P + 1, BX = LDS::BX, BY = LDS::BY, PL = LDS::PL;
     constexpr bool CREG = VAR == 0;  // coefficient block held in registers
@@ -110,17 +113,11 @@ __device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF
     // reaction, no source: the delta and FP32 increment forms), where u itself is not needed
     constexpr bool RX = FM == 1, MS = FM != 2;
     static_assert(MS || (!SRC && !RX), "form 2 has no term in v other than the gradient terms");
+    static_assert(VAR != 4 || P <= 2, "variant 4 carries the register block: p <= 2");
     const int ez = ez0 + lz;
     // the wave's planes: in one add instruction the 64 lanes hit 64 distinct entries
     double* mine = acc + lz * M * PL;
 
-    T R[FLUSH ? 1 : M][FLUSH ? 1 : M][FLUSH ? 1 : M];  // the element's contribution, (a, b, c)
-#pragma unroll
-    for (int a = 0; a < (FLUSH ? 1 : M); ++a)
-#pragma unroll
-        for (int b = 0; b < (FLUSH ? 1 : M); ++b)
-#pragma unroll
-            for (int c = 0; c < (FLUSH ? 1 : M); ++c) R[a][b][c] = T(0);
 
     if (ez < Z.elements) {  // wave-uniform
         // z tables of the wave's element layer: uniform addresses in uniform control flow,
@@ -301,7 +298,7 @@ __device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF
             }
         }
     }
-    if constexpr (!FLUSH) {
+    if constexpr (!FLUSH && VAR != 4) {
 #pragma unroll
         for (int c = 0; c < M; ++c)
 #pragma unroll
@@ -317,13 +314,14 @@ __device__ __forceinline__ void rhs3_element(const rhs3_buf<T, P, TX, TY, TZ, KF
 // Stage the tile whose first element is o into B through registers: its coefficient block
 // (zero outside the mesh) and its x and y tables, the backward ones weighted and with
 // -dt*grad folded in.
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC>
+template <int NTH = 0, typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC>
 __device__ __forceinline__ void rhs3_stage(rhs3_buf<T, P, TX, TY, TZ, KF, SRC>& B, tile3 o,
                                            const T* __restrict__ up, const dev_tables<T>& X,
                                            const dev_tables<T>& Y, int nz, const rhs_params<T>& prm,
                                            int tid) {
     using BUF = rhs3_buf<T, P, TX, TY, TZ, KF, SRC>;
-    constexpr int M = P + 1, NT = TX * TY * TZ, BX = BUF::BX, BY = BUF::BY, NB = BUF::NB;
+    // NTH: threads of the workgroup (default one per element of the tile)
+    constexpr int M = P + 1, NT = NTH ? NTH : TX * TY * TZ, BX = BUF::BX, BY = BUF::BY, NB = BUF::NB;
     static_assert(TX * M * M <= NT && TY * M * M <= NT, "one table entry per thread");
     const int nx = X.dofs, ny = Y.dofs;
     for (int i = tid; i < NB; i += NT) {
@@ -466,8 +464,9 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     rhs3_stage(L, o, up, X, Y, Z.dofs, prm, tid);
     for (int i = tid; i < NA; i += NT) acc[i] = 0.0;
     __syncthreads();
+    T R[P + 1][P + 1][P + 1] = {};
     rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, VAR>(L, acc, X, Y, Z, prm, lx, ly, lz, o.ex0,
-                                                      o.ey0, o.ez0);
+                                                      o.ey0, o.ez0, R);
     __syncthreads();
     rhs3_add_block<T, P, TX, TY, TZ>(acc, o, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
 }
@@ -503,17 +502,89 @@ __global__ __launch_bounds__(TX* TY* TZ, 2) void k_rhs3_pipe(
     __builtin_amdgcn_s_waitcnt(kWaitVmem);  // this wave's copies into B0 have landed
     __syncthreads();
     if (two) rhs3_fetch(B1, o1, up, X, Y, Z.dofs, prm, lz, lane);
+    T R0[P + 1][P + 1][P + 1] = {};
     rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, 3>(B0, acc0, X, Y, Z, prm, lx, ly, lz, o0.ex0,
-                                                    o0.ey0, o0.ez0);
+                                                    o0.ey0, o0.ez0, R0);
     if (two) {
         __builtin_amdgcn_s_waitcnt(kWaitVmem);  // the copies into B1
         __syncthreads();
+        T R1[P + 1][P + 1][P + 1] = {};
         rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, 3>(B1, acc1, X, Y, Z, prm, lx, ly, lz, o1.ex0,
-                                                        o1.ey0, o1.ez0);
+                                                        o1.ey0, o1.ez0, R1);
     }
     __syncthreads();
     rhs3_add_block<T, P, TX, TY, TZ>(acc0, o0, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
     if (two) rhs3_add_block<T, P, TX, TY, TZ>(acc1, o1, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
+}
+
+// Variant 4 (p <= 2): tiles of TX x TY x TZ elements with TZ/2 waves, wave w integrating the
+// element layers 2w and 2w+1 one after the other. Their contributions share P of their P+1
+// DOF planes, so the register block R is carried from the first layer to the second: after
+// the first only its lowest plane is added to LDS, after the second all P+1 planes — P+1
+// plane adds per two elements instead of 2(P+1) (a third fewer LDS adds at p = 2), and the
+// taller tile re-reads fewer halo planes. Wave w owns the LDS planes of DOF planes 2w ..
+// 2w+P+1; the block add and the colouring are those of the other variants.
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM>
+__global__ __launch_bounds__(TX* TY*(TZ / 2), 2) void k_rhs3_zpair(
+    const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
+    dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
+    constexpr int M = P + 1, NW = TZ / 2, NT = TX * TY * NW, BX = TX + P, BY = TY + P;
+    constexpr int PL = BX * BY, NB = PL * (TZ + P), WP = P + 2;  // WP: planes per wave
+    static_assert(TX * TY == 64 && TZ % 2 == 0 && TZ >= P, "one wave per pair of layers");
+    struct lds_t {
+        rhs3_buf<T, P, TX, TY, TZ, KF, SRC> b;
+        double acc[NW * WP * PL];
+    };
+    __shared__ lds_t S;
+
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = (tid / TX) % TY;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);
+    const tile3 o = rhs3_tile<TX, TY, TZ>(xcd_contiguous(blockIdx.x, gridDim.x), colour, ntx2, nty2);
+    rhs3_stage<NT>(S.b, o, up, X, Y, Z.dofs, prm, tid);
+    for (int i = tid; i < NW * WP * PL; i += NT) S.acc[i] = 0.0;
+    __syncthreads();
+
+    double* mine = S.acc + w * WP * PL;
+    T R[M][M][M] = {};
+    rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, 4>(S.b, mine, X, Y, Z, prm, lx, ly, 2 * w, o.ex0,
+                                                    o.ey0, o.ez0, R);
+#pragma unroll
+    for (int b = 0; b < M; ++b)
+#pragma unroll
+        for (int a = 0; a < M; ++a) {  // the first layer's lowest plane is complete
+            __hip_atomic_fetch_add(mine + (lx + a) + BX * (ly + b), static_cast<double>(R[a][b][0]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+#pragma unroll
+            for (int c = 0; c < P; ++c) R[a][b][c] = R[a][b][c + 1];
+            R[a][b][P] = T(0);
+        }
+    rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, 4>(S.b, mine, X, Y, Z, prm, lx, ly, 2 * w + 1,
+                                                    o.ex0, o.ey0, o.ez0, R);
+#pragma unroll
+    for (int c = 0; c < M; ++c)
+#pragma unroll
+        for (int b = 0; b < M; ++b)
+#pragma unroll
+            for (int a = 0; a < M; ++a)
+                __hip_atomic_fetch_add(mine + (1 + c) * PL + (lx + a) + BX * (ly + b),
+                                       static_cast<double>(R[a][b][c]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __syncthreads();
+    // DOF plane k of the block collects plane k - 2w of the waves w with 0 <= k - 2w <= P+1
+    for (int i = tid; i < NB; i += NT) {
+        const int bxy = i % PL, bz = i / PL;
+        const int gx = o.ex0 + bxy % BX, gy = o.ey0 + bxy / BX, gz = o.ez0 + bz;
+        double sum = 0;
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) {
+            const int pl = bz - 2 * ww;
+            if (pl >= 0 && pl < WP) sum += S.acc[(ww * WP + pl) * PL + bxy];
+        }
+        if (gx < X.dofs && gy < Y.dofs && gz < Z.dofs)
+            unsafeAtomicAdd(rhs + (gx + static_cast<long>(X.dofs) * (gy + static_cast<long>(Y.dofs) * gz)),
+                            static_cast<T>(sum));
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -747,12 +818,22 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
         const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2, nz2 = (ntz - cz + 1) / 2;
         const long tiles = static_cast<long>(nx2) * ny2 * nz2;
         if (tiles == 0) continue;
-        if constexpr (VAR == 3) {
+        if constexpr (VAR == 4 && P <= 2) {
+            // tiles twice as tall (2 TL::Z element layers), TL::Z waves
+            constexpr int TZ2 = 2 * TL::Z;
+            const int ntz2 = (d[2].elements + TZ2 - 1) / TZ2, nz22 = (ntz2 - cz + 1) / 2;
+            const long tiles2 = static_cast<long>(nx2) * ny2 * nz22;
+            if (tiles2 > 0)
+                hipLaunchKernelGGL((k_rhs3_zpair<T, P, TL::X, TL::Y, TZ2, KF, SRC, FM>), dim3(tiles2),
+                                   dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm, c, nx2, ny2);
+        } else if constexpr (VAR == 3) {
             hipLaunchKernelGGL((k_rhs3_pipe<T, P, TL::X, TL::Y, TL::Z, KF, SRC, FM>),
                                dim3((tiles + 1) / 2), dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm,
                                c, nx2, ny2, static_cast<int>(tiles));
         } else {
-            hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, FM, VAR>), dim3(tiles),
+            // variant 4 at p = 3 (its register block would spill) runs as variant 2
+            constexpr int V = VAR == 4 ? 2 : VAR;
+            hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, FM, V>), dim3(tiles),
                                dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm, c, nx2, ny2);
         }
         hipError_t e = hipGetLastError();
@@ -807,6 +888,7 @@ hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
         if (variant == 1) return rhs3_fields<T, P, 1>(up, rhs, d, prm, s);
         if (variant == 2) return rhs3_fields<T, P, 2>(up, rhs, d, prm, s);
         if (variant == 3) return rhs3_fields<T, P, 3>(up, rhs, d, prm, s);
+        if (variant == 4) return rhs3_fields<T, P, 4>(up, rhs, d, prm, s);
         return rhs3_fields<T, P, 0>(up, rhs, d, prm, s);
     }
     if (kf && src) return rhs2_colours<T, P, true, true>(up, rhs, d, prm, s);

@@ -262,10 +262,12 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
  * ADSG_TUNE_RHS_VARIANT selects the 3D element-kernel variant (0: element coefficients held
  * in registers, 1: re-read from LDS, 2: as 1 with the element's contribution added to LDS
  * per x Gauss point, 3: two tiles per workgroup, the second tile's data copied to LDS while
- * the first is integrated, integrated like 1 (like 2 at p = 3)); they compute the same sums
- * (variant 2 adds them in another order; in FP64 variant 3 equals 1, or 2 at p = 3, bit for
- * bit). A new handle starts with the variant measured fastest for its degree and precision
- * (p = 3: 2; FP64 p <= 2: 3 from 2^21 elements, else 1; FP32: 1).
+ * the first is integrated, integrated like 1 (like 2 at p = 3), 4: each wave integrates two
+ * element layers and carries their shared DOF planes in registers (p <= 2; 2 at p = 3));
+ * they compute the same sums (variants 2 and 4 add them in other orders; in FP64 variant 3
+ * equals 1, or 2 at p = 3, bit for bit). A new handle starts with the variant measured
+ * fastest for its degree and precision (p = 3: 2; FP64 p <= 2: 4 from 2^21 elements, else
+ * 1; FP32: 1).
  * ADSG_TUNE_SWEEP_VARIANT selects the line sweep (0: lines cut into segments solved by
  * separate waves and joined through FP64-computed influence coefficients, where the factors
  * are unpivoted and the lines are 2..16 segments long; 1: one lane per whole line).

@@ -108,7 +108,7 @@ def test_rhs_heat_matches_oracle(dim, p, ne):
     assert po.rel_l2(got, expect) < 1e-13
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("dim,p,ne", [(2, 2, (20, 17)), (3, 2, (9, 8, 10)), (2, 3, (11, 12)), (3, 1, (8, 9, 7)),
                                       (3, 3, (9, 7, 8))])
 def test_rhs_general_forms_match_oracle(dim, p, ne, variant):
@@ -247,15 +247,18 @@ def test_config2_full_size_one_step_and_conservation():
     g.set(ag.U, u0)
     g.step(6, form)
     assert np.array_equal(g.get(ag.U), x6)
-    # the pipelined element variant (the one the autotuner picks at this size) gives the same
-    # bits as variant 1 at full size
-    for v in (1, 3):
+    # the pipelined element variant gives the same bits as variant 1 at full size, and the
+    # layer-pair variant (the default at this size) the oracle's step
+    for v in (1, 3, 4):
         g.set_tuning(ag.TUNE_RHS_VARIANT, v)
         g.set(ag.U, u0)
         g.step(1, form)
         if v == 1:
             xv1 = g.get(ag.U)
-    assert np.array_equal(g.get(ag.U), xv1)
+        elif v == 3:
+            assert np.array_equal(g.get(ag.U), xv1)
+        else:
+            assert po.rel_l2(g.get(ag.U), y1) <= FP64_STEP_TOL
     assert po.rel_l2(xv1, y1) <= FP64_STEP_TOL
 
 
@@ -283,7 +286,7 @@ def test_fp32_dirichlet_2d_against_fp64_oracle():
     _log(f"FP32 2D Dirichlet worst per-step rel L2 vs FP64 oracle = {worst:.3e}")
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("p,ne,dt", [(2, (20, 17, 19), 1e-5), (3, (13, 11, 12), 1e-5), (1, (30, 9, 8), 1e-4)])
 def test_fp32_every_element_variant_against_fp64_oracle(p, ne, dt, variant):
     """FP32 increment steps with each element-kernel variant (variant 3 reads tables weighted
@@ -326,7 +329,7 @@ def test_fp32_step_error_against_fp64_oracle():
 
 # ------------------------------------------------------------------ tuning variants
 @pytest.mark.parametrize("rhs_variant,sweep_variant", [(0, 0), (1, 1), (0, 1), (1, 0), (2, 0), (2, 1), (3, 0),
-                                                       (3, 1)])
+                                                       (3, 1), (4, 0), (4, 1)])
 @pytest.mark.parametrize("dim,p,ne", [(3, 2, (20, 17, 19)), (3, 3, (13, 11, 12)), (2, 2, (70, 45)),
                                       (3, 1, (30, 9, 8)), (2, 3, (129, 33))])
 def test_every_kernel_variant_matches_oracle(dim, p, ne, rhs_variant, sweep_variant):
@@ -394,7 +397,7 @@ def test_autotune_keeps_state_and_picks_a_variant():
     u = o.init_gaussians(seed=2, count=2, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
     g.set(ag.U, u)
     v = g.autotune(ag.rhs_form(1e-4))
-    assert v in (0, 1, 2, 3) and g.get_tuning(ag.TUNE_SWEEP_VARIANT) in (0, 1)
+    assert v in (0, 1, 2, 3, 4) and g.get_tuning(ag.TUNE_SWEEP_VARIANT) in (0, 1)
     assert np.array_equal(g.get(ag.U), u)
     g.step(2, ag.rhs_form(1e-4))
     ref = o.solve(o.compute_rhs(u, po.heat_form(1e-4, 3), threads=8))
