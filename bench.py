@@ -317,7 +317,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_rhs3_heat3d_p2_128.md")
 
 
 def launch_traffic(kernel):
-    """HBM bytes per launch of the element kernel `kernel` (k_rhs3 or k_rhs3_pipe) at 128^3
+    """HBM bytes per launch of the element kernel `kernel` (k_rhs3, k_rhs3_pipe, k_rhs3_zpair) at 128^3
     from the committed counter summary (the line `traffic_bytes_per_launch <kernel>: <bytes>`),
     or None when there is none."""
     key = f"traffic_bytes_per_launch {kernel}:"
@@ -398,10 +398,10 @@ def main():
     value = N * args.steps / t64
     ms_step = 1e3 * t64 / args.steps
 
-    # dominant kernel: the element kernel (k_rhs3, or k_rhs3_pipe for variant 3), launched
+    # dominant kernel: the element kernel (k_rhs3; k_rhs3_pipe, k_rhs3_zpair for variants 3, 4), launched
     # once per parity colour (8 launches per step, each over E/8 elements); HIP events on the
     # handle's stream bracket the 8 launches
-    rhs_kernel = "k_rhs3_pipe" if var64 == 3 else "k_rhs3"
+    rhs_kernel = {3: "k_rhs3_pipe", 4: "k_rhs3_zpair"}.get(var64, "k_rhs3")
     launches = 8
     launch_flops = f_elem(3, 2) * E / launches
     launch_ms = ph64["rhs"] / launches
