@@ -16,7 +16,10 @@ namespace {
 // VAR 0: hold the element's coefficient block in registers (fewer LDS reads); 1: re-read it
 // from LDS for every x Gauss point (fewer registers: two waves per SIMD); 2: as 1, and the
 // element's contribution is added to the LDS planes after every x Gauss point instead of
-// being accumulated in registers (fewer registers again; FP32 runs four waves per SIMD).
+// being accumulated in registers (fewer registers again; FP32 runs four waves per SIMD);
+// 3: two tiles per workgroup, the second copied global-to-LDS while the first is integrated
+// (k_rhs3_pipe); 4: each wave integrates two element layers and carries their shared DOF
+// planes in registers (k_rhs3_zpair, p <= 2).
 //
 // One lane per element; the x/y tables of the tile's elements are staged in LDS, the
 // forward ones as tabulated and the backward ones with the quadrature weight (and, for the
