@@ -70,7 +70,7 @@ EXPORTS = [
     "adsg_batch_set_tuning", "adsg_batch_get_tuning", "adsg_batch_autotune",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
     "adsg_autotune", "adsg_project", "adsg_project_device", "adsg_quad_points", "adsg_error",
-    "adsg_sample",
+    "adsg_sample", "adsg_norm_noalloc",
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
@@ -144,6 +144,7 @@ def lib():
         L.adsg_project_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
         L.adsg_error.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.adsg_norm_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp]
         L.adsg_sample.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, C.c_int, _dp]
         _lib = L
     return _lib
@@ -283,6 +284,16 @@ class Simulation:
         """normL2 (kind 0), H1_0 seminorm (1) or normH1 (2) of a state vector, on the device."""
         out = np.zeros(1)
         self._check(lib().adsg_norm(self.h, which, kind, out.ctypes.data_as(_dp)), "norm")
+        return float(out[0])
+
+    def norm_noalloc(self, kind=0, which=U, sq_dev=None):
+        """adsg_norm_noalloc: norm (sq_dev None) or error against function values held in
+        device memory at address sq_dev (layout of error()); which=None with sq_dev gives
+        the norm of those values. Allocates no device memory."""
+        out = np.zeros(1)
+        w = -1 if which is None else which
+        self._check(lib().adsg_norm_noalloc(self.h, w, kind, sq_dev, out.ctypes.data_as(_dp)),
+                    "norm_noalloc")
         return float(out[0])
 
     def quad_points(self, d):

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -786,6 +787,11 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
         const int p = dc[0]->p;
         h->rhs_variant = p == 3 ? 2 : h->precision == ADSG_FP64 && E >= (1L << 21) ? 4 : 1;
     }
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        g_create_error = "adsg_create: hipGetDevice failed (no HIP device?)";
+        delete h;
+        return ADSG_EHIP;
+    }
     if (shared) {
         h->stream = shared;
         h->owns_stream = false;
@@ -1268,33 +1274,78 @@ int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, 
 
 namespace {
 
-// sqrt of the functional of launch_norm: u = vec[which] (null for which = -1), sq on the device
-int norm_impl(adsg_sim* h, int which, const double* sq, int kind, double* out) {
-    constexpr int kBlocks = 1024;
-    if (!h->norm_scratch &&
-        hipMalloc(reinterpret_cast<void**>(&h->norm_scratch), sizeof(double) * (kBlocks + 1)) != hipSuccess)
-        return fail(h, ADSG_ENOMEM, "norm: no memory for partial sums");
+// block partial sums of launch_norm: one per block of 256 lanes, at most kNormBlocks blocks
+// (the kernel strides over the elements, so every mesh fits), then the total
+constexpr int kNormBlocks = 1024;
+
+int norm_blocks(const adsg_sim* h) {
     long E = 1;
     for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
-    const int blocks = static_cast<int>(std::min<long>(kBlocks, (E + 255) / 256));
+    return static_cast<int>(std::min<long>(kNormBlocks, (E + 255) / 256));
+}
+
+// sqrt of the functional of launch_norm: u = vec[which] (null for which = -1), sq on the
+// device; partial holds `blocks` block sums and the total after them (device memory)
+int norm_impl(adsg_sim* h, int which, const double* sq, int kind, double* out, double* partial, int blocks) {
     const double* u = which >= 0 ? static_cast<const double*>(h->vec[which]) : nullptr;
     hipError_t e;
     if (h->precision == ADSG_FP32) {
         adsg::dev_tables<float> t[3];
         dir_tables(h, t);
-        e = adsg::launch_norm<float>(h->dim, h->dirs[0].tab.p, u, sq, t, kind, h->norm_scratch, blocks,
-                                     h->norm_scratch + kBlocks, h->stream);
+        e = adsg::launch_norm<float>(h->dim, h->dirs[0].tab.p, u, sq, t, kind, partial, blocks,
+                                     partial + blocks, h->stream);
     } else {
         adsg::dev_tables<double> t[3];
         dir_tables(h, t);
-        e = adsg::launch_norm<double>(h->dim, h->dirs[0].tab.p, u, sq, t, kind, h->norm_scratch, blocks,
-                                      h->norm_scratch + kBlocks, h->stream);
+        e = adsg::launch_norm<double>(h->dim, h->dirs[0].tab.p, u, sq, t, kind, partial, blocks,
+                                      partial + blocks, h->stream);
     }
     if (e != hipSuccess) return hip_fail(h, e, "norm");
     double total = 0;
-    HIP_TRY(h, hipMemcpyAsync(&total, h->norm_scratch + kBlocks, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(&total, partial + blocks, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     *out = std::sqrt(total);
+    return ADSG_OK;
+}
+
+// the same with the handle's own partial-sum buffer (allocated on first use)
+int norm_impl_own(adsg_sim* h, int which, const double* sq, int kind, double* out) {
+    if (!h->norm_scratch &&
+        hipMalloc(reinterpret_cast<void**>(&h->norm_scratch), sizeof(double) * (kNormBlocks + 1)) != hipSuccess)
+        return fail(h, ADSG_ENOMEM, "norm: no memory for partial sums");
+    return norm_impl(h, which, sq, kind, out, h->norm_scratch, norm_blocks(h));
+}
+
+}  // namespace
+
+// Partial sums of adsg_norm_noalloc: one device array per GPU, held until the process ends.
+// Host code never uses this name as an address (on the host it is only a stand-in for the
+// symbol); it takes the device address from scratch_ptr, which resolves it per device.
+static __device__ __align__(256) double scratch[kNormBlocks + 1];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_dev;  // device ordinal -> address of that device's scratch
+
+// the address of `scratch` on the handle's device, resolved once per device with that
+// device current (each GPU holds its own copy of the array at its own address)
+int scratch_ptr(adsg_sim* h, double** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_dev.find(h->device);
+    if (it != g_scratch_dev.end()) {
+        *out = it->second;
+        return ADSG_OK;
+    }
+    int cur = 0;
+    HIP_TRY(h, hipGetDevice(&cur));
+    if (cur != h->device) HIP_TRY(h, hipSetDevice(h->device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != h->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) return hip_fail(h, e != hipSuccess ? e : hipErrorInvalidSymbol, "norm_noalloc: scratch address");
+    g_scratch_dev[h->device] = static_cast<double*>(p);
+    *out = static_cast<double*>(p);
     return ADSG_OK;
 }
 
@@ -1303,7 +1354,7 @@ int norm_impl(adsg_sim* h, int which, const double* sq, int kind, double* out) {
 int adsg_norm(adsg_sim* h, int which, int kind, double* out) {
     if (!h || !out || (which != ADSG_U && which != ADSG_U_PREV) || kind < 0 || kind > 2)
         return fail(h, ADSG_EINVAL, "norm: bad argument (U or U_PREV; kind 0, 1 or 2)");
-    return norm_impl(h, which, nullptr, kind, out);
+    return norm_impl_own(h, which, nullptr, kind, out);
 }
 
 int adsg_error(adsg_sim* h, int which, int kind, const double* sq, double* out) {
@@ -1316,10 +1367,25 @@ int adsg_error(adsg_sim* h, int which, int kind, const double* sq, double* out) 
     int rc = ADSG_OK;
     hipError_t e = hipMemcpyAsync(d, sq, bytes, hipMemcpyHostToDevice, h->stream);
     if (e != hipSuccess) rc = hip_fail(h, e, "error: upload");
-    if (rc == ADSG_OK) rc = norm_impl(h, which, static_cast<const double*>(d), kind, out);
+    if (rc == ADSG_OK) rc = norm_impl_own(h, which, static_cast<const double*>(d), kind, out);
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(d);
     return rc;
+}
+
+int adsg_norm_noalloc(adsg_sim* h, int which, int kind, const double* sq_dev, double* out) {
+    if (!h || !out || kind < 0 || kind > 2 || (which != ADSG_U && which != ADSG_U_PREV && which != -1) ||
+        (which == -1 && !sq_dev))
+        return fail(h, ADSG_EINVAL, "norm_noalloc: bad argument (U, U_PREV, or -1 with sq; kind 0, 1 or 2)");
+    if (which >= 0 && !h->vec[which]) return fail(h, ADSG_ESTATE, "norm_noalloc: vector not allocated");
+    const int blocks = norm_blocks(h);
+    if (blocks < 1 || blocks > kNormBlocks)
+        return fail(h, ADSG_EINVAL, "norm_noalloc: the reduction needs " + std::to_string(blocks) +
+                                        " partial sums; the static scratch holds " + std::to_string(kNormBlocks));
+    double* partial = nullptr;
+    int rc = scratch_ptr(h, &partial);
+    if (rc) return rc;
+    return norm_impl(h, which, sq_dev, kind, out, partial, blocks);
 }
 
 int adsg_project(adsg_sim* h, int which, const double* fq) {

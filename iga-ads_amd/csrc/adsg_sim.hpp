@@ -61,6 +61,7 @@ struct adsg_sim {
     long N = 0;
     hipStream_t stream = nullptr;
     bool owns_stream = true;
+    int device = 0;  // the HIP device current at creation, which holds every buffer
     // u and u_prev are FP64 in both precisions; buffer, kfield and source are in the step
     // precision. In FP32 mode shadow[0]/shadow[1] are FP32 copies of u/u_prev (they swap
     // with them) and a step is computed as an increment: compute_rhs assembles

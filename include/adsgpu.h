@@ -164,6 +164,16 @@ int adsg_norm(adsg_sim* h, int which, int kind, double* out);
  * [value] for ADSG_NORM_L2, [value | d/dx | d/dy (| d/dz)] otherwise; host memory. The
  * reference's relative errors (error_relative_L2/H1, :338-356) are the ratio of the two. */
 int adsg_error(adsg_sim* h, int which, int kind, const double* sq, double* out);
+/* adsg_norm (sq_dev = NULL) and adsg_error with the function values already in device memory
+ * (sq_dev, same layout), allocating nothing: the block partial sums live in a static device
+ * array of 1025 doubles per GPU (up to 1024 partials and the total; the kernel strides over
+ * the elements, so every mesh fits and the same partials as adsg_norm / adsg_error are
+ * formed, giving the same bits). No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). All calls on one GPU share that array: calls that overlap on one
+ * GPU (handles on different streams used concurrently) race. Returns the host result after
+ * synchronising the handle's stream. */
+int adsg_norm_noalloc(adsg_sim* h, int which, int kind, const double* sq_dev, double* out);
 
 /* projection(v, f) (simulation_2d.hpp:43-46 / simulation_3d.hpp:45-48 -> projector,
  * projection.hpp:11-153): adds the load vector of the L2 projection of f,
