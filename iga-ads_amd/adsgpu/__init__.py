@@ -17,6 +17,7 @@ FP64, FP32 = 0, 1
 U, U_PREV, BUFFER, KFIELD, SOURCE, GRAD_K, GRAD_F = 0, 1, 2, 3, 4, 5, 6
 STEP_DIRICHLET = 1
 STEP_REFINE = 2
+STEP_CHECK_FINITE = 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -70,7 +71,7 @@ EXPORTS = [
     "adsg_batch_set_tuning", "adsg_batch_get_tuning", "adsg_batch_autotune",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
     "adsg_autotune", "adsg_project", "adsg_project_device", "adsg_quad_points", "adsg_error",
-    "adsg_sample", "adsg_norm_noalloc",
+    "adsg_sample", "adsg_norm_noalloc", "adsg_check_finite",
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
@@ -144,6 +145,7 @@ def lib():
         L.adsg_project_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.adsg_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
         L.adsg_error.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.adsg_check_finite.argtypes = [C.c_void_p, C.c_int]
         L.adsg_norm_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp]
         L.adsg_sample.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, C.c_int, _dp]
         _lib = L
@@ -262,6 +264,11 @@ class Simulation:
 
     def step(self, n, form, flags=0):
         self._check(lib().adsg_step(self.h, n, C.byref(form), flags), "step")
+
+    def check_finite(self, which=U):
+        """adsg_check_finite: the return code (OK or ENONFINITE) and the library's message."""
+        rc = lib().adsg_check_finite(self.h, which)
+        return rc, (lib().adsg_last_error(self.h).decode() if rc != OK else "")
 
     def step_delta(self, n, form, slots, flags=0):
         """n steps of u := u_prev + A_slots^-1 r(u_prev), r the integrand of form."""
@@ -405,12 +412,39 @@ class Simulation:
         return lib().adsg_device_ptr(self.h, which)
 
 
+def hip_runtimes(maps_text=None):
+    """Paths of the distinct HIP runtime libraries (libamdhip64) mapped into this process,
+    read from /proc/self/maps (or from `maps_text`, the same format)."""
+    if maps_text is None:
+        with open("/proc/self/maps") as fh:
+            maps_text = fh.read()
+    paths = set()
+    for line in maps_text.splitlines():
+        parts = line.split(None, 5)
+        if len(parts) == 6 and "libamdhip64" in os.path.basename(parts[5].strip()):
+            paths.add(os.path.realpath(parts[5].strip()))
+    return paths
+
+
+def check_single_hip_runtime(maps_text=None):
+    """Raise AdsgError when more than one HIP runtime is mapped. torch ships its own
+    libamdhip64; when libadsgpu.so is loaded before torch, torch then loads its copy beside
+    the one libadsgpu.so uses, and device pointers of one runtime must not reach the other.
+    Importing torch before the first use of this module gives one runtime, torch's."""
+    paths = hip_runtimes(maps_text)
+    if len(paths) > 1:
+        raise AdsgError("two HIP runtimes are loaded in this process (" + ", ".join(sorted(paths)) +
+                        "): import torch before adsgpu loads libadsgpu.so, so that both use "
+                        "torch's runtime")
+
+
 def heat_k(sim, k, u0, steps, every, dt, dirichlet=True):
     """u(T) of `steps` explicit heat steps with diffusion field k, as a differentiable torch
     operation (libtorch autograd Function; backward = hand-written adjoint, checkpoints
     every `every` steps). k and u0: float64 GPU tensors of sim.size entries."""
     from . import adsg_torch  # needs torch; built beside libadsgpu.so
 
+    check_single_hip_runtime()
     return adsg_torch.heat_k(k, u0, sim.h.value, steps, every, dt, dirichlet)
 
 
@@ -419,6 +453,7 @@ def heat_kf(sim, k, f, u0, steps, every, dt, dirichlet=True):
     differentiable in k, f and u0."""
     from . import adsg_torch
 
+    check_single_hip_runtime()
     return adsg_torch.heat_kf(k, f, u0, sim.h.value, steps, every, dt, dirichlet)
 
 

@@ -873,6 +873,26 @@ bool projectable(const adsg_sim* h, int which) {
     return which >= ADSG_U && which <= ADSG_GRAD_F && elt_of(h, which) == sizeof(double);
 }
 
+// NaN / Inf count of an FP64 vector of the handle (u and u_prev are FP64 in both
+// precisions); ADSG_ENONFINITE when there is any, with the count in the message
+int check_finite_impl(adsg_sim* h, int which, const char* where) {
+    if (!h->vec[which]) return fail(h, ADSG_ESTATE, std::string(where) + ": vector not allocated");
+    if (elt_of(h, which) != sizeof(double))
+        return fail(h, ADSG_EINVAL, std::string(where) + ": FP64 vectors only (u, u_prev, gradients; any vector of FP64 handles)");
+    if (!h->nonfinite && hipMalloc(reinterpret_cast<void**>(&h->nonfinite), sizeof(unsigned)) != hipSuccess)
+        return fail(h, ADSG_ENOMEM, std::string(where) + ": no memory for the count");
+    unsigned bad = 0;
+    HIP_TRY(h, hipMemsetAsync(h->nonfinite, 0, sizeof(unsigned), h->stream));  // eager, never captured
+    hipError_t e = adsg::launch_count_nonfinite(static_cast<const double*>(h->vec[which]), h->N, h->nonfinite, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, where);
+    HIP_TRY(h, hipMemcpyAsync(&bad, h->nonfinite, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (bad)
+        return fail(h, ADSG_ENONFINITE, std::string(where) + ": " + std::to_string(bad) + " of " +
+                                            std::to_string(h->N) + " coefficients are NaN or Inf");
+    return ADSG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -895,6 +915,7 @@ void adsg_destroy(adsg_sim* h) {
     for (auto& v : h->shadow)
         if (v) (void)hipFree(v);
     if (h->norm_scratch) (void)hipFree(h->norm_scratch);
+    if (h->nonfinite) (void)hipFree(h->nonfinite);
     if (h->source_load) (void)hipFree(h->source_load);
     if (h->ckpt) (void)hipFree(h->ckpt);
     if (h->ck_seg) (void)hipFree(h->ck_seg);
@@ -1163,7 +1184,10 @@ int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags) {
     if (!h || !form || nsteps < 0) return fail(h, ADSG_EINVAL, "step: bad argument");
     if (!h->prepared) return fail(h, ADSG_ESTATE, "step: prepare_matrices first");
     if (nsteps == 0) return ADSG_OK;
-    return run_graphs(h, EXPLICIT_STEP, nsteps, form, flags, nullptr);
+    // the check runs after the replayed steps, so the captured graphs do not depend on it
+    const int rc = run_graphs(h, EXPLICIT_STEP, nsteps, form, flags & ~ADSG_STEP_CHECK_FINITE, nullptr);
+    if (rc || !(flags & ADSG_STEP_CHECK_FINITE)) return rc;
+    return check_finite_impl(h, ADSG_U, "step");
 }
 
 int adsg_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots, int flags) {
@@ -1172,7 +1196,14 @@ int adsg_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const in
     if ((flags & ADSG_STEP_REFINE) && h->precision != ADSG_FP32)
         return fail(h, ADSG_EINVAL, "step_delta: refinement applies to FP32 handles");
     if (nsteps == 0) return ADSG_OK;
-    return run_graphs(h, DELTA_STEP, nsteps, form, flags, slots);
+    const int rc = run_graphs(h, DELTA_STEP, nsteps, form, flags & ~ADSG_STEP_CHECK_FINITE, slots);
+    if (rc || !(flags & ADSG_STEP_CHECK_FINITE)) return rc;
+    return check_finite_impl(h, ADSG_U, "step_delta");
+}
+
+int adsg_check_finite(adsg_sim* h, int which) {
+    if (!h || which < ADSG_U || which > ADSG_GRAD_F) return fail(h, ADSG_EINVAL, "check_finite: bad argument");
+    return check_finite_impl(h, which, "check_finite");
 }
 
 int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {

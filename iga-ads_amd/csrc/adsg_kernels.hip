@@ -122,6 +122,17 @@ __global__ void k_boundary_residual(float* __restrict__ d, const float* __restri
     }
 }
 
+// number of NaN / Inf entries of v, added into *count (one atomic add per wave with any)
+__global__ __launch_bounds__(256) void k_count_nonfinite(const double* __restrict__ v, long n,
+                                                         unsigned* __restrict__ count) {
+    unsigned bad = 0;
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x)
+        bad += !isfinite(v[i]);
+    for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor(bad, off);
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
+}
+
 inline unsigned grid_for(long n) {
     long b = (n + 255) / 256;
     return static_cast<unsigned>(b < 4096 ? (b > 0 ? b : 1) : 4096);
@@ -154,6 +165,11 @@ template <typename T>
 hipError_t launch_zero_boundary(int dim, T* u, int nx, int ny, int nz, hipStream_t s) {
     const long n = static_cast<long>(nx) * ny * nz;
     hipLaunchKernelGGL((k_zero_boundary<T>), dim3(grid_for(n)), dim3(256), 0, s, u, dim, nx, ny, nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_nonfinite(const double* v, long n, unsigned* count, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_nonfinite, dim3(grid_for(n)), dim3(256), 0, s, v, n, count);
     return hipGetLastError();
 }
 

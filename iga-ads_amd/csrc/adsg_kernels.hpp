@@ -197,6 +197,9 @@ hipError_t launch_axpy(T* out, const T* x, long n, T a, hipStream_t s);
 template <typename T>
 hipError_t launch_zero_boundary(int dim, T* u, int nx, int ny, int nz, hipStream_t s);
 
+// *count += number of NaN / Inf entries among v[0..n)
+hipError_t launch_count_nonfinite(const double* v, long n, unsigned* count, hipStream_t s);
+
 hipError_t launch_convert_d2f(const double* in, float* out, long n, hipStream_t s);
 hipError_t launch_convert_f2d(const float* in, double* out, long n, hipStream_t s);
 

@@ -78,6 +78,7 @@ struct adsg_sim {
     std::string err;
     hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
     double* norm_scratch = nullptr;  // adsg_norm: block partial sums and the total
+    unsigned* nonfinite = nullptr;   // adsg_check_finite: count of NaN / Inf entries
     // load vector of the source, b_i = sum_e sum_q w J f B_i (step precision): forms with a
     // source start their rhs from dt * b; recomputed after the source changes
     void* source_load = nullptr;

@@ -11,7 +11,12 @@
 #include <torch/extension.h>
 
 #include <hip/hip_runtime_api.h>
+#include <link.h>
 
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -41,6 +46,33 @@ adsg_rhs_form make_form(double dt, bool source = false) {
     f.use_kfield = 1;
     f.use_source = source ? 1 : 0;
     return f;
+}
+
+// torch ships its own libamdhip64; when libadsgpu.so is loaded first, torch later maps its
+// copy beside the one the library uses, and device pointers must not cross between two
+// runtimes. Refuse to run in that case (loading torch first gives one runtime, torch's).
+int collect_hip_runtime(dl_phdr_info* info, size_t, void* data) {
+    const char* name = info->dlpi_name;
+    if (name && *name) {
+        const char* base = std::strrchr(name, '/');
+        base = base ? base + 1 : name;
+        if (std::strstr(base, "libamdhip64")) {
+            char real[PATH_MAX];
+            static_cast<std::set<std::string>*>(data)->insert(realpath(name, real) ? real : name);
+        }
+    }
+    return 0;
+}
+
+void check_single_hip_runtime() {
+    std::set<std::string> paths;
+    dl_iterate_phdr(collect_hip_runtime, &paths);
+    if (paths.size() > 1) {
+        std::string list;
+        for (const auto& p : paths) list += (list.empty() ? "" : ", ") + p;
+        throw std::runtime_error("two HIP runtimes are loaded in this process (" + list +
+                                 "): load torch before libadsgpu.so so that both use torch's runtime");
+    }
 }
 
 void check_vector(const Tensor& t, long n, const char* name) {
@@ -161,11 +193,13 @@ struct HeatKF : public torch::autograd::Function<HeatKF> {
 
 Tensor heat_k(Tensor k, Tensor u0, int64_t handle, int64_t steps, int64_t every, double dt,
               bool dirichlet) {
+    check_single_hip_runtime();
     return HeatK::apply(k, u0, handle, steps, every, dt, dirichlet);
 }
 
 Tensor heat_kf(Tensor k, Tensor f, Tensor u0, int64_t handle, int64_t steps, int64_t every,
                double dt, bool dirichlet) {
+    check_single_hip_runtime();
     return HeatKF::apply(k, f, u0, handle, steps, every, dt, dirichlet);
 }
 

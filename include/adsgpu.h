@@ -209,6 +209,13 @@ int adsg_ads_solve_transposed(adsg_sim* h, const int* slots);
  * validation.hpp:73-81: swap; compute_rhs(form); [zero_boundary]; ads_solve(slot 0).
  * Replayed from a HIP graph captured once per (form, flags). */
 #define ADSG_STEP_DIRICHLET 1
+/* ADSG_STEP_CHECK_FINITE (adsg_step, adsg_step_delta): after the steps, count the NaN / Inf
+ * coefficients of u and return ADSG_ENONFINITE if there are any (the message gives the
+ * count). A NaN or Inf reaches every coefficient its support touches within a step and no
+ * step removes it, so one check after nsteps steps finds it; the reference ignores such
+ * states (and LAPACK's info, band_solve.hpp:16-31). The check reads u once (about 3 us for
+ * 2.2 M coefficients) and synchronises the handle's stream. */
+#define ADSG_STEP_CHECK_FINITE 4
 int adsg_step(adsg_sim* h, int nsteps, const adsg_rhs_form* form, int flags);
 
 /* nsteps of the linearly implicit step in increment (delta) form:
@@ -258,6 +265,11 @@ int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, 
                             int flags);
 
 int adsg_synchronize(adsg_sim* h);
+
+/* The same NaN / Inf count for any FP64 vector of the handle (ADSG_U .. ADSG_GRAD_F; u and
+ * u_prev in both precisions): ADSG_OK, or ADSG_ENONFINITE with the count in the message.
+ * No counterpart in the reference. Synchronises the handle's stream. */
+int adsg_check_finite(adsg_sim* h, int which);
 
 /* Device time per phase of nsteps eagerly launched steps -- adsg_step's sequence when
  * slots is NULL, adsg_step_delta's with these slots otherwise -- measured with HIP events on
