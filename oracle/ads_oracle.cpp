@@ -779,6 +779,21 @@ void orc_sim_compute_rhs(const orc_sim* s, const orc_rhs_form* f, const double* 
     }
 }
 
+// the sequential element loop of orc_sim_compute_rhs over elements [first, first + count)
+// of the reference's element order, added into rhs (not zeroed): a bounded sample of a full
+// assembly for timing on one core
+void orc_sim_compute_rhs_sample(const orc_sim* s, const orc_rhs_form* f, const double* u_prev,
+                                double* rhs, long first, long count) {
+    const long E = element_count(s);
+    std::vector<double> U(64);
+    int e[3];
+    for (long l = std::max(0L, first); l < std::min(E, first + count); ++l) {
+        element_index(s, l, e);
+        element_rhs(s, f, u_prev, e, U.data(), nullptr);
+        scatter(s, e, U.data(), rhs);
+    }
+}
+
 void orc_sim_compute_rhs_threaded(const orc_sim* s, const orc_rhs_form* f, const double* u_prev,
                                   double* rhs, int threads) {
     for (long i = 0; i < s->size; ++i) rhs[i] = 0;

@@ -119,6 +119,10 @@ double orc_sim_norm_sq(const orc_sim* s, const double* u, int kind);
  * (executor/sequential.hpp:12-26, executor/galois.hpp:27-44) with std::thread workers ---- */
 void orc_sim_compute_rhs_threaded(const orc_sim* s, const orc_rhs_form* f, const double* u_prev,
                                   double* rhs, int threads);
+/* the sequential loop over elements [first, first + count) only, added into rhs: a bounded
+ * sample of one assembly for single-core timing */
+void orc_sim_compute_rhs_sample(const orc_sim* s, const orc_rhs_form* f, const double* u_prev,
+                                double* rhs, long first, long count);
 
 /* ---- initial states of the benchmark problems ---- */
 /* d/dk_b of w^T rhs(u; k) for the gradient term of form (-dt k grad u . grad v), added

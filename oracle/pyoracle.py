@@ -54,6 +54,7 @@ def lib():
         L.orc_sim_prepare_matrices.argtypes = [C.c_void_p]
         L.orc_sim_compute_rhs.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp]
         L.orc_sim_compute_rhs_threaded.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_int]
+        L.orc_sim_compute_rhs_sample.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_long, C.c_long]
         L.orc_sim_zero_boundary.argtypes = [C.c_void_p, _dp]
         L.orc_sim_solve.argtypes = [C.c_void_p, _dp, _ip]
         L.orc_sim_solve_transposed.argtypes = [C.c_void_p, _dp, _ip]
@@ -210,6 +211,15 @@ class Sim:
             lib().orc_sim_compute_rhs_threaded(self.h, C.byref(form), _d(u_prev), _d(rhs), threads)
         else:
             lib().orc_sim_compute_rhs(self.h, C.byref(form), _d(u_prev), _d(rhs))
+        return rhs
+
+    def compute_rhs_sample(self, u_prev, form, first, count, rhs=None):
+        """The sequential element loop over elements [first, first + count) only, added into
+        rhs (a bounded sample of one assembly, for single-core timing)."""
+        u_prev = np.ascontiguousarray(u_prev, dtype=np.float64)
+        if rhs is None:
+            rhs = self.zeros()
+        lib().orc_sim_compute_rhs_sample(self.h, C.byref(form), _d(u_prev), _d(rhs), first, count)
         return rhs
 
     def kgrad(self, u, w, form):

@@ -247,3 +247,20 @@ def test_norms_equal_quadratic_forms_of_the_1d_matrices(dim, p, ne):
     assert abs(h10 - u @ Kt @ u) <= 1e-12 * abs(h10)
     assert abs(h1 - (l2 + h10)) <= 1e-12 * h1
     assert abs(l2 - o.l2norm_sq(u)) <= 1e-13 * l2
+
+
+def test_sample_loop_partitions_the_sequential_loop():
+    """compute_rhs_sample over consecutive element ranges replays the sequential loop's
+    additions in the same order, so the pieces give the full assembly bit for bit (the
+    single-core CPU baseline times such a sample)."""
+    sim = po.Sim(3, [2, 2, 2], [6, 5, 4])
+    sim.prepare_matrices()
+    u = np.random.default_rng(11).standard_normal(sim.size)
+    form = po.heat_form(1e-3, 3)
+    full = sim.compute_rhs(u, form)
+    E = 6 * 5 * 4
+    rhs = sim.zeros()
+    for first in range(0, E, 37):
+        sim.compute_rhs_sample(u, form, first, 37, rhs)
+    assert np.array_equal(rhs, full)
+    assert np.array_equal(sim.compute_rhs_sample(u, form, 0, E), full)
