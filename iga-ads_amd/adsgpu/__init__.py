@@ -77,6 +77,7 @@ TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
 TUNE_RHS_KRONECKER = 3
 TUNE_BATCH_TILE = 2
+TUNE_RHS_COLUMNS = 4
 
 
 def lib():

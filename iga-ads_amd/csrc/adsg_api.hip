@@ -52,6 +52,9 @@ void mark(adsg_sim* h, int i) {
     if (h->marks) (void)hipEventRecord(h->marks[i], h->stream);
 }
 
+// element-kernel choice passed to launch_rhs: the variant, and the column shape of variant 5
+int rhs_code(const adsg_sim* h) { return h->rhs_variant | (h->rhs_col_shape << 4); }
+
 template <typename T>
 int upload(adsg_sim* h, void** dst, const std::vector<double>& src) {
     if (*dst) {
@@ -269,7 +272,7 @@ int source_load_impl(adsg_sim* h) {
     T* b = static_cast<T*>(h->source_load);
     hipError_t e = adsg::launch_fill<T>(b, h->N, T(0), h->stream);
     if (e == hipSuccess)
-        e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, prm.source, b, t, prm, h->rhs_variant, h->stream);
+        e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, prm.source, b, t, prm, rhs_code(h), h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "source load vector");
     h->source_load_ok = true;
     return ADSG_OK;
@@ -334,7 +337,7 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
         e = adsg::launch_rhs_kron<T>(h->dim, h->dirs[0].tab.p, up, rhs, kd, mass, form->dt, form->grad,
                                      form->use_source, h->stream);
     } else {
-        e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, h->rhs_variant, h->stream);
+        e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, rhs_code(h), h->stream);
     }
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     mark(h, 2);
@@ -486,7 +489,7 @@ int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, d
     }
     if ((e = adsg::launch_fill<double>(out, h->N, 0.0, h->stream)) != hipSuccess) return hip_fail(h, e, "zero");
     mark(h, 1);
-    e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, h->rhs_variant, h->stream);
+    e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, rhs_code(h), h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     mark(h, 2);
     return ADSG_OK;
@@ -513,7 +516,7 @@ int gram_apply(adsg_sim* h, const double* v, double* out) {
         prm.dt = 0.0;
         e = adsg::launch_fill<double>(out, h->N, 0.0, h->stream);
         if (e == hipSuccess)
-            e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, h->rhs_variant, h->stream);
+            e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, rhs_code(h), h->stream);
     }
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "Gram product");
 }
@@ -778,14 +781,16 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
         h->dirs[d].slots.push_back(std::move(s0));
     }
     // default element-kernel variant (adsg_autotune refines it): the one measured fastest for
-    // the degree and precision. p = 3: 2 (variants 0 and 1 need more registers than allow
-    // two waves per SIMD); FP64 p <= 2: 4 when the mesh has enough tiles of 512 elements for
-    // two workgroups per CU in every colour launch (2^21 elements), else 1; FP32: 1.
+    // the degree and precision. p = 3: FP64 2 (variants 0 and 1 need more registers than allow
+    // two waves per SIMD), FP32 5 (element columns on the matrix cores); FP64 p <= 2: 4 when
+    // the mesh has enough tiles of 512 elements for two workgroups per CU in every colour
+    // launch (2^21 elements), else 1; FP32 p <= 2: 1.
     {
         long E = 1;
         for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
         const int p = dc[0]->p;
-        h->rhs_variant = p == 3 ? 2 : h->precision == ADSG_FP64 && E >= (1L << 21) ? 4 : 1;
+        h->rhs_variant = p == 3 ? (h->precision == ADSG_FP32 ? 5 : 2) : h->precision == ADSG_FP64 && E >= (1L << 21) ? 4 : 1;
+        h->rhs_col_shape = 1;  // variant 5: two columns per wave, one chunk per workgroup
     }
     if (hipGetDevice(&h->device) != hipSuccess) {
         g_create_error = "adsg_create: hipGetDevice failed (no HIP device?)";
@@ -1564,8 +1569,9 @@ int adsg_set_tuning(adsg_sim* h, int key, int value) {
     int* slot = key == ADSG_TUNE_RHS_VARIANT     ? &h->rhs_variant
                 : key == ADSG_TUNE_SWEEP_VARIANT ? &h->sweep_variant
                 : key == ADSG_TUNE_RHS_KRONECKER ? &h->rhs_kron
+                : key == ADSG_TUNE_RHS_COLUMNS   ? &h->rhs_col_shape
                                                  : nullptr;
-    const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 4 : 1;
+    const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 5 : key == ADSG_TUNE_RHS_COLUMNS ? 3 : 1;
     if (!slot || value < 0 || value > max_value) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
     if (*slot != value) drop_graphs(h);
     *slot = value;
@@ -1577,6 +1583,7 @@ int adsg_get_tuning(const adsg_sim* h, int key) {
     if (key == ADSG_TUNE_RHS_VARIANT) return h->rhs_variant;
     if (key == ADSG_TUNE_SWEEP_VARIANT) return h->sweep_variant;
     if (key == ADSG_TUNE_RHS_KRONECKER) return h->rhs_kron;
+    if (key == ADSG_TUNE_RHS_COLUMNS) return h->rhs_col_shape;
     return ADSG_EINVAL;
 }
 
@@ -1601,11 +1608,14 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
         (void)hipMemcpyAsync(p, w.first, w.second, hipMemcpyDeviceToDevice, h->stream);
         saved.push_back(p);
     }
-    int best = h->rhs_variant, best_sweep = h->sweep_variant;
+    int best = h->rhs_variant, best_shape = h->rhs_col_shape, best_sweep = h->sweep_variant;
     double best_ms = 1e30, best_sweep_ms = 1e30;
     int rc = ADSG_OK;
-    for (int v = 0; v <= 4 && rc == ADSG_OK; ++v) {
+    // variants 0..4, then the column variant (5) in each of its shapes
+    for (int k = 0; k <= 8 && rc == ADSG_OK; ++k) {
+        const int v = k < 5 ? k : 5;
         h->rhs_variant = v;
+        h->rhs_col_shape = k < 5 ? 0 : k - 5;
         h->sweep_variant = v <= 1 ? v : best_sweep;  // both sweeps timed with variants 0, 1
         double ms[7];
         rc = adsg_time_phases(h, 1, form, slots, flags, ms);  // warm-up (code load, caches)
@@ -1614,6 +1624,7 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
         if (!rc && ms[0] < best_ms) {
             best_ms = ms[0];
             best = v;
+            best_shape = h->rhs_col_shape;
         }
         const double sweeps = ms[2] + ms[3] + ms[4];
         if (!rc && v <= 1 && sweeps < best_sweep_ms) {
@@ -1632,6 +1643,7 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
     for (void* q : saved) (void)hipFree(q);
     h->pending = h->pending_dirichlet = false;
     h->rhs_variant = best;
+    h->rhs_col_shape = best_shape;
     h->sweep_variant = best_sweep;
     drop_graphs(h);
     return rc;

@@ -91,10 +91,19 @@ struct rhs_params {
 };
 
 // variant: kernel tuning choice (0 = coefficient block held in registers, 1 = re-read from
-// LDS); both compute the same sums in the same order.
+// LDS, ..., 5 = element columns); bits 4 and up carry the shape of variant 5.
 template <typename T>
 hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
                       const rhs_params<T>& prm, int variant, hipStream_t s);
+
+// Variant 5 (3D, p = 2 or 3, forms without a k field or a source in the kernel): element
+// columns with the x-y contractions on the matrix cores (adsg_rhs_col.hip); shape 0..3
+// selects columns per wave (bit 0: two) and chunks per z-range (bit 1: three).
+template <typename T>
+bool rhs3_col_supported(int p, const rhs_params<T>& prm);
+template <typename T>
+hipError_t launch_rhs3_col(int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs, const rhs_params<T>& prm,
+                           int shape, hipStream_t s);
 
 // Batched 2D element kernels: one item per problem (degree p, no k field, no source);
 // ntx2[c] tiles per tile row and first[c] the first block of the item in colour c's launch.

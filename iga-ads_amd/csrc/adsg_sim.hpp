@@ -73,6 +73,7 @@ struct adsg_sim {
     bool pending_dirichlet = false;  // FP32: boundary rows of the increment were fixed
     bool prepared = false;
     int rhs_variant = 1;    // element-kernel tuning choice (set per degree at creation)
+    int rhs_col_shape = 0;  // variant 5: columns per wave and chunks per z-range (ADSG_TUNE_RHS_COLUMNS)
     int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
     int rhs_kron = 0;       // 1: separable integrands through the 1D matrices (adsg_set_tuning)
     std::string err;

@@ -305,6 +305,10 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
  * products exactly per element, so this is the element loop's sum in another order (equal
  * to rounding) at a few dozen operations per coefficient instead of thousands per element. */
 #define ADSG_TUNE_RHS_KRONECKER 3
+/* ADSG_TUNE_RHS_COLUMNS (0..3): the shape of element-kernel variant 5 (one wave per column
+ * of elements along z, the x-y contractions on the matrix cores): bit 0 two columns per
+ * wave, bit 1 three chunks of 16 DOF planes per workgroup; the autotuner times all four. */
+#define ADSG_TUNE_RHS_COLUMNS 4
 int adsg_set_tuning(adsg_sim* h, int key, int value);
 int adsg_get_tuning(const adsg_sim* h, int key);
 int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags);

@@ -108,7 +108,7 @@ def test_rhs_heat_matches_oracle(dim, p, ne):
     assert po.rel_l2(got, expect) < 1e-13
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("dim,p,ne", [(2, 2, (20, 17)), (3, 2, (9, 8, 10)), (2, 3, (11, 12)), (3, 1, (8, 9, 7)),
                                       (3, 3, (9, 7, 8))])
 def test_rhs_general_forms_match_oracle(dim, p, ne, variant):
@@ -286,7 +286,7 @@ def test_fp32_dirichlet_2d_against_fp64_oracle():
     _log(f"FP32 2D Dirichlet worst per-step rel L2 vs FP64 oracle = {worst:.3e}")
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("p,ne,dt", [(2, (20, 17, 19), 1e-5), (3, (13, 11, 12), 1e-5), (1, (30, 9, 8), 1e-4)])
 def test_fp32_every_element_variant_against_fp64_oracle(p, ne, dt, variant):
     """FP32 increment steps with each element-kernel variant (variant 3 reads tables weighted
@@ -329,7 +329,7 @@ def test_fp32_step_error_against_fp64_oracle():
 
 # ------------------------------------------------------------------ tuning variants
 @pytest.mark.parametrize("rhs_variant,sweep_variant", [(0, 0), (1, 1), (0, 1), (1, 0), (2, 0), (2, 1), (3, 0),
-                                                       (3, 1), (4, 0), (4, 1)])
+                                                       (3, 1), (4, 0), (4, 1), (5, 0), (5, 1)])
 @pytest.mark.parametrize("dim,p,ne", [(3, 2, (20, 17, 19)), (3, 3, (13, 11, 12)), (2, 2, (70, 45)),
                                       (3, 1, (30, 9, 8)), (2, 3, (129, 33))])
 def test_every_kernel_variant_matches_oracle(dim, p, ne, rhs_variant, sweep_variant):
@@ -350,6 +350,43 @@ def test_every_kernel_variant_matches_oracle(dim, p, ne, rhs_variant, sweep_vari
             u = o.solve(rhs)
         g.step(3, ag.rhs_form(1e-4), flags)
         assert po.rel_l2(g.get(ag.U), u) <= FP64_STEP_TOL
+
+
+@pytest.mark.parametrize("p_ne", [(2, (45, 23, 97)), (3, (17, 30, 80)), (2, (4, 5, 43)), (3, (9, 4, 14)),
+                                      (2, (33, 7, 3))])
+@pytest.mark.parametrize("precision", [ag.FP64, ag.FP32])
+def test_column_rhs_variant_matches_oracle(p_ne, precision):
+    """Variant 5 (element columns, x-y contractions on the matrix cores): meshes with partial
+    tiles in x and y, several chunks of 16 planes and several z-ranges per column (the
+    z-range is 3 chunks), and fewer elements than one chunk; every form it takes, against
+    the oracle's element loop (FP64 <= 1e-13; FP32 tables and contractions <= 1e-6)."""
+    p, ne = p_ne
+    _log(f"case: column rhs p={p} ne={ne} precision={precision}")
+    g, o = _pair(3, p, ne, precision=precision)
+    g.set_tuning(ag.TUNE_RHS_VARIANT, 5)
+    rng = np.random.default_rng(17)
+    c = rng.uniform(0, 1, g.size)
+    g.set(ag.U_PREV, c)
+    dt = 2e-3
+    tol = 1e-13 if precision == ag.FP64 else 1e-6
+    for gk in (dict(), dict(grad=(0.3, 1.0, 2.0)), dict(mass=0.0), dict(reaction=1.0),
+               dict(mass=0.5, grad=(0.0, 0.0, 1.0), reaction=0.25)):
+        if precision == ag.FP64:
+            g.compute_rhs(ag.rhs_form(dt, **gk))
+            got = g.get(ag.U)
+            f = po.heat_form(dt, 3, grad=gk.get("grad", (1.0, 1.0, 1.0)), reaction=gk.get("reaction", 0.0))
+            f.mass = gk.get("mass", 1.0)
+            expect = o.compute_rhs(c, f)
+            assert po.rel_l2(got, expect) < tol, gk
+    if precision == ag.FP32:
+        # FP32 increment steps from a smooth state against the FP64 oracle
+        u = o.init_gaussians(seed=31, count=3, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+        g.set(ag.U, u)
+        for _ in range(3):
+            u = o.solve(o.compute_rhs(u, po.heat_form(1e-5, 3), threads=8))
+            g.step(1, ag.rhs_form(1e-5))
+            assert po.rel_l2(g.get(ag.U), u) <= tol
+    g.close()
 
 
 @pytest.mark.parametrize("p,ne,same_as", [(2, (40, 33, 37), 1), (3, (26, 19, 23), 2), (1, (21, 30, 9), 1)])
