@@ -449,8 +449,11 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain D2D copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--eager-only", action="store_true",
-                    help="only run `steps` eager FP64 steps of the 3D p=2 workload (no graph "
+                    help="only run `steps` eager steps of the 3D p=2 workload (no graph "
                          "replay; for counter collection) and exit")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
+                    help="arithmetic of the --eager-only steps")
+    ap.add_argument("--degree", type=int, default=2, help="B-spline degree of the --eager-only steps")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, 4)
@@ -473,9 +476,13 @@ def main():
     E = ne ** 3
 
     if args.eager_only:
-        sim = ag.Simulation(3, [2, 2, 2], [ne, ne, ne], precision=ag.FP64)
+        prec = ag.FP64 if args.precision == "fp64" else ag.FP32
+        pd = [args.degree] * 3
+        sim = ag.Simulation(3, pd, [ne, ne, ne], precision=prec)
         sim.prepare_matrices()
-        sim.set(ag.U, u0)
+        sim.set(ag.U, u0 if args.degree == 2 else
+                wl.gaussians(pd, [ne] * 3, seed=2024, count=8, sigma=(0.05, 0.15), centre=(0.2, 0.8),
+                             noise=1e-3))
         form = ag.rhs_form(dt)
         variant = args.rhs_variant
         if variant is None:
@@ -484,10 +491,11 @@ def main():
             sim.set_tuning(ag.TUNE_RHS_VARIANT, variant)
         phases = sim.time_phases(args.steps, form)
         sim.close()
-        print(json.dumps({"eager_steps": args.steps, "rhs_variant": variant, "phases_ms": phases}),
-              flush=True)
+        print(json.dumps({"eager_steps": args.steps, "precision": args.precision, "degree": args.degree,
+                          "elements": ne, "rhs_variant": variant, "phases_ms": phases}), flush=True)
         return
 
+    rates = ag.device_rates()  # measured FMA ceilings of this device (vector and matrix)
     t64, ph64, x64, var64 = run_form(ag, ag.FP64, ne, args.steps, args.warmup, u0, dt,
                                      args.rhs_variant)
     value = N * args.steps / t64
@@ -542,6 +550,10 @@ def main():
         },
         "phases_ms": ph64,
         "rhs_variant": var64,
+        "device_rates_tflops": dict(rates, note=(
+            "FMA ceilings measured on this device by adsg_device_rates (independent FMA chains "
+            "filling the device): vector FP64 / FP32 (plain FMAs, no packed instructions) and "
+            "16x16x4 MFMA FP64 / FP32")),
         "solve_roofline": {
             "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
             "achieved_xyz": sweep_gbs,

@@ -71,7 +71,7 @@ EXPORTS = [
     "adsg_batch_set_tuning", "adsg_batch_get_tuning", "adsg_batch_autotune",
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
     "adsg_autotune", "adsg_project", "adsg_project_device", "adsg_quad_points", "adsg_error",
-    "adsg_sample", "adsg_norm_noalloc", "adsg_check_finite",
+    "adsg_sample", "adsg_norm_noalloc", "adsg_check_finite", "adsg_device_rates",
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
@@ -147,10 +147,20 @@ def lib():
         L.adsg_quad_points.argtypes = [C.c_void_p, C.c_int, _dp]
         L.adsg_error.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
         L.adsg_check_finite.argtypes = [C.c_void_p, C.c_int]
+        L.adsg_device_rates.argtypes = [_dp]
         L.adsg_norm_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp]
         L.adsg_sample.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, C.c_int, _dp]
         _lib = L
     return _lib
+
+
+def device_rates():
+    """adsg_device_rates: measured FMA ceilings of the current device in TFLOP/s."""
+    out = np.zeros(4)
+    rc = lib().adsg_device_rates(out.ctypes.data_as(_dp))
+    if rc != OK:
+        raise AdsgError(f"device_rates: {rc}")
+    return dict(zip(("valu_fp64", "valu_fp32", "mfma_fp64", "mfma_fp32"), out.tolist()))
 
 
 def linspace(a, b, n):

@@ -885,13 +885,14 @@ hipError_t rhs3_fields(const T* up, T* rhs, const dev_tables<T>* d, const rhs_pa
 
 template <typename T, int P>
 hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
-                      const rhs_params<T>& prm, int variant, hipStream_t s) {
+                      const rhs_params<T>& prm, int code, hipStream_t s) {
     const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
+    int variant = code & 15;  // bits 4 and up: the shape of variant 5
     if (dim == 3) {
-        if ((variant & 15) == 5) {
+        if (variant == 5) {
             // element columns on the matrix cores where the form allows, else the degree's
             // default per-element variant
-            if (rhs3_col_supported<T>(P, prm)) return launch_rhs3_col<T>(P, up, rhs, d, prm, variant >> 4, s);
+            if (rhs3_col_supported<T>(P, prm)) return launch_rhs3_col<T>(P, up, rhs, d, prm, code >> 4, s);
             variant = P == 3 ? 2 : 1;
         }
         if (variant == 1) return rhs3_fields<T, P, 1>(up, rhs, d, prm, s);

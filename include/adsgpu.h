@@ -354,6 +354,12 @@ int adsg_batch_autotune(adsg_batch* b, double* ms_out);
  * buffer as above) in place, pivots 1-based like dgbtrf. Returns LAPACK info. */
 int adsg_band_factorize(int n, int kl, int ku, double* ab, int* ipiv);
 
+/* Measured FMA ceilings of the current HIP device in TFLOP/s, from kernels of independent FMA
+ * chains that fill it: tflops[0] vector FP64, [1] vector FP32 (plain FMAs, no packed
+ * instructions), [2] MFMA FP64 16x16x4, [3] MFMA FP32 16x16x4. No counterpart in the
+ * reference; the benchmark reports them beside the element kernels' rates. */
+int adsg_device_rates(double* tflops);
+
 #ifdef __cplusplus
 }
 #endif

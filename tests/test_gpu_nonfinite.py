@@ -70,3 +70,4 @@ def test_check_finite_rejects_bad_arguments():
     assert ag.lib().adsg_check_finite(sim.h, ag.BUFFER) == ag.EINVAL  # FP32 buffer
     assert ag.lib().adsg_check_finite(sim.h, ag.KFIELD) == ag.ESTATE  # never set
     sim.close()
+

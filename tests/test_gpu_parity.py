@@ -389,6 +389,27 @@ def test_column_rhs_variant_matches_oracle(p_ne, precision):
     g.close()
 
 
+def test_each_variant_runs_its_own_kernel():
+    """The tuning choice reaches the dispatch: variants 0, 1 and 3 add the same products in
+    the same order (bitwise equal), while variants 2, 4 and 5 sum in other orders, so on a
+    random state their rhs differ from variant 1 in the last bits -- which they could not if
+    every choice ran one kernel. The column shape set for variant 5 must not change the
+    kernel the other variants run."""
+    g, _ = _pair(3, 2, (20, 17, 19))
+    g.set(ag.U_PREV, np.random.default_rng(23).standard_normal(g.size))
+    g.set_tuning(ag.TUNE_RHS_COLUMNS, 1)
+    out = {}
+    for v in range(6):
+        g.set_tuning(ag.TUNE_RHS_VARIANT, v)
+        g.compute_rhs(ag.rhs_form(1e-3))
+        out[v] = g.get(ag.U)
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[1], out[3])
+    for v in (2, 4, 5):
+        assert not np.array_equal(out[v], out[1]), v
+        assert po.rel_l2(out[v], out[1]) < 1e-13, v
+    g.close()
+
+
 @pytest.mark.parametrize("p,ne,same_as", [(2, (40, 33, 37), 1), (3, (26, 19, 23), 2), (1, (21, 30, 9), 1)])
 def test_pipelined_rhs_variant_is_bitwise_its_single_tile_variant(p, ne, same_as):
     """Variant 3 (two tiles per workgroup, the second copied to LDS during the first's
