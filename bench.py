@@ -454,6 +454,8 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
                     help="arithmetic of the --eager-only steps")
     ap.add_argument("--degree", type=int, default=2, help="B-spline degree of the --eager-only steps")
+    ap.add_argument("--mass", type=float, default=1.0,
+                    help="mass coefficient of the --eager-only form (0: the mass-free increment form)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, 4)
@@ -483,7 +485,7 @@ def main():
         sim.set(ag.U, u0 if args.degree == 2 else
                 wl.gaussians(pd, [ne] * 3, seed=2024, count=8, sigma=(0.05, 0.15), centre=(0.2, 0.8),
                              noise=1e-3))
-        form = ag.rhs_form(dt)
+        form = ag.rhs_form(dt, mass=args.mass)
         variant = args.rhs_variant
         if variant is None:
             variant = sim.autotune(form)

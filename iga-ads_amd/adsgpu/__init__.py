@@ -172,7 +172,11 @@ def linspace(a, b, n):
 class Simulation:
     """One adsg_sim handle: the simulation_2d / simulation_3d surface on the GPU."""
 
-    def __init__(self, dim, p, elements, dt=0.0, steps=0, precision=FP64, a=None, b=None):
+    def __init__(self, dim, p, elements, dt=0.0, steps=0, precision=FP64, a=None, b=None,
+                 quad_order=None, repeated_nodes=None):
+        """dim_config per direction (simulation/config.hpp:11-29): degree p, elements,
+        interval [a, b], quad_order (Gauss points per element; None or 0 = p + 1) and
+        repeated_nodes (extra copies of every interior knot)."""
         L = lib()
         cfg = Config()
         cfg.dim = dim
@@ -184,8 +188,8 @@ class Simulation:
                 dc.elements = es[i]
                 dc.a = a[i] if a else 0.0
                 dc.b = b[i] if b else 1.0
-                dc.quad_order = 0
-                dc.repeated_nodes = 0
+                dc.quad_order = quad_order[i] if quad_order else 0
+                dc.repeated_nodes = repeated_nodes[i] if repeated_nodes else 0
         cfg.step_count = steps
         cfg.dt = dt
         cfg.derivatives = 1

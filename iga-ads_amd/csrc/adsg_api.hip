@@ -117,6 +117,11 @@ int upload_tables(adsg_sim* h, dev_dir& d) {
     // the 1D matrices in FP64 in both precisions (the separable rhs contracts in FP64)
     if ((rc = upload<double>(h, &d.kM, M))) return rc;
     if ((rc = upload<double>(h, &d.kK, K))) return rc;
+    // first DOF of every element
+    if (hipMalloc(reinterpret_cast<void**>(&d.first), sizeof(int) * std::max(1, d.tab.elements)) != hipSuccess)
+        return fail(h, ADSG_ENOMEM, "hipMalloc failed for tables");
+    if (hipMemcpy(d.first, d.tab.first.data(), sizeof(int) * d.tab.elements, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(h, ADSG_EHIP, "upload of the first DOFs failed");
     return ADSG_OK;
 }
 
@@ -131,7 +136,25 @@ void dir_tables(const adsg_sim* h, adsg::dev_tables<TT>* t) {
         t[d].bder = static_cast<const TT*>(h->dirs[d].bder);
         t[d].elements = h->dirs[d].tab.elements;
         t[d].dofs = h->dirs[d].tab.dofs;
+        t[d].first = h->dirs[d].first;
+        t[d].nq = h->dirs[d].tab.nq;
+        t[d].p = h->dirs[d].tab.p;
     }
+}
+
+// the element loop of the handle: the general kernel for configurations outside the
+// sum-factorised kernels', else the tuned variant
+template <typename T>
+hipError_t launch_elements(const adsg_sim* h, const T* up, T* out, const adsg::dev_tables<T>* t,
+                           const adsg::rhs_params<T>& prm) {
+    if (h->general) return adsg::launch_rhs_general<T>(h->dim, up, out, t, prm, h->stream);
+    return adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, out, t, prm, rhs_code(h), h->stream);
+}
+
+// entry points whose kernels assume one degree and p+1 Gauss points in every direction
+int refuse_general(adsg_sim* h, const char* what) {
+    return fail(h, ADSG_EINVAL, std::string(what) + ": not available for configurations with quad_order != p+1, "
+                                "repeated interior knots or a degree per direction");
 }
 
 void free_slot(dev_slot& s) {
@@ -272,7 +295,7 @@ int source_load_impl(adsg_sim* h) {
     T* b = static_cast<T*>(h->source_load);
     hipError_t e = adsg::launch_fill<T>(b, h->N, T(0), h->stream);
     if (e == hipSuccess)
-        e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, prm.source, b, t, prm, rhs_code(h), h->stream);
+        e = launch_elements<T>(h, prm.source, b, t, prm);
     if (e != hipSuccess) return hip_fail(h, e, "source load vector");
     h->source_load_ok = true;
     return ADSG_OK;
@@ -317,7 +340,7 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
     }
     // separable integrand (no k field, no reaction): the 1D-matrix form, which writes every
     // entry, so only a source needs a start value
-    const bool kron = h->rhs_kron && h->dim == 3 && !form->use_kfield && form->reaction == 0.0;
+    const bool kron = h->rhs_kron && h->dim == 3 && !h->general && !form->use_kfield && form->reaction == 0.0;
     hipError_t e = hipSuccess;
     if (form->use_source)
         e = adsg::launch_scale<T>(rhs, static_cast<const T*>(h->source_load), h->N, prm.dt, h->stream);
@@ -337,7 +360,7 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
         e = adsg::launch_rhs_kron<T>(h->dim, h->dirs[0].tab.p, up, rhs, kd, mass, form->dt, form->grad,
                                      form->use_source, h->stream);
     } else {
-        e = adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, rhs, t, prm, rhs_code(h), h->stream);
+        e = launch_elements<T>(h, up, rhs, t, prm);
     }
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     mark(h, 2);
@@ -361,6 +384,8 @@ int zero_boundary_impl(adsg_sim* h) {
     if (h->precision == ADSG_FP32) {
         // fixed rows: the rhs entry is 0, so the increment entry is 0 - (M_f u_prev)_b
         if (!h->pending) return fail(h, ADSG_ESTATE, "zero_boundary (FP32): call compute_rhs first");
+        if (h->mixed_p)
+            return fail(h, ADSG_EINVAL, "zero_boundary (FP32): the boundary residual needs one degree in every direction");
         for (int d = 0; d < h->dim; ++d)
             if (!h->dirs[d].rows)
                 return fail(h, ADSG_ESTATE, "zero_boundary (FP32): slot 0 must be a factored degree-p band matrix");
@@ -472,7 +497,7 @@ int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, d
         prm.kfield = static_cast<const double*>(h->vec[ADSG_KFIELD]);
     }
     hipError_t e;
-    if (h->rhs_kron && h->dim == 3 && !form->use_kfield) {  // separable: the 1D-matrix form
+    if (h->rhs_kron && h->dim == 3 && !h->general && !form->use_kfield) {  // separable: the 1D-matrix form
         mark(h, 1);
         adsg::kron_dir kd[3];
         for (int d = 0; d < 3; ++d) {
@@ -489,7 +514,7 @@ int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, d
     }
     if ((e = adsg::launch_fill<double>(out, h->N, 0.0, h->stream)) != hipSuccess) return hip_fail(h, e, "zero");
     mark(h, 1);
-    e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, rhs_code(h), h->stream);
+    e = launch_elements<double>(h, v, out, t, prm);
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     mark(h, 2);
     return ADSG_OK;
@@ -499,7 +524,7 @@ int apply_linear_form(adsg_sim* h, const adsg_rhs_form* form, const double* v, d
 // the form {mass 1, dt 0}, or its 1D-matrix form when that path is selected
 int gram_apply(adsg_sim* h, const double* v, double* out) {
     hipError_t e;
-    if (h->rhs_kron && h->dim == 3) {
+    if (h->rhs_kron && h->dim == 3 && !h->general) {
         adsg::kron_dir kd[3];
         for (int d = 0; d < 3; ++d) {
             kd[d].M = static_cast<const double*>(h->dirs[d].kM);
@@ -516,7 +541,7 @@ int gram_apply(adsg_sim* h, const double* v, double* out) {
         prm.dt = 0.0;
         e = adsg::launch_fill<double>(out, h->N, 0.0, h->stream);
         if (e == hipSuccess)
-            e = adsg::launch_rhs<double>(h->dim, h->dirs[0].tab.p, v, out, t, prm, rhs_code(h), h->stream);
+            e = launch_elements<double>(h, v, out, t, prm);
     }
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "Gram product");
 }
@@ -768,11 +793,10 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
             delete h;
             return ADSG_EINVAL;
         }
-        if (dc[d]->p != dc[0]->p) {
-            g_create_error = "adsg_create: the element kernels need the same degree in every direction";
-            delete h;
-            return ADSG_EINVAL;
-        }
+        // configurations outside the sum-factorised kernels' take the general element loop
+        const int nq = dc[d]->quad_order > 0 ? dc[d]->quad_order : dc[d]->p + 1;
+        if (nq != dc[d]->p + 1 || dc[d]->repeated_nodes != 0 || dc[d]->p != dc[0]->p) h->general = true;
+        if (dc[d]->p != dc[0]->p) h->mixed_p = true;
         h->N *= h->dirs[d].tab.dofs;
         dev_slot s0;
         s0.set = true;
@@ -848,6 +872,7 @@ long quad_point_count(const adsg_sim* h) {
 }
 
 int project_impl(adsg_sim* h, int which, const double* fq_dev) {
+    if (h->general) return refuse_general(h, "project");
     if (!h->vec[which]) {
         if (hipMalloc(&h->vec[which], h->N * sizeof(double)) != hipSuccess)
             return fail(h, ADSG_ENOMEM, "project: device allocation failed");
@@ -914,6 +939,7 @@ void adsg_destroy(adsg_sim* h) {
         for (auto& s : d.slots) free_slot(s);
         for (void** p : {&d.val, &d.der, &d.wj, &d.bval, &d.bder, &d.rows, &d.kM, &d.kK})
             if (*p) (void)hipFree(*p);
+        if (d.first) (void)hipFree(d.first);
     }
     for (auto& v : h->vec)
         if (v) (void)hipFree(v);
@@ -1219,6 +1245,7 @@ int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
         return fail(h, ADSG_EINVAL, "adjoint_step: the form must be linear in u (reaction = 0)");
     if (form->use_kfield && !h->vec[ADSG_GRAD_K])
         return fail(h, ADSG_ESTATE, "adjoint_step: set ADSG_GRAD_K (the accumulator) first");
+    if (form->use_kfield && h->general) return refuse_general(h, "adjoint_step (k-field gradient)");
     return one_adjoint_step(h, form, flags);
 }
 
@@ -1226,6 +1253,7 @@ int adsg_forward_checkpointed(adsg_sim* h, int nsteps, int every, const adsg_rhs
     if (!h || !form || nsteps < 1 || every < 1) return fail(h, ADSG_EINVAL, "forward_checkpointed: bad argument");
     if (!h->prepared) return fail(h, ADSG_ESTATE, "forward_checkpointed: prepare_matrices first");
     if (h->precision != ADSG_FP64) return fail(h, ADSG_EINVAL, "forward_checkpointed: FP64 handles only");
+    if (form->use_kfield && h->general) return refuse_general(h, "forward_checkpointed (k-field gradient)");
     const int nchk = (nsteps + every - 1) / every;
     const size_t need = static_cast<size_t>(nchk) * h->N;
     if (h->ckpt_cap < need) {
@@ -1323,6 +1351,7 @@ int norm_blocks(const adsg_sim* h) {
 // sqrt of the functional of launch_norm: u = vec[which] (null for which = -1), sq on the
 // device; partial holds `blocks` block sums and the total after them (device memory)
 int norm_impl(adsg_sim* h, int which, const double* sq, int kind, double* out, double* partial, int blocks) {
+    if (h->general) return refuse_general(h, "norm");
     const double* u = which >= 0 ? static_cast<const double*>(h->vec[which]) : nullptr;
     hipError_t e;
     if (h->precision == ADSG_FP32) {
@@ -1452,6 +1481,7 @@ int adsg_sample(adsg_sim* h, int which, const int* counts, const double* px, con
     const double* pts[3] = {px, py, pz};
     if (!h || !counts || !out || !projectable(h, which) || !h->vec[which])
         return fail(h, ADSG_EINVAL, "sample: bad argument (an FP64 vector that is set)");
+    if (h->mixed_p) return fail(h, ADSG_EINVAL, "sample: needs one degree in every direction");
     long total = 1;
     for (int d = 0; d < h->dim; ++d) {
         if (counts[d] <= 0 || !pts[d]) return fail(h, ADSG_EINVAL, "sample: empty direction");

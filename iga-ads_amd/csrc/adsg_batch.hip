@@ -375,6 +375,13 @@ int adsg_batch_create(const adsg_config* cfgs, int count, adsg_batch** out) {
             adsg_batch_destroy(b);
             return rc;
         }
+        if (h->general) {
+            adsg_destroy(h);
+            g_batch_error = "batch_create: problem " + std::to_string(i) +
+                            ": batches take p+1 Gauss points and simple knots (quad_order 0 or p+1, repeated_nodes 0)";
+            adsg_batch_destroy(b);
+            return ADSG_EINVAL;
+        }
         b->sims.push_back(h);
         b->off.push_back(b->total);
         b->total += h->N;

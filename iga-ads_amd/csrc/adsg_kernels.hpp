@@ -79,6 +79,9 @@ struct dev_tables {
     const T* bder = nullptr;
     int elements = 0;
     int dofs = 0;
+    const int* first = nullptr;  // first DOF of every element (general element loop)
+    int nq = 0;                  // Gauss points per element (general element loop)
+    int p = 0;                   // degree (general element loop)
 };
 
 // Integrand of compute_rhs at a Gauss point:
@@ -95,6 +98,12 @@ struct rhs_params {
 template <typename T>
 hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
                       const rhs_params<T>& prm, int variant, hipStream_t s);
+
+// The element loop for any quadrature order, repeated interior knots and a degree per
+// direction (1..3), one lane per element (adsg_rhs_gen.hip); dirs[d].first, nq and p set.
+template <typename T>
+hipError_t launch_rhs_general(int dim, const T* u_prev, T* rhs, const dev_tables<T>* dirs, const rhs_params<T>& prm,
+                              hipStream_t s);
 
 // Variant 5 (3D, p = 2 or 3, forms without a k field or a source in the kernel): element
 // columns with the x-y contractions on the matrix cores (adsg_rhs_col.hip); shape 0..3

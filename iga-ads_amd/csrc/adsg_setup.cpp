@@ -132,13 +132,13 @@ bool build_direction(direction& d, int p, int elements, double a, double b, int 
         err = "need at least one element per direction";
         return false;
     }
-    if (repeated_nodes != 0) {
-        err = "repeated interior nodes are not supported by the GPU tables";
+    if (repeated_nodes < 0 || repeated_nodes > p) {
+        err = "repeated_nodes must be in 0..p (each interior knot repeated up to p+1 times)";
         return false;
     }
     int nq = quad_order > 0 ? quad_order : p + 1;  // config.hpp:27-28
-    if (nq != p + 1) {
-        err = "the HIP element kernels integrate with p+1 Gauss points per direction";
+    if (nq < 2 || nq > 64) {
+        err = "quad_order must be in 2..64 (the Gauss rules of quad/gauss_data.cpp:8-38)";
         return false;
     }
     d.p = p;

@@ -48,6 +48,7 @@ struct dev_dir {
     // separable-integrand rhs (launch_rhs_kron)
     void* kM = nullptr;
     void* kK = nullptr;
+    int* first = nullptr;  // first DOF of every element (bspline.cpp:162-172)
 };
 
 }  // namespace adsg
@@ -62,6 +63,11 @@ struct adsg_sim {
     hipStream_t stream = nullptr;
     bool owns_stream = true;
     int device = 0;  // the HIP device current at creation, which holds every buffer
+    // a configuration outside the sum-factorised kernels' (quad_order != p+1, repeated
+    // interior knots or a degree per direction): the general element loop assembles the rhs,
+    // and the entry points that assume one degree and p+1 Gauss points refuse the handle
+    bool general = false;
+    bool mixed_p = false;
     // u and u_prev are FP64 in both precisions; buffer, kfield and source are in the step
     // precision. In FP32 mode shadow[0]/shadow[1] are FP32 copies of u/u_prev (they swap
     // with them) and a step is computed as an increment: compute_rhs assembles

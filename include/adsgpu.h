@@ -49,7 +49,14 @@ extern "C" {
 
 typedef struct adsg_sim adsg_sim;
 
-/* ads::dim_config (simulation/config.hpp:11-29). quad_order <= 0 means p + 1. */
+/* ads::dim_config (simulation/config.hpp:11-29). quad_order <= 0 means p + 1; accepted:
+ * p 1..3 per direction, quad_order 2..64 (quad/gauss_data.cpp:8-38), repeated_nodes 0..p
+ * (bspline.cpp:26-43: every interior knot repeated repeated_nodes + 1 times). A handle with
+ * quad_order != p + 1, repeated nodes or a degree per direction assembles its rhs with the
+ * general element loop (one lane per element, the reference's per-point loop); on such a
+ * handle adsg_norm, adsg_error, adsg_norm_noalloc, adsg_project(_device) and the k-field
+ * gradient of the explicit adjoint return ADSG_EINVAL, and so do adsg_sample and FP32
+ * Dirichlet steps when the degree differs between directions. */
 typedef struct adsg_dim_config {
     int p;
     int elements;

@@ -264,3 +264,105 @@ def test_sample_loop_partitions_the_sequential_loop():
         sim.compute_rhs_sample(u, form, first, 37, rhs)
     assert np.array_equal(rhs, full)
     assert np.array_equal(sim.compute_rhs_sample(u, form, 0, E), full)
+
+
+# ------------------------------------------------------------------ general configurations
+# dim_config (simulation/config.hpp:11-29) beyond the defaults: quad_order != p+1, repeated
+# interior knots (bspline.cpp:26-43), a degree per direction.
+
+GENERAL = [
+    # dim, p, elements, quad_order, repeated_nodes
+    (2, (2, 2), (6, 5), (5, 4), (0, 0)),
+    (2, (3, 1), (5, 7), (0, 0), (0, 0)),
+    (3, (1, 2, 3), (4, 3, 5), (0, 0, 0), (0, 0, 0)),
+    (2, (2, 3), (5, 4), (0, 0), (1, 1)),
+    (3, (2, 2, 2), (4, 3, 4), (4, 3, 5), (1, 0, 2)),
+]
+
+
+def _gsim(dim, p, ne, q, r):
+    return po.Sim(dim, list(p), list(ne), quad_order=list(q), repeated_nodes=list(r))
+
+
+@pytest.mark.parametrize("dim,p,ne,q,r", GENERAL)
+def test_general_config_1d_matrices_closed_forms(dim, p, ne, q, r):
+    """Gram and stiffness matrices of each direction: with quad_order >= p+1 the Gauss rule is
+    exact, so they equal the default rule's matrices; column sums of the Gram matrix are the
+    integrals of the B-splines, (t_{j+p+1} - t_j) / (p+1), also with repeated knots; the
+    stiffness rows sum to zero (the derivative of the partition of unity)."""
+    sim = _gsim(dim, p, ne, q, r)
+    ref = po.Sim(dim, list(p), list(ne), repeated_nodes=list(r))
+    Ms, Ks = _mats(sim)
+    Mr, Kr = _mats(ref)
+    for d in range(dim):
+        assert np.abs(Ms[d] - Mr[d]).max() < 1e-15 * max(1.0, np.abs(Mr[d]).max()) * 16
+        assert np.abs(Ks[d] - Kr[d]).max() < 1e-12 * np.abs(Kr[d]).max()
+        k = po.knots(0.0, 1.0, p[d], ne[d], r[d])
+        n = sim.shape[d]
+        integrals = np.array([(k[j + p[d] + 1] - k[j]) / (p[d] + 1) for j in range(n)])
+        assert np.allclose(Ms[d].sum(axis=0), integrals, rtol=0, atol=1e-15)
+        assert np.abs(Ks[d].sum(axis=1)).max() < 1e-11 * np.abs(Ks[d]).max()
+
+
+@pytest.mark.parametrize("dim,p,ne,q,r", GENERAL)
+def test_general_config_heat_rhs_equals_kronecker_form(dim, p, ne, q, r):
+    sim = _gsim(dim, p, ne, q, r)
+    rng = np.random.default_rng(77 + dim)
+    c = rng.standard_normal(sim.size)
+    dt = 1e-3
+    Ms, Ks = _mats(sim)
+    expect = _apply(Ms, c, sim.shape)
+    for d in range(dim):
+        mats = list(Ms)
+        mats[d] = Ks[d]
+        expect = expect - dt * _apply(mats, c, sim.shape)
+    got = sim.compute_rhs(c, po.heat_form(dt, dim))
+    assert po.rel_l2(got, expect) < 2e-14
+
+
+def _general_quadrature_matrices(p, ne, nq, r):
+    """Global basis values / derivatives at the nq Gauss points of every element of one
+    direction (elements between distinct knots), from point evaluation of the basis."""
+    k = po.knots(0.0, 1.0, p, ne, r)
+    n = len(k) - p - 1
+    X, W = po.gauss(nq)
+    pts = np.unique(k)
+    rows_b, rows_d, wts = [], [], []
+    for e in range(ne):
+        a, b = pts[e], pts[e + 1]
+        for qq in range(nq):
+            x = a + (b - a) * (X[qq] + 1) / 2
+            s = po.find_span(x, k, p)
+            v = po.eval_basis_with_derivatives(s, x, k, p, 1)
+            rb, rd = np.zeros(n), np.zeros(n)
+            rb[s - p: s + 1] = v[0]
+            rd[s - p: s + 1] = v[1]
+            rows_b.append(rb)
+            rows_d.append(rd)
+            wts.append(W[qq] * (b - a) / 2)
+    return np.array(rows_b), np.array(rows_d), np.array(wts)
+
+
+@pytest.mark.parametrize("p,ne,nq,r", [((3, 2), (5, 4), (2, 3), (0, 0)), ((2, 3), (4, 5), (2, 6), (1, 0)),
+                                       ((1, 3), (6, 3), (4, 3), (0, 2))])
+def test_general_config_forms_match_quadrature_matrix_form(p, ne, nq, r):
+    """2D, every form, against global quadrature matrices built by point evaluation: covers
+    quad_order < p+1 (under-integration, where no exact form exists), repeated knots and a
+    degree per direction."""
+    sim = po.Sim(2, list(p), list(ne), quad_order=list(nq), repeated_nodes=list(r))
+    Bx, Dx, wx = _general_quadrature_matrices(p[0], ne[0], nq[0], r[0])
+    By, Dy, wy = _general_quadrature_matrices(p[1], ne[1], nq[1], r[1])
+    B, DX, DY, w = np.kron(By, Bx), np.kron(By, Dx), np.kron(Dy, Bx), np.kron(wy, wx)
+    rng = np.random.default_rng(5)
+    c = rng.uniform(0, 1, sim.size)
+    kf = np.exp(rng.uniform(-0.5, 0.5, sim.size))
+    src = rng.standard_normal(sim.size)
+    dt = 3e-3
+    for kw in (dict(), dict(kfield=kf, reaction=0.7, source=src)):
+        u, ux, uy = B @ c, DX @ c, DY @ c
+        kv = B @ kw["kfield"] if "kfield" in kw else np.ones_like(u)
+        fv = B @ kw["source"] if "source" in kw else np.zeros_like(u)
+        val = u + dt * kw.get("reaction", 0.0) * u * (1 - u) + dt * fv
+        expect = B.T @ (w * val) - dt * (DX.T @ (w * kv * ux) + DY.T @ (w * kv * uy))
+        got = sim.compute_rhs(c, po.heat_form(dt, 2, **kw))
+        assert po.rel_l2(got, expect) < 1e-13, kw.keys()
