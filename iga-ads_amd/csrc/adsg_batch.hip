@@ -308,7 +308,24 @@ int batch_step(adsg_batch* b) {
     return b->precision == ADSG_FP32 ? batch_step_impl<float>(b) : batch_step_impl<double>(b);
 }
 
+// Steps per captured graph: longer runs replay graphs of this many steps one after another
+// (even, so the state arenas' parity is the same at every chunk start). It bounds the node
+// count of one graph (about 20 kernels per step) and its instantiation cost.
+constexpr int kMaxGraphSteps = 50;
+
+int run_batch_chunk(adsg_batch* b, int nsteps);
+
 int run_batch(adsg_batch* b, int nsteps) {
+    while (nsteps > 0) {
+        const int n = nsteps > kMaxGraphSteps ? kMaxGraphSteps : nsteps;
+        const int rc = run_batch_chunk(b, n);
+        if (rc) return rc;
+        nsteps -= n;
+    }
+    return ADSG_OK;
+}
+
+int run_batch_chunk(adsg_batch* b, int nsteps) {
     const auto key = std::make_pair(nsteps, b->phase);
     auto it = b->graphs.find(key);
     hipGraphExec_t ex = nullptr;

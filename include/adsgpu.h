@@ -327,8 +327,8 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
  * same in x and y), and a form without k field or source (e.g. the reaction-diffusion form
  * {mass 1, dt, grad eps, reaction 1}). The state of all problems lives in shared arenas;
  * one step is a fixed sequence of grouped launches over all problems (element kernels per
- * degree and colour, x and y sweeps per degree, one update), and nsteps steps are captured
- * once into a HIP graph and replayed. FP32 batches use the increment scheme of ADSG_FP32. */
+ * degree and colour, x and y sweeps per degree, one update), and nsteps steps are replayed
+ * from HIP graphs of at most 50 steps, each captured once per step count. FP32 batches use the increment scheme of ADSG_FP32. */
 typedef struct adsg_batch adsg_batch;
 int adsg_batch_create(const adsg_config* cfgs, int count, adsg_batch** out);
 void adsg_batch_destroy(adsg_batch* b);
