@@ -72,6 +72,7 @@ EXPORTS = [
     "adsg_band_factorize", "adsg_time_phases", "adsg_set_tuning", "adsg_get_tuning",
     "adsg_autotune", "adsg_project", "adsg_project_device", "adsg_quad_points", "adsg_error",
     "adsg_sample", "adsg_norm_noalloc", "adsg_check_finite", "adsg_device_rates",
+    "adsg_save_state", "adsg_load_state",
 ]
 TUNE_RHS_VARIANT = 0
 TUNE_SWEEP_VARIANT = 1
@@ -148,6 +149,8 @@ def lib():
         L.adsg_error.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
         L.adsg_check_finite.argtypes = [C.c_void_p, C.c_int]
         L.adsg_device_rates.argtypes = [_dp]
+        L.adsg_save_state.argtypes = [C.c_void_p, C.c_char_p, C.c_long, C.c_double]
+        L.adsg_load_state.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_long), _dp]
         L.adsg_norm_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp]
         L.adsg_sample.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _dp, C.c_int, _dp]
         _lib = L
@@ -279,6 +282,17 @@ class Simulation:
 
     def step(self, n, form, flags=0):
         self._check(lib().adsg_step(self.h, n, C.byref(form), flags), "step")
+
+    def save_state(self, path, step=0, t=0.0):
+        """adsg_save_state: u and u_prev with a checked header (configuration, step, time)."""
+        self._check(lib().adsg_save_state(self.h, os.fsencode(path), step, t), "save_state")
+
+    def load_state(self, path):
+        """adsg_load_state: restore u and u_prev; returns (step, t) as saved."""
+        st, t = C.c_long(0), np.zeros(1)
+        self._check(lib().adsg_load_state(self.h, os.fsencode(path), C.byref(st), t.ctypes.data_as(_dp)),
+                    "load_state")
+        return int(st.value), float(t[0])
 
     def check_finite(self, which=U):
         """adsg_check_finite: the return code (OK or ENONFINITE) and the library's message."""

@@ -146,6 +146,7 @@ bool build_direction(direction& d, int p, int elements, double a, double b, int 
     d.a = a;
     d.b = b;
     d.nq = nq;
+    d.repeated_nodes = repeated_nodes;
     d.knot = make_knots(a, b, p, elements, repeated_nodes);
     d.dofs = static_cast<int>(d.knot.size()) - p - 1;
     std::vector<double> X(nq), W(nq);

@@ -27,6 +27,7 @@ struct direction {
     int elements = 0;
     int dofs = 0;
     int nq = 0;
+    int repeated_nodes = 0;
     double a = 0, b = 1;
     std::vector<double> knot, val, der, wj, xq, w, J;
     std::vector<int> first;

@@ -273,6 +273,17 @@ int adsg_adjoint_step_delta(adsg_sim* h, int nsteps, const adsg_rhs_form* form, 
 
 int adsg_synchronize(adsg_sim* h);
 
+/* Checkpoint and resume (no counterpart in the reference, whose text dumps of coefficients,
+ * implicit.hpp:121-127, are not read back into a run): adsg_save_state writes u and u_prev
+ * (FP64) with a 128-byte header -- "ADSGSTAT", version, the configuration (dim, p, elements,
+ * dofs, quad_order, repeated_nodes per direction), precision, the caller's step and time and
+ * an FNV-1a checksum of the payload -- to a temporary file renamed over path when complete.
+ * adsg_load_state checks the header against the handle's configuration and the checksum
+ * before it changes the state, returns step and time, and fails with ADSG_EINVAL (message in
+ * adsg_last_error) on a foreign, truncated or corrupted file. Both synchronise the handle. */
+int adsg_save_state(adsg_sim* h, const char* path, long step, double t);
+int adsg_load_state(adsg_sim* h, const char* path, long* step, double* t);
+
 /* The same NaN / Inf count for any FP64 vector of the handle (ADSG_U .. ADSG_GRAD_F; u and
  * u_prev in both precisions): ADSG_OK, or ADSG_ENONFINITE with the count in the message.
  * No counterpart in the reference. Synchronises the handle's stream. */
