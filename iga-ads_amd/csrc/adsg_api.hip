@@ -55,18 +55,24 @@ void mark(adsg_sim* h, int i) {
 // element-kernel choice passed to launch_rhs: the variant, and the column shape of variant 5
 int rhs_code(const adsg_sim* h) { return h->rhs_variant | (h->rhs_col_shape << 4); }
 
+// a device copy of bytes at *dst (freed and replaced if set), or staged for a batch's arena
+int upload_bytes(adsg_sim* h, void** dst, const void* data, size_t bytes) {
+    if (*dst && !h->in_arena) (void)hipFree(*dst);
+    *dst = nullptr;
+    if (h->stage) {
+        h->stage->add(dst, data, bytes);
+        return ADSG_OK;
+    }
+    if (hipMalloc(dst, bytes) != hipSuccess) return fail(h, ADSG_ENOMEM, "hipMalloc failed for tables");
+    HIP_TRY(h, hipMemcpy(*dst, data, bytes, hipMemcpyHostToDevice));
+    return ADSG_OK;
+}
+
 template <typename T>
 int upload(adsg_sim* h, void** dst, const std::vector<double>& src) {
-    if (*dst) {
-        (void)hipFree(*dst);
-        *dst = nullptr;
-    }
-    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
-    if (hipMalloc(dst, bytes) != hipSuccess) return fail(h, ADSG_ENOMEM, "hipMalloc failed for tables");
     std::vector<T> tmp(std::max<size_t>(src.size(), 1), T(0));
     for (size_t i = 0; i < src.size(); ++i) tmp[i] = static_cast<T>(src[i]);
-    HIP_TRY(h, hipMemcpy(*dst, tmp.data(), bytes, hipMemcpyHostToDevice));
-    return ADSG_OK;
+    return upload_bytes(h, dst, tmp.data(), tmp.size() * sizeof(T));
 }
 
 // band storage (ld = 3p+1, A(i,j) at j*ld + 2p + i - j) -> rows M[i][b] = A(i, i+b-p)
@@ -83,9 +89,6 @@ std::vector<double> band_rows(const std::vector<double>& ab, int n, int p) {
 
 int upload_tables(adsg_sim* h, dev_dir& d) {
     int rc;
-    // form_matrix.cpp:8-42 on the same tables: the Gram and stiffness matrices, unfixed
-    const std::vector<double> M = band_rows(adsg::band_form(d.tab, 1.0, 0.0), d.tab.dofs, d.tab.p);
-    const std::vector<double> K = band_rows(adsg::band_form(d.tab, 0.0, 1.0), d.tab.dofs, d.tab.p);
     // the element tables zero-padded to whole kTablePad-element tiles, and their weighted copies
     const int per = d.tab.nq * (d.tab.p + 1);
     const int padded = (d.tab.elements + adsg::kTablePad - 1) / adsg::kTablePad * adsg::kTablePad;
@@ -114,15 +117,18 @@ int upload_tables(adsg_sim* h, dev_dir& d) {
         if ((rc = upload<double>(h, &d.bder, bder))) return rc;
         if ((rc = upload<double>(h, &d.wj, d.tab.wj))) return rc;
     }
-    // the 1D matrices in FP64 in both precisions (the separable rhs contracts in FP64)
-    if ((rc = upload<double>(h, &d.kM, M))) return rc;
-    if ((rc = upload<double>(h, &d.kK, K))) return rc;
+    // form_matrix.cpp:8-42 on the same tables: the Gram and stiffness matrices, unfixed, in
+    // FP64 in both precisions (the separable rhs contracts in FP64); a batch's problems have
+    // no separable rhs and skip them
+    if (!h->in_arena) {
+        const std::vector<double> M = band_rows(adsg::band_form(d.tab, 1.0, 0.0), d.tab.dofs, d.tab.p);
+        const std::vector<double> K = band_rows(adsg::band_form(d.tab, 0.0, 1.0), d.tab.dofs, d.tab.p);
+        if ((rc = upload<double>(h, &d.kM, M))) return rc;
+        if ((rc = upload<double>(h, &d.kK, K))) return rc;
+    }
     // first DOF of every element
-    if (hipMalloc(reinterpret_cast<void**>(&d.first), sizeof(int) * std::max(1, d.tab.elements)) != hipSuccess)
-        return fail(h, ADSG_ENOMEM, "hipMalloc failed for tables");
-    if (hipMemcpy(d.first, d.tab.first.data(), sizeof(int) * d.tab.elements, hipMemcpyHostToDevice) != hipSuccess)
-        return fail(h, ADSG_EHIP, "upload of the first DOFs failed");
-    return ADSG_OK;
+    return upload_bytes(h, reinterpret_cast<void**>(&d.first), d.tab.first.data(),
+                        sizeof(int) * std::max<size_t>(d.tab.first.size(), 1));
 }
 
 // the device tables of every direction, in the handle's precision TT
@@ -157,14 +163,15 @@ int refuse_general(adsg_sim* h, const char* what) {
                                 "repeated interior knots or a degree per direction");
 }
 
-void free_slot(dev_slot& s) {
+// the slot's device factors (pointers into a batch's arena are only forgotten)
+void free_slot(dev_slot& s, bool in_arena = false) {
     for (void** p : {&s.lower, &s.upper, &s.rdiag, &s.pack, &s.packt}) {
-        if (*p) (void)hipFree(*p);
+        if (*p && !in_arena) (void)hipFree(*p);
         *p = nullptr;
     }
-    if (s.poff) (void)hipFree(s.poff);
+    if (s.poff && !in_arena) (void)hipFree(s.poff);
     s.poff = nullptr;
-    if (s.arow) (void)hipFree(s.arow);
+    if (s.arow && !in_arena) (void)hipFree(s.arow);
     s.arow = nullptr;
 }
 
@@ -663,7 +670,7 @@ int prepare_line_matrices(adsg_sim* h, int dir, dev_slot& s) {
             poff[static_cast<size_t>(j) * L + l] = f.poff[j];
         }
     }
-    free_slot(s);
+    free_slot(s, h->in_arena);
     int rc;
     if (h->precision == ADSG_FP32) {
         if ((rc = upload<float>(h, &s.lower, lower))) return rc;
@@ -674,9 +681,8 @@ int prepare_line_matrices(adsg_sim* h, int dir, dev_slot& s) {
         if ((rc = upload<double>(h, &s.upper, upper))) return rc;
         if ((rc = upload<double>(h, &s.rdiag, rdiag))) return rc;
     }
-    if (hipMalloc(reinterpret_cast<void**>(&s.poff), sizeof(int) * poff.size()) != hipSuccess)
-        return fail(h, ADSG_ENOMEM, "hipMalloc failed for pivots");
-    HIP_TRY(h, hipMemcpy(s.poff, poff.data(), sizeof(int) * poff.size(), hipMemcpyHostToDevice));
+    if ((rc = upload_bytes(h, reinterpret_cast<void**>(&s.poff), poff.data(), sizeof(int) * poff.size())))
+        return rc;
     s.lf = adsg::line_factors{};
     s.lf.n = n;
     s.lf.kl = kl;
@@ -764,7 +770,7 @@ int run_graphs(adsg_sim* h, step_kind kind, int nsteps, const adsg_rhs_form* for
 
 // the handle of adsg_create; with a shared stream (not owned) and without state vectors for
 // the problems of a batch, whose vectors live in the batch's arenas
-int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, adsg_sim** out) {
+int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, adsg_sim** out, stager* stage) {
     if (!cfg || !out) {
         g_create_error = "adsg_create: null argument";
         return ADSG_EINVAL;
@@ -829,6 +835,8 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
         delete h;
         return ADSG_EHIP;
     }
+    h->stage = stage;  // a batch stages the tables into its arena
+    h->in_arena = stage != nullptr;
     for (int d = 0; d < h->dim; ++d) {
         if (upload_tables(h, h->dirs[d])) {
             g_create_error = "adsg_create: " + h->err;
@@ -836,6 +844,7 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
             return ADSG_EHIP;
         }
     }
+    h->stage = nullptr;
     for (int v = ADSG_U; v <= ADSG_BUFFER && vectors; ++v) {
         if (hipMalloc(&h->vec[v], h->N * elt_of(h, v)) != hipSuccess) {
             g_create_error = "adsg_create: device allocation of the state vectors failed";
@@ -936,10 +945,10 @@ void adsg_destroy(adsg_sim* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
     for (auto& d : h->dirs) {
-        for (auto& s : d.slots) free_slot(s);
+        for (auto& s : d.slots) free_slot(s, h->in_arena);
         for (void** p : {&d.val, &d.der, &d.wj, &d.bval, &d.bder, &d.rows, &d.kM, &d.kK})
-            if (*p) (void)hipFree(*p);
-        if (d.first) (void)hipFree(d.first);
+            if (*p && !h->in_arena) (void)hipFree(*p);
+        if (d.first && !h->in_arena) (void)hipFree(d.first);
     }
     for (auto& v : h->vec)
         if (v) (void)hipFree(v);
@@ -978,7 +987,7 @@ int adsg_set_matrix(adsg_sim* h, int dir, int slot, double mass, double stiff) {
     dev_dir& d = h->dirs[dir];
     if (static_cast<int>(d.slots.size()) <= slot) d.slots.resize(slot + 1);
     dev_slot& s = d.slots[slot];
-    free_slot(s);
+    free_slot(s, h->in_arena);
     s = dev_slot{};
     s.set = true;
     s.kl = s.ku = d.tab.p;
@@ -996,7 +1005,7 @@ int adsg_set_band_matrix(adsg_sim* h, int dir, int slot, int kl, int ku, const d
     dev_dir& d = h->dirs[dir];
     if (static_cast<int>(d.slots.size()) <= slot) d.slots.resize(slot + 1);
     dev_slot& s = d.slots[slot];
-    free_slot(s);
+    free_slot(s, h->in_arena);
     s = dev_slot{};
     s.set = true;
     s.kl = kl;
@@ -1016,7 +1025,7 @@ int adsg_set_line_matrices(adsg_sim* h, int dir, int slot, int kl, int ku, const
     dev_dir& d = h->dirs[dir];
     if (static_cast<int>(d.slots.size()) <= slot) d.slots.resize(slot + 1);
     dev_slot& s = d.slots[slot];
-    free_slot(s);
+    free_slot(s, h->in_arena);
     s = dev_slot{};
     s.set = true;
     s.kl = kl;
@@ -1050,7 +1059,7 @@ int adsg_prepare_matrices(adsg_sim* h) {
                 if (!first_info) first_info = info;
                 continue;
             }
-            free_slot(s);
+            free_slot(s, h->in_arena);
             int rc;
             if (h->precision == ADSG_FP32) {
                 if ((rc = upload<float>(h, &s.lower, s.lf.lower))) return rc;
@@ -1081,15 +1090,12 @@ int adsg_prepare_matrices(adsg_sim* h) {
                         if (j + t >= 0 && j + t < n)
                             rows[static_cast<size_t>(j) * w + t + s.kl] =
                                 s.ab[static_cast<size_t>(j + t) * ld + s.kl + s.ku - t];
-                void* ar = nullptr;
-                if ((rc = upload<double>(h, &ar, rows))) return rc;
-                s.arow = static_cast<double*>(ar);
+                if ((rc = upload<double>(h, reinterpret_cast<void**>(&s.arow), rows))) return rc;
             } else {
                 s.seg = adsg::line_segments{};
             }
-            if (hipMalloc(reinterpret_cast<void**>(&s.poff), sizeof(int) * s.lf.n) != hipSuccess)
-                return fail(h, ADSG_ENOMEM, "hipMalloc failed for pivots");
-            HIP_TRY(h, hipMemcpy(s.poff, s.lf.poff.data(), sizeof(int) * s.lf.n, hipMemcpyHostToDevice));
+            if ((rc = upload_bytes(h, reinterpret_cast<void**>(&s.poff), s.lf.poff.data(), sizeof(int) * s.lf.n)))
+                return rc;
             s.factored = true;
         }
         // FP32 increments need the rows of the (fixed) slot-0 matrix for boundary residuals

@@ -5,14 +5,20 @@
 // One step is a fixed sequence of grouped launches over all problems: zero fill of the
 // arena, per degree p and parity colour one element-kernel launch whose blocks cover the
 // tiles of every problem of that degree, per degree one x-sweep and one y-sweep launch over
-// all their lines, and one elementwise update of the whole arena. The step sequence for
-// nsteps steps is captured once into a HIP graph and replayed.
+// all their lines, and one elementwise update of the whole arena. The step sequences are
+// replayed from captured HIP graphs of at most 50 steps. The problems' host set-up (tables,
+// band LU, segment coefficients) runs on worker threads and stages its uploads, which then
+// go to the device as one arena per phase (create, prepare) with one copy each.
 #include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <thread>
 
 #include <algorithm>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -61,10 +67,29 @@ struct adsg_batch {
     std::vector<batch_group> groups[2];
     void* packs = nullptr;  // packed sweep rows of every problem and direction (step precision)
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (nsteps, phase at start)
+    // every problem's tables (create) and factors (prepare), each set in one device arena
+    void* arena[2] = {nullptr, nullptr};
     std::string err;
 };
 
 namespace {
+
+// f(0) .. f(count - 1) on up to 16 host threads (the problems' set-up is independent)
+template <typename F>
+void for_each_problem(int count, F&& f) {
+    const int hw = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+    const int nt = std::max(1, std::min({16, hw, count}));
+    int dev = 0;
+    (void)hipGetDevice(&dev);  // the workers use the caller's device (HIP's is per thread)
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&] {
+            (void)hipSetDevice(dev);
+            for (int i = next++; i < count; i = next++) f(i);
+        });
+    for (auto& th : pool) th.join();
+}
 
 int bfail(adsg_batch* b, int code, const std::string& msg) {
     if (b) b->err = msg;
@@ -385,23 +410,40 @@ int adsg_batch_create(const adsg_config* cfgs, int count, adsg_batch** out) {
             adsg_batch_destroy(b);
             return ADSG_EINVAL;
         }
-        adsg_sim* h = nullptr;
-        int rc = adsg::create_sim(&c, b->stream, false, &h);
-        if (rc) {
-            g_batch_error = "batch_create: problem " + std::to_string(i) + ": " + adsg_last_error(nullptr);
-            adsg_batch_destroy(b);
-            return rc;
+    }
+    // the problems' host set-up on worker threads, each staging its tables; then one upload
+    std::vector<adsg::stager> st(count);
+    std::vector<adsg_sim*> hs(count, nullptr);
+    std::vector<int> rcs(count, ADSG_OK);
+    std::vector<std::string> msgs(count);
+    for_each_problem(count, [&](int i) {
+        rcs[i] = adsg::create_sim(&cfgs[i], b->stream, false, &hs[i], &st[i]);
+        if (rcs[i]) msgs[i] = adsg_last_error(nullptr);  // this thread's create error
+    });
+    int bad = -1;
+    for (int i = 0; i < count && bad < 0; ++i)
+        if (rcs[i] || hs[i]->general) bad = i;
+    for (int i = 0; i < count; ++i)
+        if (hs[i]) {
+            b->sims.push_back(hs[i]);
+            b->off.push_back(b->total);
+            b->total += hs[i]->N;
         }
-        if (h->general) {
-            adsg_destroy(h);
-            g_batch_error = "batch_create: problem " + std::to_string(i) +
-                            ": batches take p+1 Gauss points and simple knots (quad_order 0 or p+1, repeated_nodes 0)";
-            adsg_batch_destroy(b);
-            return ADSG_EINVAL;
-        }
-        b->sims.push_back(h);
-        b->off.push_back(b->total);
-        b->total += h->N;
+    if (bad >= 0) {
+        g_batch_error = "batch_create: problem " + std::to_string(bad) + ": " +
+                        (rcs[bad] ? msgs[bad]
+                                  : std::string("batches take p+1 Gauss points and simple knots (quad_order 0 or p+1, "
+                                                "repeated_nodes 0)"));
+        const int rc = rcs[bad] ? rcs[bad] : ADSG_EINVAL;
+        adsg_batch_destroy(b);
+        return rc;
+    }
+    adsg::stager tables;
+    for (auto& x : st) tables.take(x);
+    if (tables.commit(&b->arena[0]) != hipSuccess) {
+        g_batch_error = "batch_create: uploading the problems' tables failed";
+        adsg_batch_destroy(b);
+        return ADSG_ENOMEM;
     }
     bool ok = true;
     for (auto& a : b->u) ok = ok && hipMalloc(reinterpret_cast<void**>(&a), b->total * sizeof(double)) == hipSuccess;
@@ -434,6 +476,8 @@ void adsg_batch_destroy(adsg_batch* b) {
     drop_batch_graphs(b);
     free_groups(b);
     for (adsg_sim* h : b->sims) adsg_destroy(h);
+    for (void* a : b->arena)
+        if (a) (void)hipFree(a);
     for (auto& a : b->u)
         if (a) (void)hipFree(a);
     for (auto& a : b->sh)
@@ -455,9 +499,22 @@ int adsg_batch_dofs(const adsg_batch* b, int problem, int dir) {
 
 int adsg_batch_prepare(adsg_batch* b) {
     if (!b) return ADSG_EINVAL;
+    if (b->prepared) return ADSG_OK;  // the factors live in the batch's arena
+    // every problem's factors on worker threads, each staged; then one upload
+    const int count = static_cast<int>(b->sims.size());
+    std::vector<adsg::stager> st(count);
+    std::vector<int> rcs(count, ADSG_OK);
+    for_each_problem(count, [&](int i) {
+        adsg_sim* h = b->sims[i];
+        h->stage = &st[i];
+        rcs[i] = adsg_prepare_matrices(h);
+        h->stage = nullptr;
+    });
+    adsg::stager factors;
+    for (auto& x : st) factors.take(x);
     for (size_t i = 0; i < b->sims.size(); ++i) {
         adsg_sim* h = b->sims[i];
-        int rc = adsg_prepare_matrices(h);
+        const int rc = rcs[i];
         if (rc) return bfail(b, rc, "batch_prepare: problem " + std::to_string(i) + ": " + h->err);
         for (int d = 0; d < 2; ++d) {
             const auto& s = h->dirs[d].slots[0];
@@ -466,6 +523,8 @@ int adsg_batch_prepare(adsg_batch* b) {
                                                  ": batched sweeps need unpivoted degree-p band factors");
         }
     }
+    hipError_t e = factors.commit(&b->arena[1]);
+    if (e != hipSuccess) return bhip(b, e, "batch_prepare: uploading the factors");
     int rc = build(b);
     if (rc) return rc;
     b->prepared = true;
