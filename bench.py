@@ -55,8 +55,9 @@ def run_form(ag, precision, ne, steps, warmup, u0, dt, variant=None, kron=False)
     t1 = time.perf_counter()
     out = sim.get(ag.U)
     phases = sim.time_phases(3, form)  # eager, events per phase
+    tiling = sim.get_tuning(ag.TUNE_RHS_TILING)
     sim.close()
-    return (t1 - t0), phases, out, variant
+    return (t1 - t0), phases, out, variant, tiling
 
 
 def run_increment_form(ag, precision, ne, steps, warmup, u0, dt):
@@ -436,6 +437,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--rhs-variant", type=int, default=None,
                     help="element-kernel variant (default: autotuned)")
+    ap.add_argument("--rhs-tiling", type=int, default=None, choices=(0, 1),
+                    help="--eager-only: element-kernel tiling, 0 colour launches, 1 one launch "
+                         "(default: the handle's)")
     ap.add_argument("--no-implicit", action="store_true",
                     help="skip the 3D p=3 implicit / adjoint block")
     ap.add_argument("--implicit-elements", type=int, default=192)
@@ -491,25 +495,34 @@ def main():
             variant = sim.autotune(form)
         else:
             sim.set_tuning(ag.TUNE_RHS_VARIANT, variant)
+        if args.rhs_tiling is not None:
+            sim.set_tuning(ag.TUNE_RHS_TILING, args.rhs_tiling)
+        tiling = sim.get_tuning(ag.TUNE_RHS_TILING)
         phases = sim.time_phases(args.steps, form)
         sim.close()
         print(json.dumps({"eager_steps": args.steps, "precision": args.precision, "degree": args.degree,
-                          "elements": ne, "rhs_variant": variant, "phases_ms": phases}), flush=True)
+                          "elements": ne, "rhs_variant": variant, "rhs_tiling": tiling,
+                          "phases_ms": phases}), flush=True)
         return
 
     rates = ag.device_rates()  # measured FMA ceilings of this device (vector and matrix)
-    t64, ph64, x64, var64 = run_form(ag, ag.FP64, ne, args.steps, args.warmup, u0, dt,
-                                     args.rhs_variant)
+    t64, ph64, x64, var64, til64 = run_form(ag, ag.FP64, ne, args.steps, args.warmup, u0, dt,
+                                            args.rhs_variant)
     value = N * args.steps / t64
     ms_step = 1e3 * t64 / args.steps
 
-    # dominant kernel: the element kernel (k_rhs3; k_rhs3_pipe, k_rhs3_zpair for variants 3, 4), launched
-    # once per parity colour (8 launches per step, each over E/8 elements); HIP events on the
-    # handle's stream bracket the 8 launches
+    # dominant kernel: the element kernel (k_rhs3; k_rhs3_pipe, k_rhs3_zpair for variants 3, 4),
+    # launched once over all tiles followed by the spill fix-up k_rhs3_spill (tiling 1), or once
+    # per parity colour (8 launches per step, each over E/8 elements); HIP events on the
+    # handle's stream bracket the launches of the phase
     rhs_kernel = {3: "k_rhs3_pipe", 4: "k_rhs3_zpair"}.get(var64, "k_rhs3")
-    launches = 8
+    one = til64 == 1 and var64 != 3
+    launches = 1 if one else 8
     launch_flops = f_elem(3, 2) * E / launches
     launch_ms = ph64["rhs"] / launches
+    traffic_key = rhs_kernel + ("+k_rhs3_spill" if one else "")
+    launch_desc = ("one launch over all tiles + the k_rhs3_spill fix-up; achieved over both "
+                   "kernels' time" if one else "one launch per parity colour")
     rhs_tflops = launch_flops / (launch_ms * 1e-3) / 1e12
     sweep_bytes = 2 * N * 8  # one directional sweep: read and write every coefficient once
     sweep_ms = [ph64["sweep_x"], ph64["sweep_y"], ph64["sweep_z"]]
@@ -534,24 +547,25 @@ def main():
             "dim": 3, "p": 2, "elements": ne, "dofs": N, "dt": dt,
         },
         "roofline": {
-            "kernel": f"{rhs_kernel} (element-loop RHS assembly, variant {var64}; one launch "
-                      "per parity colour)",
+            "kernel": f"{rhs_kernel} (element-loop RHS assembly, variant {var64}; {launch_desc})",
             "bound": "valu_fp64",
             "achieved": rhs_tflops,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": rhs_tflops / PEAK_FP64_TFLOPS,
-            "traffic": launch_traffic(rhs_kernel),
+            "traffic": launch_traffic(traffic_key),
             "launch_flops": launch_flops,
             "launch_ms": launch_ms,
             "note": ("FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 matrix peak = "
-                     "78.6 TFLOP/s); achieved = 3051 flop/element x E/8 elements per launch / "
-                     "average launch time (HIP events around the 8 launches on the handle's "
-                     "stream); traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE and "
-                     "WRITE_SIZE scaled by a copy-kernel calibration"),
+                     "78.6 TFLOP/s); achieved = 3051 flop/element x the elements of one launch "
+                     "(E, or E/8 per colour launch) / average launch time (HIP events around the "
+                     "phase's launches on the handle's stream, the fix-up kernel included); "
+                     "traffic = HBM bytes per launch (both kernels) from rocprofv3 FETCH_SIZE "
+                     "and WRITE_SIZE scaled by a copy-kernel calibration"),
         },
         "phases_ms": ph64,
         "rhs_variant": var64,
+        "rhs_tiling": til64,
         "device_rates_tflops": dict(rates, note=(
             "FMA ceilings measured on this device by adsg_device_rates (independent FMA chains "
             "filling the device): vector FP64 / FP32 (plain FMAs, no packed instructions) and "
@@ -565,12 +579,12 @@ def main():
     }
 
     if not args.no_fp32:
-        t32, ph32, x32, var32 = run_form(ag, ag.FP32, ne, args.steps, args.warmup, u0, dt,
-                                         args.rhs_variant)
+        t32, ph32, x32, var32, til32 = run_form(ag, ag.FP32, ne, args.steps, args.warmup, u0, dt,
+                                                args.rhs_variant)
         ti, phi, xi, vari = run_increment_form(ag, ag.FP64, ne, args.steps, args.warmup, u0, dt)
         err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
         kern32 = {3: "k_rhs3_pipe", 4: "k_rhs3_zpair"}.get(var32, "k_rhs3")
-        tf32 = launch_flops / (ph32["rhs"] / launches * 1e-3) / 1e12
+        tf32 = f_elem(3, 2) * E / (ph32["rhs"] * 1e-3) / 1e12
         result["forms"] = {
             "fp64": {"value": value, "ms_per_step": ms_step},
             "fp64_increment": {
@@ -591,14 +605,15 @@ def main():
                 "rel_l2_vs_fp64_after_all_steps": err,
                 "function_rel_l2_vs_fp64_after_all_steps": func_rel_l2(ag, 2, ne, x32, x64),
                 "rhs_variant": var32,
+                "rhs_tiling": til32,
                 "phases_ms": ph32,
                 "rhs_roofline": {
-                    "kernel": f"{kern32} (FP32, variant {var32})",
+                    "kernel": f"{kern32} (FP32, variant {var32}, tiling {til32})",
                     "bound": "valu_fp32",
                     "achieved": tf32, "unit": "TFLOP/s",
                     "peak": 78.6, "frac": tf32 / 78.6,
                     "peak_mfma_fp32": 157.3, "frac_of_mfma_fp32": tf32 / 157.3,
-                    "note": ("3051 flop/element x E/8 elements per launch / average launch time; "
+                    "note": ("3051 flop/element x E elements / the rhs phase's time; "
                              "FP32 VALU without packed instructions issues at the FP64 rate "
                              "(78.6 TFLOP/s); 157.3 TFLOP/s is the FP32 MFMA / packed peak"),
                 },
@@ -608,7 +623,7 @@ def main():
     if not args.no_kron:
         kron = {}
         for name, prec in (("fp64", ag.FP64), ("fp32", ag.FP32)):
-            tk, phk, xk, _ = run_form(ag, prec, ne, args.steps, args.warmup, u0, dt, args.rhs_variant,
+            tk, phk, xk, _, _ = run_form(ag, prec, ne, args.steps, args.warmup, u0, dt, args.rhs_variant,
                                       kron=True)
             kron[name] = {"value": N * args.steps / tk, "ms_per_step": 1e3 * tk / args.steps,
                           "speedup_vs_element_fp64": t64 / tk, "phases_ms": phk,

@@ -79,6 +79,7 @@ TUNE_SWEEP_VARIANT = 1
 TUNE_RHS_KRONECKER = 3
 TUNE_BATCH_TILE = 2
 TUNE_RHS_COLUMNS = 4
+TUNE_RHS_TILING = 5
 
 
 def lib():
@@ -420,7 +421,8 @@ class Simulation:
 
     def time_phases(self, n, form, slots=None, flags=0):
         """Average device ms per phase of n eager steps (explicit steps, or delta steps with
-        slots): element kernels, boundary handling, x/y/z sweeps, whole step, zero fill."""
+        slots): element kernels (with the rhs zero fill where one is needed), boundary
+        handling, x/y/z sweeps, whole step, rhs start (a source's load vector)."""
         ms = np.zeros(7)
         self._check(lib().adsg_time_phases(self.h, n, C.byref(form), self._slots(slots), flags,
                                            ms.ctypes.data_as(_dp)), "time_phases")

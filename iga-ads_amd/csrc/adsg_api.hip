@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,7 +54,23 @@ void mark(adsg_sim* h, int i) {
 }
 
 // element-kernel choice passed to launch_rhs: the variant, and the column shape of variant 5
-int rhs_code(const adsg_sim* h) { return h->rhs_variant | (h->rhs_col_shape << 4); }
+int rhs_code(const adsg_sim* h) {
+    return h->rhs_variant | (h->rhs_col_shape << 4) | (h->rhs_tiling && h->spill ? adsg::kRhsOneLaunch : 0);
+}
+
+// the spill array of the one-launch tiling, allocated when the tiling is first selected
+// (never inside a captured step); FP64-sized in both precisions, since FP32 handles also run
+// FP64 element loops (the adjoint's and the norms' products)
+int ensure_spill(adsg_sim* h) {
+    if (h->spill || h->dim != 3 || h->general) return ADSG_OK;
+    const int el[3] = {h->dirs[0].tab.elements, h->dirs[1].tab.elements, h->dirs[2].tab.elements};
+    const size_t bytes = static_cast<size_t>(adsg::rhs3_spill_slots(h->dirs[0].tab.p, el)) * sizeof(double);
+    if (bytes && hipMalloc(&h->spill, bytes) != hipSuccess) {
+        h->spill = nullptr;
+        return fail(h, ADSG_ENOMEM, "one-launch tiling: no memory for the spill array");
+    }
+    return ADSG_OK;
+}
 
 // a device copy of bytes at *dst (freed and replaced if set), or staged for a batch's arena
 int upload_bytes(adsg_sim* h, void** dst, const void* data, size_t bytes) {
@@ -150,11 +167,21 @@ void dir_tables(const adsg_sim* h, adsg::dev_tables<TT>* t) {
 
 // the element loop of the handle: the general kernel for configurations outside the
 // sum-factorised kernels', else the tuned variant
+// (overwrite: out := the element sums, its start value unused; else out += them)
 template <typename T>
 hipError_t launch_elements(const adsg_sim* h, const T* up, T* out, const adsg::dev_tables<T>* t,
-                           const adsg::rhs_params<T>& prm) {
-    if (h->general) return adsg::launch_rhs_general<T>(h->dim, up, out, t, prm, h->stream);
-    return adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, out, t, prm, rhs_code(h), h->stream);
+                           const adsg::rhs_params<T>& prm, bool overwrite = false) {
+    if (h->general) {
+        if (overwrite) {
+            const hipError_t e = adsg::launch_fill<T>(out, h->N, T(0), h->stream);
+            if (e != hipSuccess) return e;
+        }
+        return adsg::launch_rhs_general<T>(h->dim, up, out, t, prm, h->stream);
+    }
+    adsg::rhs_params<T> q = prm;
+    q.spill = h->rhs_tiling ? static_cast<T*>(h->spill) : nullptr;
+    return adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, out, t, q,
+                               rhs_code(h) | (overwrite ? adsg::kRhsOverwrite : 0), h->stream);
 }
 
 // entry points whose kernels assume one degree and p+1 Gauss points in every direction
@@ -349,10 +376,10 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
     // entry, so only a source needs a start value
     const bool kron = h->rhs_kron && h->dim == 3 && !h->general && !form->use_kfield && form->reaction == 0.0;
     hipError_t e = hipSuccess;
+    // zero(rhs): done by the element launch (launch_elements with overwrite), which needs no
+    // fill in the one-launch tiling
     if (form->use_source)
         e = adsg::launch_scale<T>(rhs, static_cast<const T*>(h->source_load), h->N, prm.dt, h->stream);
-    else if (!kron)
-        e = adsg::launch_fill<T>(rhs, h->N, T(0), h->stream);  // zero(rhs)
     if (e != hipSuccess) return hip_fail(h, e, "rhs start");
     mark(h, 1);
     if (kron) {
@@ -367,7 +394,7 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
         e = adsg::launch_rhs_kron<T>(h->dim, h->dirs[0].tab.p, up, rhs, kd, mass, form->dt, form->grad,
                                      form->use_source, h->stream);
     } else {
-        e = launch_elements<T>(h, up, rhs, t, prm);
+        e = launch_elements<T>(h, up, rhs, t, prm, !form->use_source);
     }
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     mark(h, 2);
@@ -812,15 +839,17 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
     }
     // default element-kernel variant (adsg_autotune refines it): the one measured fastest for
     // the degree and precision. p = 3: FP64 2 (variants 0 and 1 need more registers than allow
-    // two waves per SIMD), FP32 5 (element columns on the matrix cores); FP64 p <= 2: 4 when
-    // the mesh has enough tiles of 512 elements for two workgroups per CU in every colour
-    // launch (2^21 elements), else 1; FP32 p <= 2: 1.
+    // two waves per SIMD), FP32 5 (element columns on the matrix cores); p <= 2: 4 from 2^21
+    // elements (enough tiles of 512 elements to fill the device twice over), else 1. 3D
+    // handles start with the one-launch tiling (ADSG_TUNE_RHS_TILING), which was faster for
+    // every variant measured (p = 2 at 128^3, p = 3 at 192^3, both precisions).
     {
         long E = 1;
         for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
         const int p = dc[0]->p;
-        h->rhs_variant = p == 3 ? (h->precision == ADSG_FP32 ? 5 : 2) : h->precision == ADSG_FP64 && E >= (1L << 21) ? 4 : 1;
+        h->rhs_variant = p == 3 ? (h->precision == ADSG_FP32 ? 5 : 2) : E >= (1L << 21) ? 4 : 1;
         h->rhs_col_shape = 1;  // variant 5: two columns per wave, one chunk per workgroup
+        h->rhs_tiling = h->dim == 3 && !h->general ? 1 : 0;
     }
     if (hipGetDevice(&h->device) != hipSuccess) {
         g_create_error = "adsg_create: hipGetDevice failed (no HIP device?)";
@@ -867,6 +896,8 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
             }
         }
     }
+    // the one-launch tiling's spill array (without it the handle falls back to colour launches)
+    if (h->rhs_tiling && ensure_spill(h) != ADSG_OK) h->rhs_tiling = 0;
     (void)hipStreamSynchronize(h->stream);
     *out = h;
     return ADSG_OK;
@@ -957,6 +988,7 @@ void adsg_destroy(adsg_sim* h) {
     if (h->norm_scratch) (void)hipFree(h->norm_scratch);
     if (h->nonfinite) (void)hipFree(h->nonfinite);
     if (h->source_load) (void)hipFree(h->source_load);
+    if (h->spill) (void)hipFree(h->spill);
     if (h->ckpt) (void)hipFree(h->ckpt);
     if (h->ck_seg) (void)hipFree(h->ck_seg);
     if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);
@@ -1606,9 +1638,14 @@ int adsg_set_tuning(adsg_sim* h, int key, int value) {
                 : key == ADSG_TUNE_SWEEP_VARIANT ? &h->sweep_variant
                 : key == ADSG_TUNE_RHS_KRONECKER ? &h->rhs_kron
                 : key == ADSG_TUNE_RHS_COLUMNS   ? &h->rhs_col_shape
+                : key == ADSG_TUNE_RHS_TILING    ? &h->rhs_tiling
                                                  : nullptr;
     const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 5 : key == ADSG_TUNE_RHS_COLUMNS ? 3 : 1;
     if (!slot || value < 0 || value > max_value) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
+    if (key == ADSG_TUNE_RHS_TILING && value == 1) {
+        const int rc = ensure_spill(h);
+        if (rc) return rc;
+    }
     if (*slot != value) drop_graphs(h);
     *slot = value;
     return ADSG_OK;
@@ -1620,6 +1657,7 @@ int adsg_get_tuning(const adsg_sim* h, int key) {
     if (key == ADSG_TUNE_SWEEP_VARIANT) return h->sweep_variant;
     if (key == ADSG_TUNE_RHS_KRONECKER) return h->rhs_kron;
     if (key == ADSG_TUNE_RHS_COLUMNS) return h->rhs_col_shape;
+    if (key == ADSG_TUNE_RHS_TILING) return h->rhs_tiling;
     return ADSG_EINVAL;
 }
 
@@ -1645,14 +1683,24 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
         saved.push_back(p);
     }
     int best = h->rhs_variant, best_shape = h->rhs_col_shape, best_sweep = h->sweep_variant;
+    int best_tiling = h->rhs_tiling;
     double best_ms = 1e30, best_sweep_ms = 1e30;
     int rc = ADSG_OK;
-    // variants 0..4, then the column variant (5) in each of its shapes
-    for (int k = 0; k <= 8 && rc == ADSG_OK; ++k) {
-        const int v = k < 5 ? k : 5;
+    // candidates {variant, shape of variant 5, tiling}: variants 0..4 in colour launches, the
+    // column variant (5) in each of its shapes, and (3D) variants 0, 1, 2, 4 in one launch
+    std::vector<std::array<int, 3>> cand;
+    for (int v = 0; v <= 4; ++v) cand.push_back({v, 0, 0});
+    for (int c = 0; c <= 3; ++c) cand.push_back({5, c, 0});
+    if (h->dim == 3 && !h->general && ensure_spill(h) == ADSG_OK && h->spill)
+        for (int v : {0, 1, 2, 4}) cand.push_back({v, 0, 1});
+    for (size_t k = 0; k < cand.size() && rc == ADSG_OK; ++k) {
+        const int v = cand[k][0];
         h->rhs_variant = v;
-        h->rhs_col_shape = k < 5 ? 0 : k - 5;
-        h->sweep_variant = v <= 1 ? v : best_sweep;  // both sweeps timed with variants 0, 1
+        h->rhs_col_shape = cand[k][1];
+        h->rhs_tiling = cand[k][2];
+        // both sweeps timed with variants 0, 1 in colour launches
+        const bool sweep_run = v <= 1 && cand[k][2] == 0;
+        h->sweep_variant = sweep_run ? v : best_sweep;
         double ms[7];
         rc = adsg_time_phases(h, 1, form, slots, flags, ms);  // warm-up (code load, caches)
         // 8 timed steps: variants 1 and 3 are a few per cent apart at full-GPU sizes
@@ -1661,9 +1709,10 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
             best_ms = ms[0];
             best = v;
             best_shape = h->rhs_col_shape;
+            best_tiling = h->rhs_tiling;
         }
         const double sweeps = ms[2] + ms[3] + ms[4];
-        if (!rc && v <= 1 && sweeps < best_sweep_ms) {
+        if (!rc && sweep_run && sweeps < best_sweep_ms) {
             best_sweep_ms = sweeps;
             best_sweep = v;
         }
@@ -1680,6 +1729,7 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
     h->pending = h->pending_dirichlet = false;
     h->rhs_variant = best;
     h->rhs_col_shape = best_shape;
+    h->rhs_tiling = best_tiling;
     h->sweep_variant = best_sweep;
     drop_graphs(h);
     return rc;

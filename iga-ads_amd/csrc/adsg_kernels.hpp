@@ -91,13 +91,33 @@ struct rhs_params {
     T mass = 1, dt = 0, gx = 1, gy = 1, gz = 1, reaction = 0;
     const T* kfield = nullptr;  // coefficients of k on the same basis, or null (k = 1)
     const T* source = nullptr;  // coefficients of f on the same basis, or null (f = 0)
+    // 3D one-launch tiling (element-kernel code bit 8): rhs3_spill_slots(p, elements) values
+    // for the block entries a tile does not own; null selects the colour launches. The
+    // entries go to 7 arrays, one per neighbour offset delta = dx + 2 dy + 4 dz (1..7) of the
+    // owner, at spill_off[delta]; array delta has extent spill_ext[d][delta_d] in direction d
+    // (the DOF count, or P entries per tile where delta_d = 1). Set by launch_rhs.
+    T* spill = nullptr;
+    long spill_off[8] = {};
+    int spill_ext[3][2] = {};
+    // set by launch_rhs for the one-launch tiling when the rhs start value is not used: the
+    // owners store their entries instead of adding them
+    bool store = false;
 };
 
 // variant: kernel tuning choice (0 = coefficient block held in registers, 1 = re-read from
-// LDS, ..., 5 = element columns); bits 4 and up carry the shape of variant 5.
+// LDS, ..., 5 = element columns); bits 4..7 carry the shape of variant 5, bit 8 selects one
+// launch over all tiles with a spill fix-up instead of the 8 parity-colour launches (3D,
+// variants 0, 1, 2 and 4, forms without a k field or a source in the kernel; prm.spill set).
 template <typename T>
 hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
                       const rhs_params<T>& prm, int variant, hipStream_t s);
+constexpr int kRhsOneLaunch = 1 << 8;
+// code bit 9: the rhs start value is not used (rhs := the element sums): the one-launch
+// tiling then stores instead of adding and needs no zero fill; other paths fill first
+constexpr int kRhsOverwrite = 1 << 9;
+// values the spill array of the one-launch tiling needs for degree p and these element
+// counts (the largest over the variants that use it)
+long rhs3_spill_slots(int p, const int elements[3]);
 
 // The element loop for any quadrature order, repeated interior knots and a degree per
 // direction (1..3), one lane per element (adsg_rhs_gen.hip); dirs[d].first, nq and p set.

@@ -1,8 +1,12 @@
 // HIP element-loop RHS of the MI355X ADS step (gfx950): compute_rhs (validation.hpp:95-119)
 // with one workgroup per tile of elements, one lane per element, sum factorisation through
 // the 1D tables, local contributions combined in LDS, and tiles of one parity colour
-// (2^d colours) adding their DOF blocks into the global rhs without atomics.
+// (2^d colours) adding their DOF blocks into the global rhs without atomics. In 3D the tiles
+// can also run in one launch: each tile adds the block entries it owns and spills the rest,
+// which a second kernel adds in a fixed order (deterministic either way).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "adsg_kernels.hpp"
 
@@ -64,6 +68,75 @@ __device__ __forceinline__ tile3 rhs3_tile(int t, int colour, int ntx2, int nty2
     t /= nty2;
     const int tz = 2 * t + ((colour >> 2) & 1);
     return {tx * TX, ty * TY, tz * TZ};
+}
+// tile t of the launch: of the given colour, or (ONE) of all ntx x nty x ntz tiles
+template <bool ONE, int TX, int TY, int TZ>
+__device__ __forceinline__ tile3 rhs3_tile_of(int t, int colour, int ntx, int nty) {
+    if constexpr (ONE) {
+        const int tx = t % ntx;
+        t /= ntx;
+        return {tx * TX, (t % nty) * TY, (t / nty) * TZ};
+    } else {
+        return rhs3_tile<TX, TY, TZ>(t, colour, ntx, nty);
+    }
+}
+
+// Where a tile's block sums go. Colour launches (ONE false): every entry is added to the
+// rhs with a return-less global add; the tiles of one colour share no DOF, so each DOF has
+// one writer per launch. One launch over all tiles (ONE true): tile t owns DOF index g of a
+// direction when t T <= g < (t + 1) T, or t T <= g in the direction's last tile, so every
+// DOF has one owner; the owner adds its entry to the rhs (a plain read-modify-write: no
+// other workgroup writes it; a store when the rhs start value is unused) and the other tiles
+// store theirs in the spill arrays, which k_rhs3_spill adds to the rhs afterwards in a fixed
+// order. An entry whose owner lies at offset delta from the tile (delta_d = 1 where the
+// entry is in the tile's top P layers of direction d and the tile is not the last there)
+// goes to array delta at the DOF's own index in the directions with delta_d = 0 and at
+// t_d P + l_d (owner tile t_d, local index l_d < P) in the others: the fix-up then reads
+// every array at consecutive addresses for consecutive DOFs.
+template <typename T>
+struct block_sink {
+    T* rhs;
+    int tile[3];   // ONE: the tile's indices
+    bool last[3];  // ONE: the tile is the last of its direction
+};
+template <bool ONE, int TX, int TY, int TZ, typename T>
+__device__ __forceinline__ block_sink<T> rhs3_sink(T* rhs, tile3 o, int ex, int ey, int ez) {
+    if constexpr (ONE)
+        return {rhs, {o.ex0 / TX, o.ey0 / TY, o.ez0 / TZ}, {o.ex0 + TX >= ex, o.ey0 + TY >= ey, o.ez0 + TZ >= ez}};
+    else
+        return {rhs, {0, 0, 0}, {false, false, false}};
+}
+// index of a spilled entry in the spill arrays: array delta = dx + 2 dy + 4 dz, position
+// (cx, cy, cz) in its extents
+template <typename T>
+__device__ __forceinline__ long spill_index(const rhs_params<T>& prm, int dx, int dy, int dz, int cx, int cy,
+                                            int cz) {
+    const int delta = dx + 2 * dy + 4 * dz;
+    long off = 0;
+#pragma unroll
+    for (int j = 1; j < 8; ++j)
+        if (delta == j) off = prm.spill_off[j];  // selects: no dynamic index into the argument
+    const long ex = dx ? prm.spill_ext[0][1] : prm.spill_ext[0][0];
+    const long ey = dy ? prm.spill_ext[1][1] : prm.spill_ext[1][0];
+    return off + cx + ex * (cy + ey * cz);
+}
+// entry (bx, by, bz) of the block, DOF (gx, gy, gz) = g of the mesh
+template <bool ONE, int P, int TX, int TY, int TZ, typename T>
+__device__ __forceinline__ void rhs3_put(const block_sink<T>& k, const rhs_params<T>& prm, int bx, int by, int bz,
+                                         int gx, int gy, int gz, long g, double sum) {
+    if constexpr (ONE) {
+        const int dx = bx >= TX && !k.last[0], dy = by >= TY && !k.last[1], dz = bz >= TZ && !k.last[2];
+        if (!(dx | dy | dz)) {
+            k.rhs[g] = prm.store ? static_cast<T>(sum) : k.rhs[g] + static_cast<T>(sum);
+        } else {
+            const int cx = dx ? (k.tile[0] + 1) * P + bx - TX : gx;
+            const int cy = dy ? (k.tile[1] + 1) * P + by - TY : gy;
+            const int cz = dz ? (k.tile[2] + 1) * P + bz - TZ : gz;
+            prm.spill[spill_index(prm, dx, dy, dz, cx, cy, cz)] = static_cast<T>(sum);
+        }
+    } else {
+        unsafeAtomicAdd(k.rhs + g, static_cast<T>(sum));
+    }
 }
 
 // The LDS data of one tile: its coefficient block (and k / f blocks) and the x and y tables
@@ -419,12 +492,10 @@ __device__ __forceinline__ void rhs3_fetch(rhs3_buf<T, P, TX, TY, TZ, KF, SRC>& 
 }
 
 // DOF plane k of the tile's block collects plane k - lz of the waves lz = k-P .. k; the sum
-// is added to the rhs with a return-less global add (the tiles of one colour share no DOF,
-// so each DOF has one writer per launch and the result does not depend on timing). With
-// `clear`, the planes are left zero for the next tile.
-template <typename T, int P, int TX, int TY, int TZ>
-__device__ __forceinline__ void rhs3_add_block(double* acc, tile3 o, T* __restrict__ rhs, int nx,
-                                               int ny, int nz, int tid, bool clear) {
+// goes to the rhs through the sink (rhs3_put).
+template <bool ONE, typename T, int P, int TX, int TY, int TZ>
+__device__ __forceinline__ void rhs3_add_block(const double* acc, tile3 o, const block_sink<T>& k,
+                                               const rhs_params<T>& prm, int nx, int ny, int nz, int tid) {
     constexpr int M = P + 1, BX = TX + P, BY = TY + P, PL = BX * BY, NB = PL * (TZ + P);
     constexpr int NT = TX * TY * TZ;
     for (int i = tid; i < NB; i += NT) {
@@ -434,19 +505,17 @@ __device__ __forceinline__ void rhs3_add_block(double* acc, tile3 o, T* __restri
 #pragma unroll
         for (int d = 0; d < M; ++d) {
             const int w = bz - d;
-            if (w >= 0 && w < TZ) {
-                sum += acc[(w * M + d) * PL + bxy];
-                if (clear) acc[(w * M + d) * PL + bxy] = 0.0;
-            }
+            if (w >= 0 && w < TZ) sum += acc[(w * M + d) * PL + bxy];
         }
         if (gx < nx && gy < ny && gz < nz)
-            unsafeAtomicAdd(rhs + (gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz)),
-                            static_cast<T>(sum));
+            rhs3_put<ONE, P, TX, TY, TZ>(k, prm, bxy % BX, bxy / BX, bz, gx, gy, gz,
+                                         gx + static_cast<long>(nx) * (gy + static_cast<long>(ny) * gz), sum);
     }
 }
 
-// One workgroup per tile (block b -> tile by XCD-contiguous numbering).
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM, int VAR>
+// One workgroup per tile (block b -> tile by XCD-contiguous numbering): tiles of one colour,
+// or (ONE) all ntx2 x nty2 x ntz tiles with the spill array prm.spill.
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM, int VAR, bool ONE>
 __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
     dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
@@ -462,7 +531,8 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD gets a
     // contiguous run of tiles (one slab of the domain), whose halos the next colour's launch
     // then finds in that XCD's L2.
-    const tile3 o = rhs3_tile<TX, TY, TZ>(xcd_contiguous(blockIdx.x, gridDim.x), colour, ntx2, nty2);
+    const int t = xcd_contiguous(blockIdx.x, gridDim.x);
+    const tile3 o = rhs3_tile_of<ONE, TX, TY, TZ>(t, colour, ntx2, nty2);
     // the tile's DOF block (first(e) = e for these knot vectors: dof origin = element origin)
     rhs3_stage(L, o, up, X, Y, Z.dofs, prm, tid);
     for (int i = tid; i < NA; i += NT) acc[i] = 0.0;
@@ -471,7 +541,8 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     rhs3_element<T, P, TX, TY, TZ, KF, SRC, FM, VAR>(L, acc, X, Y, Z, prm, lx, ly, lz, o.ex0,
                                                       o.ey0, o.ez0, R);
     __syncthreads();
-    rhs3_add_block<T, P, TX, TY, TZ>(acc, o, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
+    const block_sink<T> sink = rhs3_sink<ONE, TX, TY, TZ>(rhs, o, X.elements, Y.elements, Z.elements);
+    rhs3_add_block<ONE, T, P, TX, TY, TZ>(acc, o, sink, prm, X.dofs, Y.dofs, Z.dofs, tid);
 }
 
 // Variant 3: two tiles per workgroup (tiles 2w, 2w+1 of the colour for workgroup w after
@@ -516,8 +587,9 @@ __global__ __launch_bounds__(TX* TY* TZ, 2) void k_rhs3_pipe(
                                                         o1.ey0, o1.ez0, R1);
     }
     __syncthreads();
-    rhs3_add_block<T, P, TX, TY, TZ>(acc0, o0, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
-    if (two) rhs3_add_block<T, P, TX, TY, TZ>(acc1, o1, rhs, X.dofs, Y.dofs, Z.dofs, tid, false);
+    const block_sink<T> sink = rhs3_sink<false, TX, TY, TZ>(rhs, o0, 0, 0, 0);
+    rhs3_add_block<false, T, P, TX, TY, TZ>(acc0, o0, sink, prm, X.dofs, Y.dofs, Z.dofs, tid);
+    if (two) rhs3_add_block<false, T, P, TX, TY, TZ>(acc1, o1, sink, prm, X.dofs, Y.dofs, Z.dofs, tid);
 }
 
 // Variant 4 (p <= 2): tiles of TX x TY x TZ elements with TZ/2 waves, wave w integrating the
@@ -526,8 +598,9 @@ __global__ __launch_bounds__(TX* TY* TZ, 2) void k_rhs3_pipe(
 // the first only its lowest plane is added to LDS, after the second all P+1 planes — P+1
 // plane adds per two elements instead of 2(P+1) (a third fewer LDS adds at p = 2), and the
 // taller tile re-reads fewer halo planes. Wave w owns the LDS planes of DOF planes 2w ..
-// 2w+P+1; the block add and the colouring are those of the other variants.
-template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM>
+// 2w+P+1; the block add, the colouring and the one-launch tiling are those of the other
+// variants.
+template <typename T, int P, int TX, int TY, int TZ, bool KF, bool SRC, int FM, bool ONE>
 __global__ __launch_bounds__(TX* TY*(TZ / 2), 2) void k_rhs3_zpair(
     const T* __restrict__ up, T* __restrict__ rhs, dev_tables<T> X, dev_tables<T> Y,
     dev_tables<T> Z, rhs_params<T> prm, int colour, int ntx2, int nty2) {
@@ -543,7 +616,8 @@ __global__ __launch_bounds__(TX* TY*(TZ / 2), 2) void k_rhs3_zpair(
     const int tid = threadIdx.x;
     const int lx = tid % TX, ly = (tid / TX) % TY;
     const int w = __builtin_amdgcn_readfirstlane(tid / 64);
-    const tile3 o = rhs3_tile<TX, TY, TZ>(xcd_contiguous(blockIdx.x, gridDim.x), colour, ntx2, nty2);
+    const int t = xcd_contiguous(blockIdx.x, gridDim.x);
+    const tile3 o = rhs3_tile_of<ONE, TX, TY, TZ>(t, colour, ntx2, nty2);
     rhs3_stage<NT>(S.b, o, up, X, Y, Z.dofs, prm, tid);
     for (int i = tid; i < NW * WP * PL; i += NT) S.acc[i] = 0.0;
     __syncthreads();
@@ -575,6 +649,7 @@ __global__ __launch_bounds__(TX* TY*(TZ / 2), 2) void k_rhs3_zpair(
                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
     __syncthreads();
     // DOF plane k of the block collects plane k - 2w of the waves w with 0 <= k - 2w <= P+1
+    const block_sink<T> sink = rhs3_sink<ONE, TX, TY, TZ>(rhs, o, X.elements, Y.elements, Z.elements);
     for (int i = tid; i < NB; i += NT) {
         const int bxy = i % PL, bz = i / PL;
         const int gx = o.ex0 + bxy % BX, gy = o.ey0 + bxy / BX, gz = o.ez0 + bz;
@@ -585,9 +660,40 @@ __global__ __launch_bounds__(TX* TY*(TZ / 2), 2) void k_rhs3_zpair(
             if (pl >= 0 && pl < WP) sum += S.acc[(ww * WP + pl) * PL + bxy];
         }
         if (gx < X.dofs && gy < Y.dofs && gz < Z.dofs)
-            unsafeAtomicAdd(rhs + (gx + static_cast<long>(X.dofs) * (gy + static_cast<long>(Y.dofs) * gz)),
-                            static_cast<T>(sum));
+            rhs3_put<ONE, P, TX, TY, TZ>(sink, prm, bxy % BX, bxy / BX, bz, gx, gy, gz,
+                                         gx + static_cast<long>(X.dofs) * (gy + static_cast<long>(Y.dofs) * gz), sum);
     }
+}
+
+// The one-launch tiling's second kernel (tiles of TX x TY x TZ elements, ntx x nty x ntz of
+// them): DOF g lies in tile t_d = min(g_d / T_d, nt_d - 1) at local index l_d = g_d - T_d t_d
+// of each direction; the tile t_d - 1 holds an entry for it (in its top P layers) when
+// l_d < P and t_d > 0. Those entries of the (up to 7) earlier tiles are added in a fixed
+// order (delta = 1, ..., 7: the x neighbour first).
+template <typename T, int P, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_rhs3_spill(T* __restrict__ rhs, rhs_params<T> prm, int nx, int ny,
+                                                    int nz, int ntx, int nty, int ntz) {
+    const long g = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (g >= static_cast<long>(nx) * ny * nz) return;
+    const int gx = static_cast<int>(g % nx);
+    const long r = g / nx;
+    const int gy = static_cast<int>(r % ny), gz = static_cast<int>(r / ny);
+    const int tx = min(gx / TX, ntx - 1), ty = min(gy / TY, nty - 1), tz = min(gz / TZ, ntz - 1);
+    const int lx = gx - TX * tx, ly = gy - TY * ty, lz = gz - TZ * tz;
+    const bool sx = lx < P && tx > 0, sy = ly < P && ty > 0, sz = lz < P && tz > 0;
+    if (!(sx || sy || sz)) return;
+    const T* __restrict__ spill = prm.spill;
+    double sum = 0;
+#pragma unroll
+    for (int d = 1; d < 8; ++d) {
+        const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+        if ((dx && !sx) || (dy && !sy) || (dz && !sz)) continue;
+        const long i = prm.spill_off[d] + (dx ? tx * P + lx : gx) +
+                       static_cast<long>(prm.spill_ext[0][dx]) *
+                           ((dy ? ty * P + ly : gy) + static_cast<long>(prm.spill_ext[1][dy]) * (dz ? tz * P + lz : gz));
+        sum += static_cast<double>(spill[i]);
+    }
+    rhs[g] += static_cast<T>(sum);
 }
 
 // ------------------------------------------------------------------------------------
@@ -808,6 +914,12 @@ struct tile<3, 3> {
     static constexpr int X = 8, Y = 8, Z = 3;
 };
 
+// element layers per tile of a variant: variant 4 (p <= 2) uses tiles twice as tall
+template <int P, int VAR>
+constexpr int rhs3_tile_z() {
+    return VAR == 4 && P <= 2 ? 2 * tile<3, P>::Z : tile<3, P>::Z;
+}
+
 template <typename T, int P, bool KF, bool SRC, int FM, int VAR>
 hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                         hipStream_t s) {
@@ -827,7 +939,7 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
             const int ntz2 = (d[2].elements + TZ2 - 1) / TZ2, nz22 = (ntz2 - cz + 1) / 2;
             const long tiles2 = static_cast<long>(nx2) * ny2 * nz22;
             if (tiles2 > 0)
-                hipLaunchKernelGGL((k_rhs3_zpair<T, P, TL::X, TL::Y, TZ2, KF, SRC, FM>), dim3(tiles2),
+                hipLaunchKernelGGL((k_rhs3_zpair<T, P, TL::X, TL::Y, TZ2, KF, SRC, FM, false>), dim3(tiles2),
                                    dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm, c, nx2, ny2);
         } else if constexpr (VAR == 3) {
             hipLaunchKernelGGL((k_rhs3_pipe<T, P, TL::X, TL::Y, TL::Z, KF, SRC, FM>),
@@ -836,13 +948,76 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
         } else {
             // variant 4 at p = 3 (its register block would spill) runs as variant 2
             constexpr int V = VAR == 4 ? 2 : VAR;
-            hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, FM, V>), dim3(tiles),
+            hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TL::Z, KF, SRC, FM, V, false>), dim3(tiles),
                                dim3(NT), 0, s, up, rhs, d[0], d[1], d[2], prm, c, nx2, ny2);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// The spill arrays of n[d] DOFs in nt[d] tiles per direction: array delta (1..7) has
+// extent n[d] in the directions with delta_d = 0 and nt[d] P in the others, at off[delta];
+// returns the total.
+template <int P>
+long spill_layout(const int n[3], const int nt[3], long off[8], int ext[3][2]) {
+    for (int d = 0; d < 3; ++d) {
+        ext[d][0] = n[d];
+        ext[d][1] = nt[d] * P;
+    }
+    long total = 0;
+    off[0] = 0;
+    for (int delta = 1; delta < 8; ++delta) {
+        off[delta] = total;
+        total += static_cast<long>(ext[0][delta & 1]) * ext[1][(delta >> 1) & 1] * ext[2][delta >> 2];
+    }
+    return total;
+}
+
+// One launch over all tiles (variants 0, 1, 2 and 4; 4 at p = 3 runs as 2), then the spill
+// fix-up over the DOFs.
+template <typename T, int P, bool KF, bool SRC, int FM, int VAR>
+hipError_t rhs3_one(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm, hipStream_t s) {
+    using TL = tile<3, P>;
+    constexpr int V = VAR == 4 && P == 3 ? 2 : VAR, TZ = rhs3_tile_z<P, V>();
+    static_assert(V != 3 && V != 5, "variants 3 and 5 have no one-launch form");
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X;
+    const int nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    const int ntz = (d[2].elements + TZ - 1) / TZ;
+    const long tiles = static_cast<long>(ntx) * nty * ntz;
+    if (tiles == 0) return hipSuccess;
+    if (!prm.spill) return hipErrorInvalidValue;
+    // the spill arrays (sized by rhs3_spill_slots for dofs = elements + P)
+    rhs_params<T> q = prm;
+    const int n[3] = {d[0].dofs, d[1].dofs, d[2].dofs}, nt[3] = {ntx, nty, ntz};
+    for (int i = 0; i < 3; ++i)
+        if (n[i] != d[i].elements + P) return hipErrorInvalidValue;
+    spill_layout<P>(n, nt, q.spill_off, q.spill_ext);
+    if constexpr (V == 4)
+        hipLaunchKernelGGL((k_rhs3_zpair<T, P, TL::X, TL::Y, TZ, KF, SRC, FM, true>), dim3(tiles),
+                           dim3(TL::X * TL::Y * TZ / 2), 0, s, up, rhs, d[0], d[1], d[2], q, 0, ntx, nty);
+    else
+        hipLaunchKernelGGL((k_rhs3<T, P, TL::X, TL::Y, TZ, KF, SRC, FM, V, true>), dim3(tiles),
+                           dim3(TL::X * TL::Y * TZ), 0, s, up, rhs, d[0], d[1], d[2], q, 0, ntx, nty);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const long N = static_cast<long>(d[0].dofs) * d[1].dofs * d[2].dofs;
+    hipLaunchKernelGGL((k_rhs3_spill<T, P, TL::X, TL::Y, TZ>), dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256),
+                       0, s, rhs, q, d[0].dofs, d[1].dofs, d[2].dofs, ntx, nty, ntz);
+    return hipGetLastError();
+}
+
+// spill values of one launch of a variant (dofs = elements + P)
+template <int P, int VAR>
+long rhs3_one_slots(const int elements[3]) {
+    using TL = tile<3, P>;
+    constexpr int TZ = rhs3_tile_z<P, VAR>();
+    const int n[3] = {elements[0] + P, elements[1] + P, elements[2] + P};
+    const int nt[3] = {(elements[0] + TL::X - 1) / TL::X, (elements[1] + TL::Y - 1) / TL::Y, (elements[2] + TZ - 1) / TZ};
+    long off[8];
+    int ext[3][2];
+    return spill_layout<P>(n, nt, off, ext);
 }
 
 template <typename T, int P, bool KF, bool SRC>
@@ -876,7 +1051,15 @@ hipError_t rhs3_forms(const T* up, T* rhs, const dev_tables<T>* d, const rhs_par
 
 template <typename T, int P, int VAR>
 hipError_t rhs3_fields(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
-                       hipStream_t s) {
+                       bool one, hipStream_t s) {
+    // one launch (forms without a k field or a source in the kernel; checked by the caller)
+    if constexpr (VAR != 3) {
+        if (one) {
+            if (prm.reaction != T(0)) return rhs3_one<T, P, false, false, 1, VAR>(up, rhs, d, prm, s);
+            if (prm.mass == T(0)) return rhs3_one<T, P, false, false, 2, VAR>(up, rhs, d, prm, s);
+            return rhs3_one<T, P, false, false, 0, VAR>(up, rhs, d, prm, s);
+        }
+    }
     if (prm.reaction != T(0)) return rhs3_forms<T, P, 1, VAR>(up, rhs, d, prm, s);
     if (prm.mass == T(0) && !prm.kfield && !prm.source)
         return rhs3_colours<T, P, false, false, 2, VAR>(up, rhs, d, prm, s);
@@ -885,21 +1068,30 @@ hipError_t rhs3_fields(const T* up, T* rhs, const dev_tables<T>* d, const rhs_pa
 
 template <typename T, int P>
 hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
-                      const rhs_params<T>& prm, int code, hipStream_t s) {
+                      const rhs_params<T>& prm_in, int code, hipStream_t s) {
+    rhs_params<T> prm = prm_in;
     const bool kf = prm.kfield != nullptr, src = prm.source != nullptr;
-    int variant = code & 15;  // bits 4 and up: the shape of variant 5
+    // bits 4..7: the shape of variant 5; bit 8: one launch; bit 9: overwrite
+    int variant = code & 15;
+    const bool overwrite = (code & kRhsOverwrite) != 0;
+    if (dim == 3 && variant == 5 && !rhs3_col_supported<T>(P, prm)) variant = P == 3 ? 2 : 1;
+    // element columns on the matrix cores where the form allows, else the degree's default
+    // per-element variant
+    const bool one = dim == 3 && (code & kRhsOneLaunch) != 0 && variant != 3 && variant != 5 && prm.spill &&
+                     !kf && !src;
+    prm.store = one && overwrite;
+    if (overwrite && !one) {
+        const long n = static_cast<long>(d[0].dofs) * d[1].dofs * (dim == 3 ? d[2].dofs : 1);
+        const hipError_t e = launch_fill<T>(rhs, n, T(0), s);
+        if (e != hipSuccess) return e;
+    }
     if (dim == 3) {
-        if (variant == 5) {
-            // element columns on the matrix cores where the form allows, else the degree's
-            // default per-element variant
-            if (rhs3_col_supported<T>(P, prm)) return launch_rhs3_col<T>(P, up, rhs, d, prm, code >> 4, s);
-            variant = P == 3 ? 2 : 1;
-        }
-        if (variant == 1) return rhs3_fields<T, P, 1>(up, rhs, d, prm, s);
-        if (variant == 2) return rhs3_fields<T, P, 2>(up, rhs, d, prm, s);
-        if (variant == 3) return rhs3_fields<T, P, 3>(up, rhs, d, prm, s);
-        if (variant == 4) return rhs3_fields<T, P, 4>(up, rhs, d, prm, s);
-        return rhs3_fields<T, P, 0>(up, rhs, d, prm, s);
+        if (variant == 5) return launch_rhs3_col<T>(P, up, rhs, d, prm, (code >> 4) & 15, s);
+        if (variant == 1) return rhs3_fields<T, P, 1>(up, rhs, d, prm, one, s);
+        if (variant == 2) return rhs3_fields<T, P, 2>(up, rhs, d, prm, one, s);
+        if (variant == 3) return rhs3_fields<T, P, 3>(up, rhs, d, prm, one, s);
+        if (variant == 4) return rhs3_fields<T, P, 4>(up, rhs, d, prm, one, s);
+        return rhs3_fields<T, P, 0>(up, rhs, d, prm, one, s);
     }
     if (kf && src) return rhs2_colours<T, P, true, true>(up, rhs, d, prm, s);
     if (kf) return rhs2_colours<T, P, true, false>(up, rhs, d, prm, s);
@@ -921,6 +1113,19 @@ hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<
             return rhs_fields<T, 3>(dim, u_prev, rhs, dirs, prm, variant, s);
         default:
             return hipErrorInvalidValue;
+    }
+}
+
+long rhs3_spill_slots(int p, const int elements[3]) {
+    switch (p) {
+        case 1:
+            return std::max({rhs3_one_slots<1, 1>(elements), rhs3_one_slots<1, 4>(elements)});
+        case 2:
+            return std::max({rhs3_one_slots<2, 1>(elements), rhs3_one_slots<2, 4>(elements)});
+        case 3:
+            return rhs3_one_slots<3, 1>(elements);
+        default:
+            return 0;
     }
 }
 

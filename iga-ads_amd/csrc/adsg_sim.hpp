@@ -127,6 +127,10 @@ struct adsg_sim {
     int rhs_col_shape = 0;  // variant 5: columns per wave and chunks per z-range (ADSG_TUNE_RHS_COLUMNS)
     int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
     int rhs_kron = 0;       // 1: separable integrands through the 1D matrices (adsg_set_tuning)
+    // 1: 3D element kernels in one launch over all tiles, the entries a tile does not own
+    // spilled to `spill` (rhs3_spill_slots values in the step precision) and added after
+    int rhs_tiling = 0;
+    void* spill = nullptr;
     std::string err;
     hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
     double* norm_scratch = nullptr;  // adsg_norm: block partial sums and the total

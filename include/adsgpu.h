@@ -294,7 +294,8 @@ int adsg_check_finite(adsg_sim* h, int which);
  * the handle's stream: the counterpart of the reference's StatTimer around compute_rhs
  * (examples/scalability/test3d.hpp:67-101). Averages per step in milliseconds: ms[0] the
  * element-loop kernels (2^dim colour launches), ms[1] boundary handling, ms[2], ms[3],
- * ms[4] line sweeps in x, y, z, ms[5] the whole step, ms[6] the zero fill of the rhs. */
+ * ms[4] line sweeps in x, y, z, ms[5] the whole step, ms[6] the start of the rhs (a
+ * source's load vector scaled in; the zero fill, where one is needed, is part of ms[0]). */
 int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots, int flags,
                      double* ms);
 
@@ -327,6 +328,13 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
  * of elements along z, the x-y contractions on the matrix cores): bit 0 two columns per
  * wave, bit 1 three chunks of 16 DOF planes per workgroup; the autotuner times all four. */
 #define ADSG_TUNE_RHS_COLUMNS 4
+/* ADSG_TUNE_RHS_TILING (3D, variants 0, 1, 2 and 4, forms without a k field): 0 = 2^3
+ * parity-colour launches, whose tiles share no DOF; 1 = one launch over all tiles, each tile
+ * adding the entries of its DOF block that it owns and storing the others in a spill array
+ * (allocated when the value is first set, a few values per element), which a second kernel
+ * adds in a fixed order. Both are deterministic; the sums are grouped differently (equal to
+ * rounding). The autotuner times both. */
+#define ADSG_TUNE_RHS_TILING 5
 int adsg_set_tuning(adsg_sim* h, int key, int value);
 int adsg_get_tuning(const adsg_sim* h, int key);
 int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags);

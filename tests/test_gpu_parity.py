@@ -247,8 +247,10 @@ def test_config2_full_size_one_step_and_conservation():
     g.set(ag.U, u0)
     g.step(6, form)
     assert np.array_equal(g.get(ag.U), x6)
-    # the pipelined element variant gives the same bits as variant 1 at full size, and the
-    # layer-pair variant (the default at this size) the oracle's step
+    # in colour launches the pipelined element variant gives the same bits as variant 1 at
+    # full size, and the layer-pair variant the oracle's step (the default, the layer-pair
+    # variant in one launch, is checked above)
+    g.set_tuning(ag.TUNE_RHS_TILING, 0)
     for v in (1, 3, 4):
         g.set_tuning(ag.TUNE_RHS_VARIANT, v)
         g.set(ag.U, u0)
@@ -394,8 +396,10 @@ def test_each_variant_runs_its_own_kernel():
     the same order (bitwise equal), while variants 2, 4 and 5 sum in other orders, so on a
     random state their rhs differ from variant 1 in the last bits -- which they could not if
     every choice ran one kernel. The column shape set for variant 5 must not change the
-    kernel the other variants run."""
+    kernel the other variants run. (Colour launches: the one-launch tiling groups the sums
+    differently again.)"""
     g, _ = _pair(3, 2, (20, 17, 19))
+    g.set_tuning(ag.TUNE_RHS_TILING, 0)
     g.set(ag.U_PREV, np.random.default_rng(23).standard_normal(g.size))
     g.set_tuning(ag.TUNE_RHS_COLUMNS, 1)
     out = {}
@@ -415,9 +419,10 @@ def test_pipelined_rhs_variant_is_bitwise_its_single_tile_variant(p, ne, same_as
     """Variant 3 (two tiles per workgroup, the second copied to LDS during the first's
     integration) adds the same products in the same order as the single-tile variant with
     the same register scheme, so in FP64 the two rhs agree bit for bit, for every form;
-    odd tile counts per colour leave workgroups with one tile."""
+    odd tile counts per colour leave workgroups with one tile (colour launches for both)."""
     _log(f"case: pipelined rhs p={p} ne={ne}")
     g, _ = _pair(3, p, ne)
+    g.set_tuning(ag.TUNE_RHS_TILING, 0)
     rng = np.random.default_rng(5)
     g.set(ag.U_PREV, rng.uniform(0, 1, g.size))
     g.set(ag.KFIELD, np.exp(rng.uniform(-0.5, 0.5, g.size)))
