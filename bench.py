@@ -242,9 +242,10 @@ def ensemble_block(ag, np, count, steps, dt):
         res[name] = {"value": dofs * steps / t, "ms_per_step": 1e3 * t / steps,
                      "setup_ms": {"create": 1e3 * (tc1 - tc0), "prepare": 1e3 * (tc2 - tc1),
                                   "in_steps": (tc2 - tc0) / (t / steps),
-                                  "note": ("host tabulation, 1D assembly, FP64 banded LU and "
-                                           "uploads of every problem (adsg_batch_create, "
-                                           "adsg_batch_prepare)")},
+                                  "note": ("every problem's basis tables (adsg_batch_create), Gram "
+                                           "bands, FP64 band LU and packed sweep rows "
+                                           "(adsg_batch_prepare), computed on the device; the "
+                                           "host lays out the arenas and uploads their descriptors")},
                      "tile_shape": ["16x16", "8x8"][tile], "line_sweeps": lines,
                      "autotune_ms_per_step": list(map(float, tile_ms)),
                      "element_tflops": flops * steps / t / 1e12}

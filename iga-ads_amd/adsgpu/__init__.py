@@ -492,8 +492,9 @@ class Batch:
     """Many independent 2D problems (degree 1..3, natural boundaries) advanced together with
     grouped launches and one captured graph per step count (adsg_batch_*)."""
 
-    def __init__(self, problems, precision=FP32):
-        """problems: list of (p, elements) pairs, square meshes on [0,1]^2."""
+    def __init__(self, problems, precision=FP32, interval=(0.0, 1.0)):
+        """problems: list of (p, elements) pairs, square meshes on interval^2 ([0,1]^2 by
+        default)."""
         L = lib()
         cfgs = (Config * len(problems))()
         for c, (p, ne) in zip(cfgs, problems):
@@ -501,7 +502,7 @@ class Batch:
             for dc in (c.x, c.y):
                 dc.p = p
                 dc.elements = ne
-                dc.a, dc.b = 0.0, 1.0
+                dc.a, dc.b = float(interval[0]), float(interval[1])
                 dc.quad_order = 0
                 dc.repeated_nodes = 0
             c.derivatives = 1

@@ -72,14 +72,10 @@ int ensure_spill(adsg_sim* h) {
     return ADSG_OK;
 }
 
-// a device copy of bytes at *dst (freed and replaced if set), or staged for a batch's arena
+// a device copy of bytes at *dst (freed and replaced if set)
 int upload_bytes(adsg_sim* h, void** dst, const void* data, size_t bytes) {
     if (*dst && !h->in_arena) (void)hipFree(*dst);
     *dst = nullptr;
-    if (h->stage) {
-        h->stage->add(dst, data, bytes);
-        return ADSG_OK;
-    }
     if (hipMalloc(dst, bytes) != hipSuccess) return fail(h, ADSG_ENOMEM, "hipMalloc failed for tables");
     HIP_TRY(h, hipMemcpy(*dst, data, bytes, hipMemcpyHostToDevice));
     return ADSG_OK;
@@ -135,14 +131,11 @@ int upload_tables(adsg_sim* h, dev_dir& d) {
         if ((rc = upload<double>(h, &d.wj, d.tab.wj))) return rc;
     }
     // form_matrix.cpp:8-42 on the same tables: the Gram and stiffness matrices, unfixed, in
-    // FP64 in both precisions (the separable rhs contracts in FP64); a batch's problems have
-    // no separable rhs and skip them
-    if (!h->in_arena) {
-        const std::vector<double> M = band_rows(adsg::band_form(d.tab, 1.0, 0.0), d.tab.dofs, d.tab.p);
-        const std::vector<double> K = band_rows(adsg::band_form(d.tab, 0.0, 1.0), d.tab.dofs, d.tab.p);
-        if ((rc = upload<double>(h, &d.kM, M))) return rc;
-        if ((rc = upload<double>(h, &d.kK, K))) return rc;
-    }
+    // FP64 in both precisions (the separable rhs contracts in FP64)
+    const std::vector<double> M = band_rows(adsg::band_form(d.tab, 1.0, 0.0), d.tab.dofs, d.tab.p);
+    const std::vector<double> K = band_rows(adsg::band_form(d.tab, 0.0, 1.0), d.tab.dofs, d.tab.p);
+    if ((rc = upload<double>(h, &d.kM, M))) return rc;
+    if ((rc = upload<double>(h, &d.kK, K))) return rc;
     // first DOF of every element
     return upload_bytes(h, reinterpret_cast<void**>(&d.first), d.tab.first.data(),
                         sizeof(int) * std::max<size_t>(d.tab.first.size(), 1));
@@ -797,7 +790,7 @@ int run_graphs(adsg_sim* h, step_kind kind, int nsteps, const adsg_rhs_form* for
 
 // the handle of adsg_create; with a shared stream (not owned) and without state vectors for
 // the problems of a batch, whose vectors live in the batch's arenas
-int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, adsg_sim** out, stager* stage) {
+int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, adsg_sim** out) {
     if (!cfg || !out) {
         g_create_error = "adsg_create: null argument";
         return ADSG_EINVAL;
@@ -864,8 +857,6 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
         delete h;
         return ADSG_EHIP;
     }
-    h->stage = stage;  // a batch stages the tables into its arena
-    h->in_arena = stage != nullptr;
     for (int d = 0; d < h->dim; ++d) {
         if (upload_tables(h, h->dirs[d])) {
             g_create_error = "adsg_create: " + h->err;
@@ -873,7 +864,6 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
             return ADSG_EHIP;
         }
     }
-    h->stage = nullptr;
     for (int v = ADSG_U; v <= ADSG_BUFFER && vectors; ++v) {
         if (hipMalloc(&h->vec[v], h->N * elt_of(h, v)) != hipSuccess) {
             g_create_error = "adsg_create: device allocation of the state vectors failed";

@@ -6,22 +6,21 @@
 // arena, per degree p and parity colour one element-kernel launch whose blocks cover the
 // tiles of every problem of that degree, per degree one x-sweep and one y-sweep launch over
 // all their lines, and one elementwise update of the whole arena. The step sequences are
-// replayed from captured HIP graphs of at most 50 steps. The problems' host set-up (tables,
-// band LU, segment coefficients) runs on worker threads and stages its uploads, which then
-// go to the device as one arena per phase (create, prepare) with one copy each.
+// replayed from captured HIP graphs of at most 50 steps. The problems' set-up runs on the
+// device (adsg_batch_setup.hip): create tabulates every direction's basis in one launch,
+// prepare assembles the Gram bands, factors them and packs the sweep rows in two more; the
+// host only lays out the arenas and uploads their descriptors (one copy per phase).
 #include <hip/hip_runtime.h>
 
-#include <atomic>
-#include <thread>
-
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <map>
 #include <string>
-#include <thread>
 #include <utility>
 #include <vector>
 
+#include "adsg_batch_setup.hpp"
 #include "adsg_sim.hpp"
 
 namespace {
@@ -65,31 +64,17 @@ struct adsg_batch {
     int shape = 0;  // element tile shape: 0 = 16 x 16, 1 = 8 x 8
     int sweep = 0;  // 0: segmented line sweeps where every problem allows, 1: one lane per line
     std::vector<batch_group> groups[2];
+    void* items = nullptr;  // the launch items of every group, one allocation
     void* packs = nullptr;  // packed sweep rows of every problem and direction (step precision)
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (nsteps, phase at start)
-    // every problem's tables (create) and factors (prepare), each set in one device arena
+    // every problem's tables (create) and factors (prepare), each set in one device arena,
+    // laid out by the descriptors of its directions (problem i: 2i x, 2i + 1 y)
     void* arena[2] = {nullptr, nullptr};
+    std::vector<adsg::bs_dir> bs;
     std::string err;
 };
 
 namespace {
-
-// f(0) .. f(count - 1) on up to 16 host threads (the problems' set-up is independent)
-template <typename F>
-void for_each_problem(int count, F&& f) {
-    const int hw = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
-    const int nt = std::max(1, std::min({16, hw, count}));
-    int dev = 0;
-    (void)hipGetDevice(&dev);  // the workers use the caller's device (HIP's is per thread)
-    std::atomic<int> next{0};
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t)
-        pool.emplace_back([&] {
-            (void)hipSetDevice(dev);
-            for (int i = next++; i < count; i = next++) f(i);
-        });
-    for (auto& th : pool) th.join();
-}
 
 int bfail(adsg_batch* b, int code, const std::string& msg) {
     if (b) b->err = msg;
@@ -101,14 +86,9 @@ int bhip(adsg_batch* b, hipError_t e, const char* what) {
 }
 
 void free_groups(adsg_batch* b) {
-    for (auto& gs : b->groups) {
-        for (auto& g : gs)
-            for (void* p : {g.rhs_items, g.sx_items, g.sy_items, g.gx_items, g.gy_items})
-                if (p) (void)hipFree(p);
-        gs.clear();
-    }
-    if (b->packs) (void)hipFree(b->packs);
-    b->packs = nullptr;
+    for (auto& gs : b->groups) gs.clear();
+    if (b->items) (void)hipFree(b->items);
+    b->items = nullptr;
 }
 
 void drop_batch_graphs(adsg_batch* b) {
@@ -142,45 +122,28 @@ adsg::dev_factors<T> factors_of(const adsg_sim* h, int d) {
     return f;
 }
 
-template <typename T>
-int upload_items(adsg_batch* b, void** dst, const std::vector<T>& items) {
-    if (items.empty()) return ADSG_OK;
-    if (hipMalloc(dst, sizeof(T) * items.size()) != hipSuccess)
-        return bfail(b, ADSG_ENOMEM, "batch: no memory for the launch items");
-    hipError_t e = hipMemcpy(*dst, items.data(), sizeof(T) * items.size(), hipMemcpyHostToDevice);
-    return e == hipSuccess ? ADSG_OK : bhip(b, e, "batch: uploading launch items");
-}
+// Host bytes of the launch items of every group, copied to the device in one allocation:
+// add() appends an item array (256-byte aligned) and returns its byte offset.
+struct item_stage {
+    std::vector<char> bytes;
+    template <typename T>
+    size_t add(const std::vector<T>& items) {
+        const size_t o = (bytes.size() + 255) / 256 * 256;
+        bytes.resize(o + sizeof(T) * items.size());
+        if (!items.empty()) std::memcpy(bytes.data() + o, items.data(), sizeof(T) * items.size());
+        return o;
+    }
+};
 
 // launch items of every degree for both arena parities (T: the step precision)
 template <typename T>
 int build_groups(adsg_batch* b) {
     free_groups(b);
     drop_batch_graphs(b);
-    // batched segmentation of every problem's two Gram matrices, packed rows in one arena
-    const size_t P = b->sims.size();
-    std::vector<adsg::line_segments> segs(2 * P);
-    std::vector<size_t> pack_off(2 * P, 0);
-    bool all_seg = true;
-    size_t total_pack = 0;
-    for (size_t i = 0; i < P && all_seg; ++i)
-        for (int d = 0; d < 2; ++d) {
-            auto& sg = segs[2 * i + d];
-            if (!adsg::segment_lines(b->sims[i]->dirs[d].slots[0].lf, sg, 0, adsg::kBatchSegments)) {
-                all_seg = false;
-                break;
-            }
-            pack_off[2 * i + d] = total_pack;
-            total_pack += sg.pack.size();
-        }
-    if (all_seg) {
-        std::vector<T> arena(std::max<size_t>(total_pack, 1));
-        for (size_t k = 0; k < 2 * P; ++k)
-            for (size_t j = 0; j < segs[k].pack.size(); ++j) arena[pack_off[k] + j] = static_cast<T>(segs[k].pack[j]);
-        if (hipMalloc(&b->packs, sizeof(T) * arena.size()) != hipSuccess)
-            return bfail(b, ADSG_ENOMEM, "batch: no memory for the sweep rows");
-        hipError_t e = hipMemcpy(b->packs, arena.data(), sizeof(T) * arena.size(), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return bhip(b, e, "batch: uploading sweep rows");
-    }
+    // the packed sweep rows come from prepare (every direction segmented, or none)
+    const bool all_seg = b->packs != nullptr;
+    item_stage stage;
+    std::vector<std::array<size_t, 5>> offs;  // per group, in the order of b->groups[0], [1]
     for (int ph = 0; ph < 2; ++ph) {
         for (int p = 1; p <= 3; ++p) {
             std::vector<adsg::rhs2_item<T>> ri;
@@ -238,21 +201,21 @@ int build_groups(adsg_batch* b) {
                 sy.push_back(yi);
                 if (all_seg) {
                     adsg::seg_item<T> a, c;
-                    const auto& sgx = segs[2 * i];
-                    const auto& sgy = segs[2 * i + 1];
+                    const adsg::bs_dir& sgx = b->bs[2 * i];
+                    const adsg::bs_dir& sgy = b->bs[2 * i + 1];
                     a.u = c.u = target;
                     a.g.n_lines = ny;  // x line l starts at l * nx
                     a.g.A = 1;
                     a.g.B = nx;
                     a.g.S = 1;
-                    a.pack = static_cast<const T*>(b->packs) + pack_off[2 * i];
+                    a.pack = static_cast<const T*>(sgx.pack);
                     a.n = static_cast<int>(nx);
                     a.m = sgx.m;
                     a.nseg = sgx.nseg;
                     a.first = g.gx_blocks;
                     g.gx_blocks += static_cast<int>((ny + 63) / 64);
                     c.g = yi.g;
-                    c.pack = static_cast<const T*>(b->packs) + pack_off[2 * i + 1];
+                    c.pack = static_cast<const T*>(sgy.pack);
                     c.n = static_cast<int>(ny);
                     c.m = sgy.m;
                     c.nseg = sgy.nseg;
@@ -264,19 +227,29 @@ int build_groups(adsg_batch* b) {
             }
             if (ri.empty()) continue;
             g.nitems = static_cast<int>(ri.size());
+            g.seg = all_seg;
+            // byte offsets of the group's item arrays, turned into device pointers below
+            offs.push_back({stage.add(ri), stage.add(sx), stage.add(sy), all_seg ? stage.add(gx) : 0,
+                            all_seg ? stage.add(gy) : 0});
             b->groups[ph].push_back(g);
-            batch_group& G = b->groups[ph].back();
-            int rc;
-            if ((rc = upload_items(b, &G.rhs_items, ri))) return rc;
-            if ((rc = upload_items(b, &G.sx_items, sx))) return rc;
-            if ((rc = upload_items(b, &G.sy_items, sy))) return rc;
-            G.seg = all_seg;
-            if (all_seg) {
-                if ((rc = upload_items(b, &G.gx_items, gx))) return rc;
-                if ((rc = upload_items(b, &G.gy_items, gy))) return rc;
-            }
         }
     }
+    if (stage.bytes.empty()) return ADSG_OK;
+    if (hipMalloc(&b->items, stage.bytes.size()) != hipSuccess)
+        return bfail(b, ADSG_ENOMEM, "batch: no memory for the launch items");
+    hipError_t e = hipMemcpy(b->items, stage.bytes.data(), stage.bytes.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bhip(b, e, "batch: uploading launch items");
+    char* base = static_cast<char*>(b->items);
+    size_t k = 0;
+    for (auto& gs : b->groups)
+        for (batch_group& g : gs) {
+            const auto& o = offs[k++];
+            g.rhs_items = base + o[0];
+            g.sx_items = base + o[1];
+            g.sy_items = base + o[2];
+            g.gx_items = g.seg ? base + o[3] : nullptr;
+            g.gy_items = g.seg ? base + o[4] : nullptr;
+        }
     return ADSG_OK;
 }
 
@@ -381,6 +354,180 @@ int run_batch_chunk(adsg_batch* b, int nsteps) {
     return ADSG_OK;
 }
 
+// A problem of the batch: the handle without state vectors, tables or factors of its own
+// (they are laid out in the batch's arenas by the device set-up below).
+adsg_sim* batch_sim(const adsg_config& c, hipStream_t stream, int device) {
+    auto* h = new adsg_sim;
+    h->dim = 2;
+    h->precision = c.precision;
+    h->step_count = c.step_count;
+    h->dt = c.dt;
+    h->device = device;
+    h->stream = stream;
+    h->owns_stream = false;
+    h->in_arena = true;
+    const adsg_dim_config* dc[2] = {&c.x, &c.y};
+    h->N = 1;
+    for (int d = 0; d < 2; ++d) {
+        adsg::direction& t = h->dirs[d].tab;
+        t.p = dc[d]->p;
+        t.elements = dc[d]->elements;
+        t.dofs = t.elements + t.p;  // clamped knots, simple interior knots
+        t.nq = t.p + 1;
+        t.a = dc[d]->a;
+        t.b = dc[d]->b;
+        h->N *= t.dofs;
+        adsg::dev_slot s0;
+        s0.set = true;
+        s0.kl = s0.ku = t.p;
+        h->dirs[d].slots.push_back(std::move(s0));
+    }
+    return h;
+}
+
+// byte layout of an arena: take() returns the 256-aligned offset of the next `bytes`
+struct arena_layout {
+    size_t size = 0;
+    size_t take(size_t bytes) {
+        const size_t o = size;
+        size += (bytes + 255) / 256 * 256;
+        return o;
+    }
+};
+
+// Gauss-Legendre rules of 2, 3 and 4 points (p = 1..3), nodes then weights, at
+// kGaussOff[p] in one array of 18 values
+constexpr int kGaussOff[4] = {0, 0, 4, 10};
+
+// create: arena 0 = [Gauss rules | descriptors | every direction's val, der, wj (T)], one
+// copy of the head, one launch for the tables
+template <typename T>
+int batch_tables(adsg_batch* b) {
+    const int count = static_cast<int>(2 * b->sims.size());
+    b->bs.assign(count, adsg::bs_dir{});
+    arena_layout L;
+    const size_t g_off = L.take(18 * sizeof(double));
+    const size_t d_off = L.take(count * sizeof(adsg::bs_dir));
+    std::vector<std::array<size_t, 3>> to(count);
+    for (int k = 0; k < count; ++k) {
+        const adsg::direction& t = b->sims[k / 2]->dirs[k % 2].tab;
+        const size_t m = t.p + 1;
+        to[k] = {L.take(sizeof(T) * t.elements * m * m), L.take(sizeof(T) * t.elements * m * m),
+                 L.take(sizeof(T) * t.elements * m)};
+    }
+    if (hipMalloc(&b->arena[0], L.size) != hipSuccess)
+        return bfail(b, ADSG_ENOMEM, "batch_create: no memory for the problems' tables");
+    char* base = static_cast<char*>(b->arena[0]);
+    std::vector<char> head(d_off + count * sizeof(adsg::bs_dir), 0);
+    double* g = reinterpret_cast<double*>(head.data() + g_off);
+    for (int p = 1; p <= 3; ++p) adsg::gauss_legendre(p + 1, g + kGaussOff[p], g + kGaussOff[p] + p + 1);
+    const double* gdev = reinterpret_cast<const double*>(base + g_off);
+    for (int k = 0; k < count; ++k) {
+        adsg_sim* h = b->sims[k / 2];
+        adsg::dev_dir& dd = h->dirs[k % 2];
+        adsg::bs_dir& d = b->bs[k];
+        d.p = dd.tab.p;
+        d.ne = dd.tab.elements;
+        d.n = dd.tab.dofs;
+        d.a = dd.tab.a;
+        d.b = dd.tab.b;
+        d.gx = gdev + kGaussOff[d.p];
+        d.gw = gdev + kGaussOff[d.p] + d.p + 1;
+        d.val = dd.val = base + to[k][0];
+        d.der = dd.der = base + to[k][1];
+        d.wj = dd.wj = base + to[k][2];
+    }
+    std::memcpy(head.data() + d_off, b->bs.data(), count * sizeof(adsg::bs_dir));
+    hipError_t e = hipMemcpyAsync(base, head.data(), head.size(), hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = adsg::launch_batch_tables<T>(reinterpret_cast<const adsg::bs_dir*>(base + d_off), count, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    return e == hipSuccess ? ADSG_OK : bhip(b, e, "batch_create: tabulating the bases");
+}
+
+// prepare: arena 1 = [descriptors | info | every direction's lower, upper, rdiag (T), poff];
+// the packed sweep rows (when every direction splits into segments) in b->packs; the FP64
+// Gram bands in a work array freed at the end
+template <typename T>
+int batch_factors(adsg_batch* b) {
+    const int count = static_cast<int>(b->bs.size());
+    for (void** a : {&b->arena[1], &b->packs})  // a failed earlier prepare
+        if (*a) {
+            (void)hipFree(*a);
+            *a = nullptr;
+        }
+    bool all_seg = true;
+    for (auto& d : b->bs) {
+        int m = 0, nseg = 0, mseg = 0;
+        if (!adsg::segment_shape(d.n, d.p, 0, adsg::kBatchSegments, m, nseg, mseg)) all_seg = false;
+        d.m = m;
+        d.nseg = nseg;
+        d.mseg = mseg;
+    }
+    arena_layout L, W, Pk;
+    const size_t d_off = L.take(count * sizeof(adsg::bs_dir));
+    const size_t i_off = L.take(count * sizeof(int));
+    std::vector<std::array<size_t, 6>> fo(count);
+    for (int k = 0; k < count; ++k) {
+        const adsg::bs_dir& d = b->bs[k];
+        fo[k] = {L.take(sizeof(T) * d.n * d.p), L.take(sizeof(T) * d.n * d.p), L.take(sizeof(T) * d.n),
+                 L.take(sizeof(int) * d.n), W.take(sizeof(double) * d.n * (3 * d.p + 1)),
+                 all_seg ? Pk.take(sizeof(T) * d.nseg * d.mseg * (4 * d.p + 1)) : 0};
+    }
+    void* work = nullptr;
+    if (hipMalloc(&b->arena[1], L.size) != hipSuccess || hipMalloc(&work, W.size) != hipSuccess ||
+        (all_seg && hipMalloc(&b->packs, Pk.size) != hipSuccess)) {
+        if (work) (void)hipFree(work);
+        return bfail(b, ADSG_ENOMEM, "batch_prepare: no memory for the factors");
+    }
+    char* base = static_cast<char*>(b->arena[1]);
+    for (int k = 0; k < count; ++k) {
+        adsg::bs_dir& d = b->bs[k];
+        d.lower = base + fo[k][0];
+        d.upper = base + fo[k][1];
+        d.rdiag = base + fo[k][2];
+        d.poff = reinterpret_cast<int*>(base + fo[k][3]);
+        d.ab = reinterpret_cast<double*>(static_cast<char*>(work) + fo[k][4]);
+        d.info = reinterpret_cast<int*>(base + i_off) + k;
+        if (!all_seg) d.nseg = 0;
+        d.pack = all_seg ? static_cast<char*>(b->packs) + fo[k][5] : nullptr;
+    }
+    std::vector<int> info(count, 0);
+    hipError_t e = hipMemcpyAsync(base + d_off, b->bs.data(), count * sizeof(adsg::bs_dir), hipMemcpyHostToDevice,
+                                  b->stream);
+    if (e == hipSuccess)
+        e = adsg::launch_batch_factors<T>(reinterpret_cast<const adsg::bs_dir*>(base + d_off), count, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(info.data(), base + i_off, count * sizeof(int), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(work);
+    for (auto& d : b->bs) d.ab = nullptr;
+    if (e != hipSuccess) return bhip(b, e, "batch_prepare: factoring");
+    for (int k = 0; k < count; ++k) {
+        if (info[k] == -1)
+            return bfail(b, ADSG_EINVAL, "batch_prepare: problem " + std::to_string(k / 2) +
+                                             ": batched sweeps need unpivoted degree-p band factors");
+        if (info[k] > 0)
+            return bfail(b, info[k], "batch_prepare: problem " + std::to_string(k / 2) + ": zero pivot at row " +
+                                         std::to_string(info[k]));
+    }
+    for (int k = 0; k < count; ++k) {
+        const adsg::bs_dir& d = b->bs[k];
+        adsg_sim* h = b->sims[k / 2];
+        adsg::dev_slot& s = h->dirs[k % 2].slots[0];
+        s.lower = d.lower;
+        s.upper = d.upper;
+        s.rdiag = d.rdiag;
+        s.poff = d.poff;
+        s.lf.n = d.n;
+        s.lf.kl = s.lf.ku = s.lf.ku_eff = d.p;
+        s.lf.pivoted = false;
+        s.factored = true;
+        h->prepared = true;
+    }
+    return ADSG_OK;
+}
+
 bool valid_problem(const adsg_batch* b, int i) { return b && i >= 0 && i < static_cast<int>(b->sims.size()); }
 
 }  // namespace
@@ -402,6 +549,8 @@ int adsg_batch_create(const adsg_config* cfgs, int count, adsg_batch** out) {
         delete b;
         return ADSG_EHIP;
     }
+    int device = 0;
+    (void)hipGetDevice(&device);
     for (int i = 0; i < count; ++i) {
         const adsg_config& c = cfgs[i];
         if (c.dim != 2 || c.precision != b->precision || c.x.p != c.y.p || c.x.p < 1 || c.x.p > 3) {
@@ -410,40 +559,32 @@ int adsg_batch_create(const adsg_config* cfgs, int count, adsg_batch** out) {
             adsg_batch_destroy(b);
             return ADSG_EINVAL;
         }
-    }
-    // the problems' host set-up on worker threads, each staging its tables; then one upload
-    std::vector<adsg::stager> st(count);
-    std::vector<adsg_sim*> hs(count, nullptr);
-    std::vector<int> rcs(count, ADSG_OK);
-    std::vector<std::string> msgs(count);
-    for_each_problem(count, [&](int i) {
-        rcs[i] = adsg::create_sim(&cfgs[i], b->stream, false, &hs[i], &st[i]);
-        if (rcs[i]) msgs[i] = adsg_last_error(nullptr);  // this thread's create error
-    });
-    int bad = -1;
-    for (int i = 0; i < count && bad < 0; ++i)
-        if (rcs[i] || hs[i]->general) bad = i;
-    for (int i = 0; i < count; ++i)
-        if (hs[i]) {
-            b->sims.push_back(hs[i]);
-            b->off.push_back(b->total);
-            b->total += hs[i]->N;
+        for (const adsg_dim_config* dc : {&c.x, &c.y}) {
+            if (dc->elements < 1 || !(dc->a < dc->b)) {
+                g_batch_error = "batch_create: problem " + std::to_string(i) +
+                                ": need at least one element and a < b in every direction";
+                adsg_batch_destroy(b);
+                return ADSG_EINVAL;
+            }
+            if ((dc->quad_order > 0 && dc->quad_order != dc->p + 1) || dc->repeated_nodes != 0) {
+                g_batch_error = "batch_create: problem " + std::to_string(i) +
+                                ": batches take p+1 Gauss points and simple knots (quad_order 0 or p+1, "
+                                "repeated_nodes 0)";
+                adsg_batch_destroy(b);
+                return ADSG_EINVAL;
+            }
         }
-    if (bad >= 0) {
-        g_batch_error = "batch_create: problem " + std::to_string(bad) + ": " +
-                        (rcs[bad] ? msgs[bad]
-                                  : std::string("batches take p+1 Gauss points and simple knots (quad_order 0 or p+1, "
-                                                "repeated_nodes 0)"));
-        const int rc = rcs[bad] ? rcs[bad] : ADSG_EINVAL;
+    }
+    for (int i = 0; i < count; ++i) {
+        b->sims.push_back(batch_sim(cfgs[i], b->stream, device));
+        b->off.push_back(b->total);
+        b->total += b->sims.back()->N;
+    }
+    // every problem's basis tables, tabulated on the device
+    if (const int rc = b->precision == ADSG_FP32 ? batch_tables<float>(b) : batch_tables<double>(b)) {
+        g_batch_error = b->err;
         adsg_batch_destroy(b);
         return rc;
-    }
-    adsg::stager tables;
-    for (auto& x : st) tables.take(x);
-    if (tables.commit(&b->arena[0]) != hipSuccess) {
-        g_batch_error = "batch_create: uploading the problems' tables failed";
-        adsg_batch_destroy(b);
-        return ADSG_ENOMEM;
     }
     bool ok = true;
     for (auto& a : b->u) ok = ok && hipMalloc(reinterpret_cast<void**>(&a), b->total * sizeof(double)) == hipSuccess;
@@ -478,6 +619,7 @@ void adsg_batch_destroy(adsg_batch* b) {
     for (adsg_sim* h : b->sims) adsg_destroy(h);
     for (void* a : b->arena)
         if (a) (void)hipFree(a);
+    if (b->packs) (void)hipFree(b->packs);
     for (auto& a : b->u)
         if (a) (void)hipFree(a);
     for (auto& a : b->sh)
@@ -500,32 +642,10 @@ int adsg_batch_dofs(const adsg_batch* b, int problem, int dir) {
 int adsg_batch_prepare(adsg_batch* b) {
     if (!b) return ADSG_EINVAL;
     if (b->prepared) return ADSG_OK;  // the factors live in the batch's arena
-    // every problem's factors on worker threads, each staged; then one upload
-    const int count = static_cast<int>(b->sims.size());
-    std::vector<adsg::stager> st(count);
-    std::vector<int> rcs(count, ADSG_OK);
-    for_each_problem(count, [&](int i) {
-        adsg_sim* h = b->sims[i];
-        h->stage = &st[i];
-        rcs[i] = adsg_prepare_matrices(h);
-        h->stage = nullptr;
-    });
-    adsg::stager factors;
-    for (auto& x : st) factors.take(x);
-    for (size_t i = 0; i < b->sims.size(); ++i) {
-        adsg_sim* h = b->sims[i];
-        const int rc = rcs[i];
-        if (rc) return bfail(b, rc, "batch_prepare: problem " + std::to_string(i) + ": " + h->err);
-        for (int d = 0; d < 2; ++d) {
-            const auto& s = h->dirs[d].slots[0];
-            if (s.lf.pivoted || s.lf.kl != h->dirs[d].tab.p || s.lf.ku_eff != h->dirs[d].tab.p)
-                return bfail(b, ADSG_EINVAL, "batch_prepare: problem " + std::to_string(i) +
-                                                 ": batched sweeps need unpivoted degree-p band factors");
-        }
-    }
-    hipError_t e = factors.commit(&b->arena[1]);
-    if (e != hipSuccess) return bhip(b, e, "batch_prepare: uploading the factors");
-    int rc = build(b);
+    // every problem's Gram bands, their LU and the packed sweep rows, on the device
+    int rc = b->precision == ADSG_FP32 ? batch_factors<float>(b) : batch_factors<double>(b);
+    if (rc) return rc;
+    rc = build(b);
     if (rc) return rc;
     b->prepared = true;
     return ADSG_OK;

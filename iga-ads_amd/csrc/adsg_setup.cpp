@@ -287,11 +287,8 @@ int factor_for_lines(const std::vector<double>& ab_in, int n, int kl, int ku, li
     return info;
 }
 
-bool segment_lines(const line_factors& f, line_segments& out, int max_nseg16, int max_nseg32) {
-    const int n = f.n, kl = f.kl, ku = f.ku_eff;
-    const int km = std::max(kl, ku);
-    if (f.pivoted || kl < 1 || ku < 1 || kl != ku) return false;
-    int mseg = 0, nseg = 0, m = 0;
+bool segment_shape(int n, int km, int max_nseg16, int max_nseg32, int& m, int& nseg, int& mseg) {
+    mseg = nseg = m = 0;
     // the fewest segments of at most `rows` rows (balanced lengths, the last one at least km
     // rows long) within the caller's count limit: the serial chain between segments costs
     // more than the extra waves of shorter segments gain (measured on 130-row lines: 9
@@ -308,7 +305,15 @@ bool segment_lines(const line_factors& f, line_segments& out, int max_nseg16, in
     };
     pick(max_nseg16, 16);
     pick(max_nseg32, 32);  // lines too long for segments of 16 rows
-    if (!mseg) return false;
+    return mseg != 0;
+}
+
+bool segment_lines(const line_factors& f, line_segments& out, int max_nseg16, int max_nseg32) {
+    const int n = f.n, kl = f.kl, ku = f.ku_eff;
+    const int km = std::max(kl, ku);
+    if (f.pivoted || kl < 1 || ku < 1 || kl != ku) return false;
+    int mseg = 0, nseg = 0, m = 0;
+    if (!segment_shape(n, km, max_nseg16, max_nseg32, m, nseg, mseg)) return false;
     out.m = m;
     out.nseg = nseg;
     out.mseg = mseg;

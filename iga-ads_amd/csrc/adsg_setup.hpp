@@ -81,5 +81,8 @@ struct line_segments {
 // The fewest segments of <= 16 rows if that count is <= max_nseg16, else the fewest of
 // <= 32 rows if <= max_nseg32 (the callers' limits keep the kernels' LDS within budget).
 bool segment_lines(const line_factors& f, line_segments& out, int max_nseg16, int max_nseg32);
+// The segment shape segment_lines picks for lines of n rows with half-bandwidth km:
+// m rows per segment, nseg segments in blocks of mseg rows; false when none fits.
+bool segment_shape(int n, int km, int max_nseg16, int max_nseg32, int& m, int& nseg, int& mseg);
 
 }  // namespace adsg

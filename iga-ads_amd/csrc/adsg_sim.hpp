@@ -15,46 +15,6 @@
 
 namespace adsg {
 
-// Uploads collected instead of made one by one: the problems of a batch stage their tables
-// and factors here and the batch copies them all into one device arena with one copy
-// (thousands of small allocations and synchronous copies otherwise dominate its set-up).
-struct stager {
-    struct chunk {
-        void** dst;
-        std::vector<unsigned char> bytes;
-    };
-    std::vector<chunk> chunks;
-    void add(void** dst, const void* data, size_t bytes) {
-        *dst = nullptr;
-        const unsigned char* b = static_cast<const unsigned char*>(data);
-        chunks.push_back({dst, std::vector<unsigned char>(b, b + bytes)});
-    }
-    // one allocation (256-byte aligned chunks) and one copy; patches every *dst; the caller
-    // owns *arena
-    // appends the chunks of other (in order) to this stager's
-    void take(stager& other) {
-        for (auto& c : other.chunks) chunks.push_back(std::move(c));
-        other.chunks.clear();
-    }
-    hipError_t commit(void** arena) {
-        *arena = nullptr;
-        size_t total = 0;
-        for (auto& c : chunks) total += (c.bytes.size() + 255) / 256 * 256;
-        if (total == 0) return hipSuccess;
-        hipError_t e = hipMalloc(arena, total);
-        if (e != hipSuccess) return e;
-        std::vector<unsigned char> host(total, 0);
-        size_t off = 0;
-        for (auto& c : chunks) {
-            std::copy(c.bytes.begin(), c.bytes.end(), host.begin() + off);
-            *c.dst = static_cast<unsigned char*>(*arena) + off;
-            off += (c.bytes.size() + 255) / 256 * 256;
-        }
-        chunks.clear();
-        return hipMemcpy(*arena, host.data(), total, hipMemcpyHostToDevice);
-    }
-};
-
 struct dev_slot {
     bool set = false;
     bool factored = false;
@@ -109,9 +69,7 @@ struct adsg_sim {
     // and the entry points that assume one degree and p+1 Gauss points refuse the handle
     bool general = false;
     bool mixed_p = false;
-    // set while a batch stages this problem's uploads (adsg::stager); in_arena: the table and
-    // factor pointers point into a batch's arena and are not freed one by one
-    adsg::stager* stage = nullptr;
+    // the table and factor pointers point into a batch's arenas and are not freed one by one
     bool in_arena = false;
     // u and u_prev are FP64 in both precisions; buffer, kfield and source are in the step
     // precision. In FP32 mode shadow[0]/shadow[1] are FP32 copies of u/u_prev (they swap
@@ -154,6 +112,5 @@ struct adsg_sim {
 namespace adsg {
 // adsg_create with a shared stream (not owned by the handle) and, without `vectors`, no
 // state vectors (the problems of a batch keep theirs in the batch's arenas)
-int create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, adsg_sim** out,
-               stager* stage = nullptr);
+int create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, adsg_sim** out);
 }  // namespace adsg

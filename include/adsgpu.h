@@ -347,7 +347,14 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
  * {mass 1, dt, grad eps, reaction 1}). The state of all problems lives in shared arenas;
  * one step is a fixed sequence of grouped launches over all problems (element kernels per
  * degree and colour, x and y sweeps per degree, one update), and nsteps steps are replayed
- * from HIP graphs of at most 50 steps, each captured once per step count. FP32 batches use the increment scheme of ADSG_FP32. */
+ * from HIP graphs of at most 50 steps, each captured once per step count. FP32 batches use
+ * the increment scheme of ADSG_FP32. The set-up runs on the device: adsg_batch_create
+ * tabulates every problem's basis (basis_data.cpp:63-114) in one launch, adsg_batch_prepare
+ * assembles the Gram matrices (form_matrix.cpp:8-42), factors them in FP64 (dgbtrf without
+ * interchanges, which these matrices never need) and packs the segmented sweep rows in two
+ * more; the results equal the host set-up of adsg_create / adsg_prepare_matrices bit for bit.
+ * Configurations: dim 2, one precision, p 1..3 the same in x and y, elements >= 1, a < b,
+ * quad_order 0 or p + 1, repeated_nodes 0; others are refused with ADSG_EINVAL. */
 typedef struct adsg_batch adsg_batch;
 int adsg_batch_create(const adsg_config* cfgs, int count, adsg_batch** out);
 void adsg_batch_destroy(adsg_batch* b);
@@ -355,7 +362,8 @@ const char* adsg_batch_last_error(const adsg_batch* b); /* NULL: last create err
 int adsg_batch_count(const adsg_batch* b);
 long adsg_batch_size(const adsg_batch* b, int problem);
 int adsg_batch_dofs(const adsg_batch* b, int problem, int dir);
-/* factor every problem's Gram matrices (FP64) and build the launch lists */
+/* assemble and factor every problem's Gram matrices (FP64, on the device) and build the
+ * launch lists; > 0: a zero pivot (its row), ADSG_EINVAL: a factor that needs interchanges */
 int adsg_batch_prepare(adsg_batch* b);
 /* one form per problem (default: explicit heat, dt = 0) */
 int adsg_batch_set_forms(adsg_batch* b, const adsg_rhs_form* forms);

@@ -118,3 +118,68 @@ def test_batch_graph_replay_and_autotune_keep_results():
     for r in runs[1:]:
         for a, c in zip(runs[0], r):
             assert np.array_equal(a, c)
+
+
+def test_device_setup_on_another_interval_equals_single_problem_path_bitwise():
+    """The batch tabulates, assembles and factors its problems on the device; on [-1, 2]^2
+    (knots, Jacobians and Gauss points other than the unit square's) its steps still equal
+    the single-problem path, whose set-up runs on the host, bit for bit."""
+    _log("case: batch device set-up on [-1, 2]^2 vs one-by-one")
+    probs = [(1, 33), (2, 47), (3, 29), (3, 64)]
+    forms = [ag.rhs_form(DT, grad=(3e-3, 3e-3, 0.0), reaction=1.0)] * len(probs)
+    for precision in (ag.FP64, ag.FP32):
+        b = ag.Batch(probs, precision=precision, interval=(-1.0, 2.0))
+        b.prepare()
+        b.set_forms(forms)
+        b.set_tuning(1, ag.TUNE_SWEEP_VARIANT)
+        u0 = [wl.smooth_unit(p, ne, seed=7 + i) for i, (p, ne) in enumerate(probs)]
+        for i, u in enumerate(u0):
+            b.set(i, u)
+        b.step(4)
+        for i, (p, ne) in enumerate(probs):
+            g = ag.Simulation(2, [p, p], [ne, ne], precision=precision, a=[-1.0, -1.0], b=[2.0, 2.0])
+            g.prepare_matrices()
+            g.set_tuning(ag.TUNE_SWEEP_VARIANT, 1)
+            g.set(ag.U, u0[i])
+            g.step(4, forms[i])
+            assert np.array_equal(g.get(ag.U), b.get(i)), (precision, i)
+            g.close()
+        b.close()
+
+
+def test_batch_create_refuses_configurations_outside_the_batched_kernels():
+    """Batches take 2D problems with p + 1 Gauss points, simple knots, one degree and a < b;
+    every other configuration is refused with ADSG_EINVAL and a message naming the problem."""
+    import ctypes as C
+
+    L = ag.lib()
+
+    def create(edit):
+        cfgs = (ag.Config * 2)()
+        for c in cfgs:
+            c.dim = 2
+            for dc in (c.x, c.y):
+                dc.p, dc.elements, dc.a, dc.b, dc.quad_order, dc.repeated_nodes = 2, 8, 0.0, 1.0, 0, 0
+            c.precision = ag.FP64
+        edit(cfgs[1])
+        h = C.c_void_p()
+        rc = L.adsg_batch_create(cfgs, 2, C.byref(h))
+        if rc == ag.OK:
+            L.adsg_batch_destroy(h)
+        return rc, L.adsg_batch_last_error(None).decode()
+
+    def setter(**kw):
+        def edit(c):
+            for k, v in kw.items():
+                if k in ("dim", "precision"):
+                    setattr(c, k, v)
+                else:
+                    setattr(c.y, k, v)
+        return edit
+
+    assert create(setter())[0] == ag.OK
+    for kw in (dict(repeated_nodes=1), dict(quad_order=5), dict(p=3), dict(dim=3), dict(precision=ag.FP32),
+               dict(elements=0), dict(a=1.0)):
+        rc, msg = create(setter(**kw))
+        assert rc == ag.EINVAL, kw
+        assert "problem 1" in msg, (kw, msg)
