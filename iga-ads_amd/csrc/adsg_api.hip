@@ -62,9 +62,9 @@ int rhs_code(const adsg_sim* h) {
 // (never inside a captured step); FP64-sized in both precisions, since FP32 handles also run
 // FP64 element loops (the adjoint's and the norms' products)
 int ensure_spill(adsg_sim* h) {
-    if (h->spill || h->dim != 3 || h->general) return ADSG_OK;
-    const int el[3] = {h->dirs[0].tab.elements, h->dirs[1].tab.elements, h->dirs[2].tab.elements};
-    const size_t bytes = static_cast<size_t>(adsg::rhs3_spill_slots(h->dirs[0].tab.p, el)) * sizeof(double);
+    if (h->spill || h->general) return ADSG_OK;
+    const int el[3] = {h->dirs[0].tab.elements, h->dirs[1].tab.elements, h->dim == 3 ? h->dirs[2].tab.elements : 1};
+    const size_t bytes = static_cast<size_t>(adsg::rhs_spill_slots(h->dim, h->dirs[0].tab.p, el)) * sizeof(double);
     if (bytes && hipMalloc(&h->spill, bytes) != hipSuccess) {
         h->spill = nullptr;
         return fail(h, ADSG_ENOMEM, "one-launch tiling: no memory for the spill array");
@@ -835,7 +835,9 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
     // two waves per SIMD), FP32 5 (element columns on the matrix cores); p <= 2: 4 from 2^21
     // elements (enough tiles of 512 elements to fill the device twice over), else 1. 3D
     // handles start with the one-launch tiling (ADSG_TUNE_RHS_TILING), which was faster for
-    // every variant measured (p = 2 at 128^3, p = 3 at 192^3, both precisions).
+    // every variant measured (p = 2 at 128^3, p = 3 at 192^3, both precisions); 2D handles with
+    // the colour launches, which the batches use (so a batched problem equals the same problem
+    // run alone, bit for bit).
     {
         long E = 1;
         for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
@@ -1681,8 +1683,8 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
     std::vector<std::array<int, 3>> cand;
     for (int v = 0; v <= 4; ++v) cand.push_back({v, 0, 0});
     for (int c = 0; c <= 3; ++c) cand.push_back({5, c, 0});
-    if (h->dim == 3 && !h->general && ensure_spill(h) == ADSG_OK && h->spill)
-        for (int v : {0, 1, 2, 4}) cand.push_back({v, 0, 1});
+    if (!h->general && ensure_spill(h) == ADSG_OK && h->spill)
+        for (int v : (h->dim == 3 ? std::vector<int>{0, 1, 2, 4} : std::vector<int>{0})) cand.push_back({v, 0, 1});
     for (size_t k = 0; k < cand.size() && rc == ADSG_OK; ++k) {
         const int v = cand[k][0];
         h->rhs_variant = v;

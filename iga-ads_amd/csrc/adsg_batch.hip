@@ -27,12 +27,14 @@ namespace {
 
 thread_local std::string g_batch_error;
 
-// the launches of one degree for one arena parity (device arrays of items)
+// the launches of one step for one arena parity, over every problem of every degree (device
+// arrays of items, one item per problem)
 struct batch_group {
-    int p = 0;
     int nitems = 0;
-    void* rhs_items = nullptr;  // rhs2_item<T>[nitems]
-    int rhs_blocks[4] = {0, 0, 0, 0};
+    void* rhs_items = nullptr;  // rhs2_item<T>[nitems], the problems of degree 1, then 2, then 3
+    int deg_first[4] = {0, 0, 0, 0};  // first item of degree p
+    int deg_count[4] = {0, 0, 0, 0};  // items of degree p
+    int deg_blocks[4][4] = {};        // element-kernel blocks of degree p in colour c
     void* sx_items = nullptr;  // sweep_item<T>[nitems], x lines
     int sx_blocks = 0;
     void* sy_items = nullptr;  // y lines
@@ -135,104 +137,111 @@ struct item_stage {
     }
 };
 
-// launch items of every degree for both arena parities (T: the step precision)
+// launch items of every problem for both arena parities (T: the step precision)
 template <typename T>
 int build_groups(adsg_batch* b) {
     free_groups(b);
     drop_batch_graphs(b);
     // the packed sweep rows come from prepare (every direction segmented, or none)
     const bool all_seg = b->packs != nullptr;
+    const int P = static_cast<int>(b->sims.size());
     item_stage stage;
     std::vector<std::array<size_t, 5>> offs;  // per group, in the order of b->groups[0], [1]
     for (int ph = 0; ph < 2; ++ph) {
-        for (int p = 1; p <= 3; ++p) {
-            std::vector<adsg::rhs2_item<T>> ri;
-            std::vector<adsg::sweep_item<T>> sx, sy;
-            std::vector<adsg::seg_item<T>> gx, gy;
-            batch_group g;
-            g.p = p;
-            for (size_t i = 0; i < b->sims.size(); ++i) {
-                const adsg_sim* h = b->sims[i];
-                if (h->dirs[0].tab.p != p) continue;
-                const adsg_rhs_form& fm = b->forms[i];
-                adsg::rhs2_item<T> it;
-                // a step starting at parity ph reads u_prev = u[ph] and writes u[1 - ph]
-                T* target;
-                if constexpr (sizeof(T) == sizeof(float)) {
-                    it.up = b->sh[ph] + b->off[i];
-                    target = b->buf + b->off[i];
-                } else {
-                    it.up = b->u[ph] + b->off[i];
-                    target = b->u[1 - ph] + b->off[i];
-                }
-                it.rhs = target;
-                it.X = tables_of<T>(h, 0);
-                it.Y = tables_of<T>(h, 1);
-                // FP32 assembles the increment rhs - M u_prev: the mass term less one
-                it.prm.mass = static_cast<T>(sizeof(T) == sizeof(float) ? fm.mass - 1.0 : fm.mass);
-                it.prm.dt = static_cast<T>(fm.dt);
-                it.prm.gx = static_cast<T>(fm.grad[0]);
-                it.prm.gy = static_cast<T>(fm.grad[1]);
-                it.prm.gz = static_cast<T>(fm.grad[2]);
-                it.prm.reaction = static_cast<T>(fm.reaction);
-                int count[4];
-                adsg::rhs2_batch_tiles(b->shape, it.X.elements, it.Y.elements, it.ntx2, count);
-                for (int c = 0; c < 4; ++c) {
-                    it.first[c] = g.rhs_blocks[c];
-                    g.rhs_blocks[c] += count[c];
-                }
-                ri.push_back(it);
-                const long nx = h->dirs[0].tab.dofs, ny = h->dirs[1].tab.dofs;
-                adsg::sweep_item<T> xi, yi;
-                xi.u = yi.u = target;
-                xi.g.n_lines = ny;
-                xi.g.S = 1;
-                xi.f = factors_of<T>(h, 0);
-                xi.first = g.sx_blocks;
-                g.sx_blocks += static_cast<int>((ny + 63) / 64);
-                yi.g.n_lines = nx;
-                yi.g.A = nx;
-                yi.g.B = nx * ny;
-                yi.g.S = nx;
-                yi.f = factors_of<T>(h, 1);
-                yi.first = g.sy_blocks;
-                g.sy_blocks += static_cast<int>((nx + 63) / 64);
-                sx.push_back(xi);
-                sy.push_back(yi);
-                if (all_seg) {
-                    adsg::seg_item<T> a, c;
-                    const adsg::bs_dir& sgx = b->bs[2 * i];
-                    const adsg::bs_dir& sgy = b->bs[2 * i + 1];
-                    a.u = c.u = target;
-                    a.g.n_lines = ny;  // x line l starts at l * nx
-                    a.g.A = 1;
-                    a.g.B = nx;
-                    a.g.S = 1;
-                    a.pack = static_cast<const T*>(sgx.pack);
-                    a.n = static_cast<int>(nx);
-                    a.m = sgx.m;
-                    a.nseg = sgx.nseg;
-                    a.first = g.gx_blocks;
-                    g.gx_blocks += static_cast<int>((ny + 63) / 64);
-                    c.g = yi.g;
-                    c.pack = static_cast<const T*>(sgy.pack);
-                    c.n = static_cast<int>(ny);
-                    c.m = sgy.m;
-                    c.nseg = sgy.nseg;
-                    c.first = g.gy_blocks;
-                    g.gy_blocks += static_cast<int>((nx + 63) / 64);
-                    gx.push_back(a);
-                    gy.push_back(c);
-                }
+        std::vector<adsg::rhs2_item<T>> ri;
+        std::vector<adsg::sweep_item<T>> sx, sy;
+        std::vector<adsg::seg_item<T>> gx, gy;
+        batch_group g;
+        std::vector<adsg::rhs2_item<T>> rdeg[4];
+        for (int i = 0; i < P; ++i) {
+            const adsg_sim* h = b->sims[i];
+            const int p = h->dirs[0].tab.p;
+            const adsg_rhs_form& fm = b->forms[i];
+            adsg::rhs2_item<T> it;
+            // a step starting at parity ph reads u_prev = u[ph] and writes u[1 - ph]
+            T* target;
+            if constexpr (sizeof(T) == sizeof(float)) {
+                it.up = b->sh[ph] + b->off[i];
+                target = b->buf + b->off[i];
+            } else {
+                it.up = b->u[ph] + b->off[i];
+                target = b->u[1 - ph] + b->off[i];
             }
-            if (ri.empty()) continue;
-            g.nitems = static_cast<int>(ri.size());
-            g.seg = all_seg;
-            // byte offsets of the group's item arrays, turned into device pointers below
-            offs.push_back({stage.add(ri), stage.add(sx), stage.add(sy), all_seg ? stage.add(gx) : 0,
-                            all_seg ? stage.add(gy) : 0});
-            b->groups[ph].push_back(g);
+            it.rhs = target;
+            it.X = tables_of<T>(h, 0);
+            it.Y = tables_of<T>(h, 1);
+            // FP32 assembles the increment rhs - M u_prev: the mass term less one
+            it.prm.mass = static_cast<T>(sizeof(T) == sizeof(float) ? fm.mass - 1.0 : fm.mass);
+            it.prm.dt = static_cast<T>(fm.dt);
+            it.prm.gx = static_cast<T>(fm.grad[0]);
+            it.prm.gy = static_cast<T>(fm.grad[1]);
+            it.prm.gz = static_cast<T>(fm.grad[2]);
+            it.prm.reaction = static_cast<T>(fm.reaction);
+            int count[4];
+            adsg::rhs2_batch_tiles(b->shape, it.X.elements, it.Y.elements, it.ntx2, count);
+            for (int c = 0; c < 4; ++c) {
+                it.first[c] = g.deg_blocks[p][c];
+                g.deg_blocks[p][c] += count[c];
+            }
+            rdeg[p].push_back(it);
+            const long nx = h->dirs[0].tab.dofs, ny = h->dirs[1].tab.dofs;
+            adsg::sweep_item<T> xi, yi;
+            xi.u = yi.u = target;
+            xi.g.n_lines = ny;
+            xi.g.S = 1;
+            xi.f = factors_of<T>(h, 0);
+            xi.first = g.sx_blocks;
+            g.sx_blocks += static_cast<int>((ny + 63) / 64);
+            yi.g.n_lines = nx;
+            yi.g.A = nx;
+            yi.g.B = nx * ny;
+            yi.g.S = nx;
+            yi.f = factors_of<T>(h, 1);
+            yi.first = g.sy_blocks;
+            g.sy_blocks += static_cast<int>((nx + 63) / 64);
+            sx.push_back(xi);
+            sy.push_back(yi);
+            if (all_seg) {
+                adsg::seg_item<T> a, c;
+                const adsg::bs_dir& sgx = b->bs[2 * i];
+                const adsg::bs_dir& sgy = b->bs[2 * i + 1];
+                a.u = c.u = target;
+                a.g.n_lines = ny;  // x line l starts at l * nx
+                a.g.A = 1;
+                a.g.B = nx;
+                a.g.S = 1;
+                a.pack = static_cast<const T*>(sgx.pack);
+                a.n = static_cast<int>(nx);
+                a.m = sgx.m;
+                a.nseg = sgx.nseg;
+                a.p = p;
+                a.first = g.gx_blocks;
+                g.gx_blocks += static_cast<int>((ny + 63) / 64);
+                c.g = yi.g;
+                c.pack = static_cast<const T*>(sgy.pack);
+                c.n = static_cast<int>(ny);
+                c.m = sgy.m;
+                c.nseg = sgy.nseg;
+                c.p = p;
+                c.first = g.gy_blocks;
+                g.gy_blocks += static_cast<int>((nx + 63) / 64);
+                gx.push_back(a);
+                gy.push_back(c);
+            }
         }
+        // the element items by degree
+        for (int p = 1; p <= 3; ++p) {
+            g.deg_first[p] = static_cast<int>(ri.size());
+            g.deg_count[p] = static_cast<int>(rdeg[p].size());
+            ri.insert(ri.end(), rdeg[p].begin(), rdeg[p].end());
+        }
+        if (ri.empty()) continue;
+        g.nitems = static_cast<int>(ri.size());
+        g.seg = all_seg;
+        // byte offsets of the group's item arrays, turned into device pointers below
+        offs.push_back({stage.add(ri), stage.add(sx), stage.add(sy), all_seg ? stage.add(gx) : 0,
+                        all_seg ? stage.add(gy) : 0});
+        b->groups[ph].push_back(g);
     }
     if (stage.bytes.empty()) return ADSG_OK;
     if (hipMalloc(&b->items, stage.bytes.size()) != hipSuccess)
@@ -257,39 +266,39 @@ int build(adsg_batch* b) {
     return b->precision == ADSG_FP32 ? build_groups<float>(b) : build_groups<double>(b);
 }
 
-// one step of every problem, starting at arena parity b->phase
+// one step of every problem, starting at arena parity b->phase: the zero fill, the element
+// kernels of the problems of each degree in one launch per colour, the x lines of every
+// problem (every degree) in one launch, the y lines in one, and (FP32) the update
 template <typename T>
 int batch_step_impl(adsg_batch* b) {
     const int ph = b->phase;
-    hipError_t e;
     T* zero = nullptr;
     if constexpr (sizeof(T) == sizeof(float))
         zero = b->buf;
     else
         zero = b->u[1 - ph];
-    if ((e = adsg::launch_fill<T>(zero, b->total, T(0), b->stream)) != hipSuccess) return bhip(b, e, "batch: zero fill");
-    for (const batch_group& g : b->groups[ph])
-        for (int c = 0; c < 4; ++c) {
-            e = adsg::launch_rhs2_batch<T>(g.p, b->shape, static_cast<const adsg::rhs2_item<T>*>(g.rhs_items),
-                                          g.nitems, c, g.rhs_blocks[c], b->stream);
-            if (e != hipSuccess) return bhip(b, e, "batch: element kernels");
-        }
+    hipError_t e = adsg::launch_fill<T>(zero, b->total, T(0), b->stream);
+    if (e != hipSuccess) return bhip(b, e, "batch: zero fill");
     for (const batch_group& g : b->groups[ph]) {
+        const auto* items = static_cast<const adsg::rhs2_item<T>*>(g.rhs_items);
+        for (int p = 1; p <= 3 && e == hipSuccess; ++p)
+            for (int c = 0; c < 4 && e == hipSuccess; ++c)
+                e = adsg::launch_rhs2_batch<T>(p, b->shape, items + g.deg_first[p], g.deg_count[p], c,
+                                               g.deg_blocks[p][c], b->stream);
+        if (e != hipSuccess) return bhip(b, e, "batch: element kernels");
         if (g.seg && b->sweep == 0)
-            e = adsg::launch_sweep_seg_batch<T>(g.p, g.p, static_cast<const adsg::seg_item<T>*>(g.gx_items),
-                                                g.nitems, g.gx_blocks, b->stream);
+            e = adsg::launch_sweep_seg_batch<T>(static_cast<const adsg::seg_item<T>*>(g.gx_items), g.nitems,
+                                                g.gx_blocks, b->stream);
         else
-            e = adsg::launch_sweep_batch<T>(g.p, g.p, static_cast<const adsg::sweep_item<T>*>(g.sx_items),
-                                            g.nitems, g.sx_blocks, b->stream);
+            e = adsg::launch_sweep_batch<T>(static_cast<const adsg::sweep_item<T>*>(g.sx_items), g.nitems,
+                                            g.sx_blocks, b->stream);
         if (e != hipSuccess) return bhip(b, e, "batch: x sweeps");
-    }
-    for (const batch_group& g : b->groups[ph]) {
         if (g.seg && b->sweep == 0)
-            e = adsg::launch_sweep_seg_batch<T>(g.p, g.p, static_cast<const adsg::seg_item<T>*>(g.gy_items),
-                                                g.nitems, g.gy_blocks, b->stream);
+            e = adsg::launch_sweep_seg_batch<T>(static_cast<const adsg::seg_item<T>*>(g.gy_items), g.nitems,
+                                                g.gy_blocks, b->stream);
         else
-            e = adsg::launch_sweep_batch<T>(g.p, g.p, static_cast<const adsg::sweep_item<T>*>(g.sy_items),
-                                            g.nitems, g.sy_blocks, b->stream);
+            e = adsg::launch_sweep_batch<T>(static_cast<const adsg::sweep_item<T>*>(g.sy_items), g.nitems,
+                                            g.sy_blocks, b->stream);
         if (e != hipSuccess) return bhip(b, e, "batch: y sweeps");
     }
     if constexpr (sizeof(T) == sizeof(float)) {

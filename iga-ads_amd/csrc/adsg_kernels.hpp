@@ -91,7 +91,7 @@ struct rhs_params {
     T mass = 1, dt = 0, gx = 1, gy = 1, gz = 1, reaction = 0;
     const T* kfield = nullptr;  // coefficients of k on the same basis, or null (k = 1)
     const T* source = nullptr;  // coefficients of f on the same basis, or null (f = 0)
-    // 3D one-launch tiling (element-kernel code bit 8): rhs3_spill_slots(p, elements) values
+    // one-launch tiling (element-kernel code bit 8): rhs_spill_slots(dim, p, elements) values
     // for the block entries a tile does not own; null selects the colour launches. The
     // entries go to 7 arrays, one per neighbour offset delta = dx + 2 dy + 4 dz (1..7) of the
     // owner, at spill_off[delta]; array delta has extent spill_ext[d][delta_d] in direction d
@@ -106,8 +106,9 @@ struct rhs_params {
 
 // variant: kernel tuning choice (0 = coefficient block held in registers, 1 = re-read from
 // LDS, ..., 5 = element columns); bits 4..7 carry the shape of variant 5, bit 8 selects one
-// launch over all tiles with a spill fix-up instead of the 8 parity-colour launches (3D,
-// variants 0, 1, 2 and 4, forms without a k field or a source in the kernel; prm.spill set).
+// launch over all tiles with a spill fix-up instead of the 2^dim parity-colour launches (2D;
+// 3D variants 0, 1, 2 and 4 with forms without a k field or a source in the kernel;
+// prm.spill set).
 template <typename T>
 hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs,
                       const rhs_params<T>& prm, int variant, hipStream_t s);
@@ -117,7 +118,7 @@ constexpr int kRhsOneLaunch = 1 << 8;
 constexpr int kRhsOverwrite = 1 << 9;
 // values the spill array of the one-launch tiling needs for degree p and these element
 // counts (the largest over the variants that use it)
-long rhs3_spill_slots(int p, const int elements[3]);
+long rhs_spill_slots(int dim, int p, const int elements[3]);
 
 // The element loop for any quadrature order, repeated interior knots and a degree per
 // direction (1..3), one lane per element (adsg_rhs_gen.hip); dirs[d].first, nq and p set.
@@ -134,25 +135,29 @@ template <typename T>
 hipError_t launch_rhs3_col(int p, const T* u_prev, T* rhs, const dev_tables<T>* dirs, const rhs_params<T>& prm,
                            int shape, hipStream_t s);
 
-// Batched 2D element kernels: one item per problem (degree p, no k field, no source);
-// ntx2[c] tiles per tile row and first[c] the first block of the item in colour c's launch.
+// Batched 2D element kernels: one item per problem (degree p, no k field, no source).
 template <typename T>
 struct rhs2_item {
     const T* up = nullptr;
-    T* rhs = nullptr;
+    T* rhs = nullptr;  // the problem's rhs (its first DOF)
     dev_tables<T> X, Y;
     rhs_params<T> prm;
-    int ntx2[4] = {0, 0, 0, 0};
-    int first[4] = {0, 0, 0, 0};
+    int ntx2[4] = {0, 0, 0, 0};   // tiles of colour c per tile row
+    int first[4] = {0, 0, 0, 0};  // first block of the item in colour c's launch
 };
 // tiles of each colour for tile shape 0 (16 x 16 elements) or 1 (8 x 8)
 void rhs2_batch_tiles(int shape, int elements_x, int elements_y, int ntx2[4], int count[4]);
+// the problems of degree p in one launch per colour
 template <typename T>
-hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int nitems, int colour,
-                             int blocks, hipStream_t s);
+hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int nitems, int colour, int blocks,
+                             hipStream_t s);
+// spill values of a 2D problem in the one-launch tiling with tiles of tx x ty elements, and
+// (with off, ext) their layout as rhs_params holds it
+long rhs2_spill_layout(int p, int tx, int ty, const int elements[2], long off[8], int ext[3][2]);
 
-// Batched line sweeps of unpivoted factors: one item per (problem, direction); 64 lines
-// per block (x lines staged through LDS, y lines one per lane), first = first block.
+// Batched line sweeps of unpivoted factors: one item per (problem, direction), problems of
+// every degree (kl = ku = p, from f) in one launch; 64 lines per block (x lines staged
+// through LDS, y lines one per lane), first = first block.
 template <typename T>
 struct sweep_item {
     T* u = nullptr;
@@ -161,24 +166,23 @@ struct sweep_item {
     int first = 0;
 };
 template <typename T>
-hipError_t launch_sweep_batch(int kl, int ku, const sweep_item<T>* items, int nitems, int blocks,
-                              hipStream_t s);
+hipError_t launch_sweep_batch(const sweep_item<T>* items, int nitems, int blocks, hipStream_t s);
 
 // Batched segmented sweeps (unpivoted factors): one item per (problem, direction) with its
-// packed rows (line_segments::pack, segmented for at most kBatchSegments segments), 64
-// lines per block of kBatchSegments waves; first = first block of the item.
+// packed rows (line_segments::pack, segmented for at most kBatchSegments segments of at
+// most 32 rows), problems of every degree p in one launch, 64 lines per block of
+// kBatchSegments waves; first = first block of the item.
 constexpr int kBatchSegments = 9;
 template <typename T>
 struct seg_item {
     T* u = nullptr;
     line_geometry g;
     const T* pack = nullptr;
-    int n = 0, m = 0, nseg = 0;
+    int n = 0, m = 0, nseg = 0, p = 0;
     int first = 0;
 };
 template <typename T>
-hipError_t launch_sweep_seg_batch(int kl, int ku, const seg_item<T>* items, int nitems, int blocks,
-                                  hipStream_t s);
+hipError_t launch_sweep_seg_batch(const seg_item<T>* items, int nitems, int blocks, hipStream_t s);
 
 // d/dk of w^T rhs(u; k) for the gradient term of prm, added into out:
 //   out_b += -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d (d_d u)(d_d w)

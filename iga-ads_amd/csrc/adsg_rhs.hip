@@ -699,12 +699,16 @@ __global__ __launch_bounds__(256) void k_rhs3_spill(T* __restrict__ rhs, rhs_par
 // ------------------------------------------------------------------------------------
 // element-loop RHS, 2D
 // ------------------------------------------------------------------------------------
-// One tile of TX x TY elements (tile t of the given parity colour; ntx2 tiles of that
-// colour per tile row), added into rhs.
-template <typename T, int P, int TX, int TY, bool KF, bool SRC>
-__device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restrict__ rhs,
-                                          const dev_tables<T>& X, const dev_tables<T>& Y,
-                                          const rhs_params<T>& prm, int colour, int t, int ntx2) {
+// One tile of TX x TY elements, tile t of the given parity colour (ntx2 tiles of that colour
+// per tile row), added into rhs; or (ONE) tile t of all tiles (ntx2 of them per tile row),
+// whose owned DOF-block entries go to the rhs and the others to the spill arrays of prm
+// (rhs3_put's ownership and layout, in two directions). The LDS arrays are separate
+// __shared__ objects: gathered into one struct, the compiler read the tables with
+// ds_read2_b64 pairs instead of ds_read_b128 and the p = 3 batch kernel ran 1.5x slower.
+template <typename T, int P, int TX, int TY, bool KF, bool SRC, bool ONE>
+__device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restrict__ rhs, const dev_tables<T>& X,
+                                          const dev_tables<T>& Y, const rhs_params<T>& prm, int colour, int t,
+                                          int ntx2) {
     constexpr int M = P + 1, NT = TX * TY;
     constexpr int BX = TX + P, BY = TY + P, NB = BX * BY;
     constexpr int WY = 64 / TX;  // element rows per wave
@@ -719,8 +723,8 @@ __device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restric
     __shared__ double acc[NW * WR * BX];  // wave w owns DOF rows w*WY .. w*WY+WR-1
 
     const int tid = threadIdx.x;
-    const int tx = 2 * (t % ntx2) + (colour & 1);
-    const int ty = 2 * (t / ntx2) + ((colour >> 1) & 1);
+    const int tx = ONE ? t % ntx2 : 2 * (t % ntx2) + (colour & 1);
+    const int ty = ONE ? t / ntx2 : 2 * (t / ntx2) + ((colour >> 1) & 1);
     const int ex0 = tx * TX, ey0 = ty * TY;
     const int nx = X.dofs, ny = Y.dofs;
 
@@ -854,6 +858,7 @@ __device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restric
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     __syncthreads();
 
+    const bool lastx = ex0 + TX >= X.elements, lasty = ey0 + TY >= Y.elements;
     for (int i = tid; i < NB; i += NT) {
         const int bx = i % BX, by = i / BX;
         const int gx = ex0 + bx, gy = ey0 + by;
@@ -863,23 +868,67 @@ __device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restric
             const int r = by - w * WY;
             if (r >= 0 && r < WR) sum += acc[(w * WR + r) * BX + bx];
         }
-        rhs[gx + static_cast<long>(nx) * gy] += static_cast<T>(sum);
+        const long g = gx + static_cast<long>(nx) * gy;
+        if constexpr (ONE) {
+            const int dx = bx >= TX && !lastx, dy = by >= TY && !lasty;
+            if (!(dx | dy)) {
+                rhs[g] = prm.store ? static_cast<T>(sum) : rhs[g] + static_cast<T>(sum);
+            } else {
+                const int cx = dx ? (tx + 1) * P + bx - TX : gx;
+                const int cy = dy ? (ty + 1) * P + by - TY : gy;
+                prm.spill[spill_index(prm, dx, dy, 0, cx, cy, 0)] = static_cast<T>(sum);
+            }
+        } else {
+            rhs[g] += static_cast<T>(sum);
+        }
     }
 }
 
-template <typename T, int P, int TX, int TY, bool KF, bool SRC>
+// The 2D one-launch tiling's fix-up (rhs, DOF counts nx x ny, ntx x nty tiles of TX x TY
+// elements): the entries of the (up to 3) earlier tiles that share a DOF, added in the order
+// delta = 1, 2, 3 (k_rhs3_spill in two directions).
+template <typename T>
+__device__ __forceinline__ void rhs2_fixup(T* __restrict__ rhs, const rhs_params<T>& prm, int p, int TX, int TY,
+                                           int nx, int ny, int ntx, int nty, long g) {
+    const int gx = static_cast<int>(g % nx), gy = static_cast<int>(g / nx);
+    const int tx = min(gx / TX, ntx - 1), ty = min(gy / TY, nty - 1);
+    const int lx = gx - TX * tx, ly = gy - TY * ty;
+    const bool sx = lx < p && tx > 0, sy = ly < p && ty > 0;
+    if (!(sx || sy)) return;
+    double sum = 0;
+#pragma unroll
+    for (int d = 1; d < 4; ++d) {
+        const int dx = d & 1, dy = d >> 1;
+        if ((dx && !sx) || (dy && !sy)) continue;
+        const long i = prm.spill_off[d] + (dx ? tx * p + lx : gx) +
+                       static_cast<long>(prm.spill_ext[0][dx]) * (dy ? ty * p + ly : gy);
+        sum += static_cast<double>(prm.spill[i]);
+    }
+    rhs[g] += static_cast<T>(sum);
+}
+
+template <typename T, int P, int TX, int TY>
+__global__ __launch_bounds__(256) void k_rhs2_spill(T* __restrict__ rhs, rhs_params<T> prm, int nx, int ny, int ntx,
+                                                    int nty) {
+    const long g = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (g >= static_cast<long>(nx) * ny) return;
+    rhs2_fixup(rhs, prm, P, TX, TY, nx, ny, ntx, nty, g);
+}
+
+template <typename T, int P, int TX, int TY, bool KF, bool SRC, bool ONE>
 __global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __restrict__ rhs,
                                                  dev_tables<T> X, dev_tables<T> Y,
                                                  rhs_params<T> prm, int colour, int ntx2) {
-    rhs2_tile<T, P, TX, TY, KF, SRC>(up, rhs, X, Y, prm, colour, xcd_contiguous(blockIdx.x, gridDim.x),
-                                     ntx2);
+    rhs2_tile<T, P, TX, TY, KF, SRC, ONE>(up, rhs, X, Y, prm, colour, xcd_contiguous(blockIdx.x, gridDim.x), ntx2);
 }
 
-// Many 2D problems of one degree in one launch: block b belongs to the last item whose
-// first[colour] <= b (items are in launch order), and is tile b - first[colour] of it.
-template <typename T, int P, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_rhs2_batch(const rhs2_item<T>* __restrict__ items,
-                                                       int nitems, int colour) {
+// Many 2D problems of degree P in one launch per parity colour: block b belongs to the last
+// item whose first[colour] <= b (items in problem order), and is tile b - first[colour] of
+// it. (A one-launch form with spill arrays measured slower here: every colour launch already
+// has thousands of tiles, and the fix-up pass costs more than the colour tails it removes.)
+template <typename T, int P, int TX>
+__global__ __launch_bounds__(TX* TX) void k_rhs2_batch(const rhs2_item<T>* __restrict__ items, int nitems,
+                                                       int colour) {
     const int b = xcd_contiguous(blockIdx.x, gridDim.x);
     int lo = 0, hi = nitems - 1;
     while (lo < hi) {
@@ -890,8 +939,8 @@ __global__ __launch_bounds__(TX* TY) void k_rhs2_batch(const rhs2_item<T>* __res
             hi = mid - 1;
     }
     const rhs2_item<T>& it = items[lo];
-    rhs2_tile<T, P, TX, TY, false, false>(it.up, it.rhs, it.X, it.Y, it.prm, colour,
-                                          b - it.first[colour], it.ntx2[colour]);
+    rhs2_tile<T, P, TX, TX, false, false, false>(it.up, it.rhs, it.X, it.Y, it.prm, colour, b - it.first[colour],
+                                                 it.ntx2[colour]);
 }
 
 // tile shapes (elements) per dimension and degree: one lane per element
@@ -988,7 +1037,7 @@ hipError_t rhs3_one(const T* up, T* rhs, const dev_tables<T>* d, const rhs_param
     const long tiles = static_cast<long>(ntx) * nty * ntz;
     if (tiles == 0) return hipSuccess;
     if (!prm.spill) return hipErrorInvalidValue;
-    // the spill arrays (sized by rhs3_spill_slots for dofs = elements + P)
+    // the spill arrays (sized by rhs_spill_slots for dofs = elements + P)
     rhs_params<T> q = prm;
     const int n[3] = {d[0].dofs, d[1].dofs, d[2].dofs}, nt[3] = {ntx, nty, ntz};
     for (int i = 0; i < 3; ++i)
@@ -1031,12 +1080,50 @@ hipError_t rhs2_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
         const int nx2 = (ntx - cx + 1) / 2, ny2 = (nty - cy + 1) / 2;
         const long blocks = static_cast<long>(nx2) * ny2;
         if (blocks == 0) continue;
-        hipLaunchKernelGGL((k_rhs2<T, P, TL::X, TL::Y, KF, SRC>), dim3(blocks),
+        hipLaunchKernelGGL((k_rhs2<T, P, TL::X, TL::Y, KF, SRC, false>), dim3(blocks),
                            dim3(TL::X * TL::Y), 0, s, up, rhs, d[0], d[1], prm, c, nx2);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// the 2D spill arrays: array delta (1..3) has extent n[d] in the directions with delta_d = 0
+// and nt[d] P in the others; returns the total (off, ext as rhs_params holds them)
+long spill_layout2(int p, const int n[2], const int nt[2], long off[8], int ext[3][2]) {
+    for (int d = 0; d < 2; ++d) {
+        ext[d][0] = n[d];
+        ext[d][1] = nt[d] * p;
+    }
+    ext[2][0] = ext[2][1] = 1;
+    long total = 0;
+    for (int k = 0; k < 8; ++k) off[k] = 0;
+    for (int delta = 1; delta < 4; ++delta) {
+        off[delta] = total;
+        total += static_cast<long>(ext[0][delta & 1]) * ext[1][delta >> 1];
+    }
+    return total;
+}
+
+// one launch over all tiles, then the fix-up over the DOFs
+template <typename T, int P, bool KF, bool SRC>
+hipError_t rhs2_one(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm, hipStream_t s) {
+    using TL = tile<2, P>;
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X, nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    const long tiles = static_cast<long>(ntx) * nty;
+    if (tiles == 0) return hipSuccess;
+    if (!prm.spill || d[0].dofs != d[0].elements + P || d[1].dofs != d[1].elements + P) return hipErrorInvalidValue;
+    rhs_params<T> q = prm;
+    const int n[2] = {d[0].dofs, d[1].dofs}, nt[2] = {ntx, nty};
+    spill_layout2(P, n, nt, q.spill_off, q.spill_ext);
+    hipLaunchKernelGGL((k_rhs2<T, P, TL::X, TL::Y, KF, SRC, true>), dim3(tiles), dim3(TL::X * TL::Y), 0, s, up, rhs,
+                       d[0], d[1], q, 0, ntx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const long N = static_cast<long>(n[0]) * n[1];
+    hipLaunchKernelGGL((k_rhs2_spill<T, P, TL::X, TL::Y>), dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0,
+                       s, rhs, q, n[0], n[1], ntx, nty);
+    return hipGetLastError();
 }
 
 template <typename T, int P, int FM, int VAR>
@@ -1077,8 +1164,8 @@ hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
     if (dim == 3 && variant == 5 && !rhs3_col_supported<T>(P, prm)) variant = P == 3 ? 2 : 1;
     // element columns on the matrix cores where the form allows, else the degree's default
     // per-element variant
-    const bool one = dim == 3 && (code & kRhsOneLaunch) != 0 && variant != 3 && variant != 5 && prm.spill &&
-                     !kf && !src;
+    const bool one = (code & kRhsOneLaunch) != 0 && prm.spill &&
+                     (dim == 2 || (variant != 3 && variant != 5 && !kf && !src));
     prm.store = one && overwrite;
     if (overwrite && !one) {
         const long n = static_cast<long>(d[0].dofs) * d[1].dofs * (dim == 3 ? d[2].dofs : 1);
@@ -1092,6 +1179,12 @@ hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
         if (variant == 3) return rhs3_fields<T, P, 3>(up, rhs, d, prm, one, s);
         if (variant == 4) return rhs3_fields<T, P, 4>(up, rhs, d, prm, one, s);
         return rhs3_fields<T, P, 0>(up, rhs, d, prm, one, s);
+    }
+    if (one) {
+        if (kf && src) return rhs2_one<T, P, true, true>(up, rhs, d, prm, s);
+        if (kf) return rhs2_one<T, P, true, false>(up, rhs, d, prm, s);
+        if (src) return rhs2_one<T, P, false, true>(up, rhs, d, prm, s);
+        return rhs2_one<T, P, false, false>(up, rhs, d, prm, s);
     }
     if (kf && src) return rhs2_colours<T, P, true, true>(up, rhs, d, prm, s);
     if (kf) return rhs2_colours<T, P, true, false>(up, rhs, d, prm, s);
@@ -1116,7 +1209,18 @@ hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<
     }
 }
 
-long rhs3_spill_slots(int p, const int elements[3]) {
+long rhs2_spill_layout(int p, int tx, int ty, const int elements[2], long off[8], int ext[3][2]) {
+    const int n[2] = {elements[0] + p, elements[1] + p};
+    const int nt[2] = {(elements[0] + tx - 1) / tx, (elements[1] + ty - 1) / ty};
+    return spill_layout2(p, n, nt, off, ext);
+}
+
+long rhs_spill_slots(int dim, int p, const int elements[3]) {
+    if (dim == 2) {
+        long off[8];
+        int ext[3][2];
+        return p >= 1 && p <= 3 ? rhs2_spill_layout(p, 16, 16, elements, off, ext) : 0;
+    }
     switch (p) {
         case 1:
             return std::max({rhs3_one_slots<1, 1>(elements), rhs3_one_slots<1, 4>(elements)});
@@ -1140,14 +1244,14 @@ void rhs2_batch_tiles(int shape, int elements_x, int elements_y, int ntx2[4], in
 }
 
 template <typename T>
-hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int nitems, int colour,
-                             int blocks, hipStream_t s) {
+hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int nitems, int colour, int blocks,
+                             hipStream_t s) {
     if (blocks <= 0 || nitems <= 0) return hipSuccess;
-#define ADSG_RB(P_, TX_)                                                                       \
-    if (p == P_ && TX_ == (shape == 0 ? 16 : 8)) {                                             \
-        hipLaunchKernelGGL((k_rhs2_batch<T, P_, TX_, TX_>), dim3(blocks), dim3(TX_ * TX_), 0, s, \
-                           items, nitems, colour);                                              \
-        return hipGetLastError();                                                               \
+#define ADSG_RB(P_, TX_)                                                                                  \
+    if (p == P_ && TX_ == (shape == 0 ? 16 : 8)) {                                                        \
+        hipLaunchKernelGGL((k_rhs2_batch<T, P_, TX_>), dim3(blocks), dim3(TX_ * TX_), 0, s, items, nitems, \
+                           colour);                                                                       \
+        return hipGetLastError();                                                                         \
     }
     ADSG_RB(1, 16)
     ADSG_RB(2, 16)
@@ -1159,10 +1263,8 @@ hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int ni
     return hipErrorInvalidValue;
 }
 
-template hipError_t launch_rhs2_batch<double>(int, int, const rhs2_item<double>*, int, int, int,
-                                              hipStream_t);
-template hipError_t launch_rhs2_batch<float>(int, int, const rhs2_item<float>*, int, int, int,
-                                             hipStream_t);
+template hipError_t launch_rhs2_batch<double>(int, int, const rhs2_item<double>*, int, int, int, hipStream_t);
+template hipError_t launch_rhs2_batch<float>(int, int, const rhs2_item<float>*, int, int, int, hipStream_t);
 template hipError_t launch_rhs<double>(int, int, const double*, double*, const dev_tables<double>*,
                                        const rhs_params<double>&, int, hipStream_t);
 template hipError_t launch_rhs<float>(int, int, const float*, float*, const dev_tables<float>*,

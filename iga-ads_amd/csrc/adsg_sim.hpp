@@ -85,8 +85,8 @@ struct adsg_sim {
     int rhs_col_shape = 0;  // variant 5: columns per wave and chunks per z-range (ADSG_TUNE_RHS_COLUMNS)
     int sweep_variant = 0;  // 0: segmented line sweeps where possible, 1: one lane per line
     int rhs_kron = 0;       // 1: separable integrands through the 1D matrices (adsg_set_tuning)
-    // 1: 3D element kernels in one launch over all tiles, the entries a tile does not own
-    // spilled to `spill` (rhs3_spill_slots values in the step precision) and added after
+    // 1: element kernels in one launch over all tiles, the entries a tile does not own
+    // spilled to `spill` (rhs_spill_slots values, FP64-sized) and added after
     int rhs_tiling = 0;
     void* spill = nullptr;
     std::string err;

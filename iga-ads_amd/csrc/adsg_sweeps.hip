@@ -255,9 +255,20 @@ __global__ __launch_bounds__(256) void k_sweep_own(T* __restrict__ u, long n_lin
 
 // Many problems in one launch (unpivoted factors): block b belongs to the last item whose
 // first <= b; x lines (S == 1) are staged through LDS, other lines are one per lane.
-template <typename T, int KL, int KU>
-__global__ __launch_bounds__(64) void k_sweep_batch(const sweep_item<T>* __restrict__ items,
-                                                    int nitems) {
+template <typename T, int P>
+__device__ __forceinline__ void sweep_batch_item(const sweep_item<T>& it, int b, T (&tile)[64][CH + 1]) {
+    const long line0 = static_cast<long>(b - it.first) * 64;
+    if (it.g.S == 1) {
+        contig_lines<T, T, P, P, false>(it.u, line0, it.g.n_lines, it.f, tile);
+    } else {
+        const long l = line0 + threadIdx.x;
+        if (l < it.g.n_lines) strided_line<T, T, P, P, false>(it.u, l, it.g.A, it.g.B, it.g.S, it.f);
+    }
+}
+
+// every degree in one launch (kl = ku = p of the item's factors)
+template <typename T>
+__global__ __launch_bounds__(64) void k_sweep_batch(const sweep_item<T>* __restrict__ items, int nitems) {
     __shared__ T tile[64][CH + 1];
     const int b = blockIdx.x;
     int lo = 0, hi = nitems - 1;
@@ -269,13 +280,10 @@ __global__ __launch_bounds__(64) void k_sweep_batch(const sweep_item<T>* __restr
             hi = mid - 1;
     }
     const sweep_item<T>& it = items[lo];
-    const long line0 = static_cast<long>(b - it.first) * 64;
-    if (it.g.S == 1) {
-        contig_lines<T, T, KL, KU, false>(it.u, line0, it.g.n_lines, it.f, tile);
-    } else {
-        const long l = line0 + threadIdx.x;
-        if (l < it.g.n_lines) strided_line<T, T, KL, KU, false>(it.u, l, it.g.A, it.g.B, it.g.S, it.f);
-    }
+    const int p = it.f.kl;  // uniform over the workgroup
+    if (p == 1) sweep_batch_item<T, 1>(it, b, tile);
+    if (p == 2) sweep_batch_item<T, 2>(it, b, tile);
+    if (p == 3) sweep_batch_item<T, 3>(it, b, tile);
 }
 
 template <typename T, typename F, int KL, int KU, bool PIV>
@@ -324,20 +332,10 @@ hipError_t launch_line_sweep(T* u, const line_geometry& g, const dev_factors<F>&
 
 
 template <typename T>
-hipError_t launch_sweep_batch(int kl, int ku, const sweep_item<T>* items, int nitems, int blocks,
-                              hipStream_t s) {
+hipError_t launch_sweep_batch(const sweep_item<T>* items, int nitems, int blocks, hipStream_t s) {
     if (blocks <= 0 || nitems <= 0) return hipSuccess;
-#define ADSG_SB(KL_, KU_)                                                                    \
-    if (kl == KL_ && ku == KU_) {                                                            \
-        hipLaunchKernelGGL((k_sweep_batch<T, KL_, KU_>), dim3(blocks), dim3(64), 0, s, items, \
-                           nitems);                                                           \
-        return hipGetLastError();                                                             \
-    }
-    ADSG_SB(1, 1)
-    ADSG_SB(2, 2)
-    ADSG_SB(3, 3)
-#undef ADSG_SB
-    return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sweep_batch<T>, dim3(blocks), dim3(64), 0, s, items, nitems);
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -354,10 +352,8 @@ template hipError_t launch_line_sweep_own<double>(double*, const line_geometry&,
                                                   const own_factors<double>&, hipStream_t);
 template hipError_t launch_line_sweep_own<float>(float*, const line_geometry&,
                                                  const own_factors<float>&, hipStream_t);
-template hipError_t launch_sweep_batch<double>(int, int, const sweep_item<double>*, int, int,
-                                               hipStream_t);
-template hipError_t launch_sweep_batch<float>(int, int, const sweep_item<float>*, int, int,
-                                              hipStream_t);
+template hipError_t launch_sweep_batch<double>(const sweep_item<double>*, int, int, hipStream_t);
+template hipError_t launch_sweep_batch<float>(const sweep_item<float>*, int, int, hipStream_t);
 template hipError_t launch_line_sweep<double, double>(double*, const line_geometry&,
                                                       const dev_factors<double>&, hipStream_t);
 template hipError_t launch_line_sweep<float, float>(float*, const line_geometry&,

@@ -105,3 +105,41 @@ def test_one_launch_graph_steps_equal_eager_and_autotune_keeps_tiling_valid():
     assert a.get_tuning(ag.TUNE_RHS_VARIANT) in range(6)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("p,ne", [(1, (33, 20)), (2, (20, 17)), (3, (40, 9)), (2, (5, 70)), (3, (16, 16))])
+def test_one_launch_rhs_2d_every_form_matches_oracle_and_colours(p, ne):
+    """2D one-launch tiling (ADSG_TUNE_RHS_TILING = 1; 16 x 16 tiles, up to 3 earlier tiles per
+    DOF in the fix-up), every form including a k field and a source in the kernel: against the
+    oracle (<= 1e-13) and the colour launches (<= 1e-14), and deterministic."""
+    _log(f"case: one-launch rhs 2D p={p} ne={ne}")
+    g = ag.Simulation(2, [p, p], list(ne))
+    o = po.Sim(2, [p, p], list(ne))
+    assert g.prepare_matrices() == 0 and o.prepare_matrices() == 0
+    rng = np.random.default_rng(17 + p)
+    c = rng.uniform(0, 1, g.size)
+    kf = np.exp(rng.uniform(-0.5, 0.5, g.size))
+    src = rng.standard_normal(g.size)
+    g.set(ag.U_PREV, c)
+    g.set(ag.KFIELD, kf)
+    g.set(ag.SOURCE, src)
+    dt = 2e-3
+    cases = [
+        (dict(), dict()),
+        (dict(mass=0.0, grad=(1.0, 0.5, 0.0)), dict(mass=0.0, grad=(1.0, 0.5, 0.0))),
+        (dict(kfield=True), dict(kfield=kf)),
+        (dict(reaction=1.0), dict(reaction=1.0)),
+        (dict(source=True), dict(source=src)),
+        (dict(kfield=True, source=True, reaction=0.5), dict(kfield=kf, source=src, reaction=0.5)),
+    ]
+    for gk, ok in cases:
+        out = []
+        for tiling in (0, 1, 1):
+            g.set_tuning(ag.TUNE_RHS_TILING, tiling)
+            g.compute_rhs(ag.rhs_form(dt, **gk))
+            out.append(g.get(ag.U))
+        expect = o.compute_rhs(c, po.heat_form(dt, 2, **ok))
+        assert po.rel_l2(out[1], expect) < 1e-13, gk
+        assert po.rel_l2(out[1], out[0]) < 1e-14, gk
+        assert np.array_equal(out[1], out[2]), gk
+    g.close()
