@@ -206,8 +206,10 @@ adsg::line_geometry geometry(const adsg_sim* h, int dir) {
     const long nx = h->dirs[0].tab.dofs, ny = h->dirs[1].tab.dofs;
     const long nz = h->dim == 3 ? h->dirs[2].tab.dofs : 1;
     adsg::line_geometry g;
-    if (dir == 0) {
+    if (dir == 0) {  // line l starts at l * nx
         g.n_lines = ny * nz;
+        g.A = 1;
+        g.B = nx;
         g.S = 1;
     } else if (dir == 1) {
         g.n_lines = nx * nz;
