@@ -837,16 +837,19 @@ int adsg::create_sim(const adsg_config* cfg, hipStream_t shared, bool vectors, a
     // two waves per SIMD), FP32 5 (element columns on the matrix cores); p <= 2: 4 from 2^21
     // elements (enough tiles of 512 elements to fill the device twice over), else 1. 3D
     // handles start with the one-launch tiling (ADSG_TUNE_RHS_TILING), which was faster for
-    // every variant measured (p = 2 at 128^3, p = 3 at 192^3, both precisions); 2D handles with
-    // the colour launches, which the batches use (so a batched problem equals the same problem
-    // run alone, bit for bit).
+    // every variant measured (p = 2 at 128^3, p = 3 at 192^3, both precisions); so do 2D
+    // handles of at most 2048 tiles of 16 x 16 elements, whose colour launches are too small to
+    // fill the device (128^2 p = 2 with a k field: 0.046 -> 0.029 ms per step). Larger 2D
+    // meshes keep the colour launches, as the batches do (with thousands of tiles per launch
+    // the fix-up pass costs more than the colour tails it removes).
     {
         long E = 1;
         for (int d = 0; d < h->dim; ++d) E *= h->dirs[d].tab.elements;
         const int p = dc[0]->p;
         h->rhs_variant = p == 3 ? (h->precision == ADSG_FP32 ? 5 : 2) : E >= (1L << 21) ? 4 : 1;
         h->rhs_col_shape = 1;  // variant 5: two columns per wave, one chunk per workgroup
-        h->rhs_tiling = h->dim == 3 && !h->general ? 1 : 0;
+        const long tiles16 = static_cast<long>((h->dirs[0].tab.elements + 15) / 16) * ((h->dirs[1].tab.elements + 15) / 16);
+        h->rhs_tiling = h->general ? 0 : h->dim == 3 || tiles16 <= 2048 ? 1 : 0;
     }
     if (hipGetDevice(&h->device) != hipSuccess) {
         g_create_error = "adsg_create: hipGetDevice failed (no HIP device?)";

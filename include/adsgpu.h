@@ -328,12 +328,15 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
  * of elements along z, the x-y contractions on the matrix cores): bit 0 two columns per
  * wave, bit 1 three chunks of 16 DOF planes per workgroup; the autotuner times all four. */
 #define ADSG_TUNE_RHS_COLUMNS 4
-/* ADSG_TUNE_RHS_TILING (3D, variants 0, 1, 2 and 4, forms without a k field): 0 = 2^3
- * parity-colour launches, whose tiles share no DOF; 1 = one launch over all tiles, each tile
- * adding the entries of its DOF block that it owns and storing the others in a spill array
- * (allocated when the value is first set, a few values per element), which a second kernel
- * adds in a fixed order. Both are deterministic; the sums are grouped differently (equal to
- * rounding). The autotuner times both. */
+/* ADSG_TUNE_RHS_TILING (2D every form; 3D variants 0, 1, 2 and 4 with forms without a k
+ * field or source in the kernel): 0 = 2^dim parity-colour launches, whose tiles share no
+ * DOF; 1 = one launch over all tiles, each tile adding the entries of its DOF block that it
+ * owns and storing the others in a spill array (allocated with the handle, about one to
+ * three values per element), which a second kernel adds in a fixed order. Both are
+ * deterministic; the sums are grouped differently (equal to rounding). Default 1 in 3D and
+ * for 2D meshes of at most 2048 tiles of 16 x 16 elements, 0 for larger 2D meshes (and in
+ * batches, whose problems therefore equal the same problems run alone with 0). The
+ * autotuner times both. */
 #define ADSG_TUNE_RHS_TILING 5
 int adsg_set_tuning(adsg_sim* h, int key, int value);
 int adsg_get_tuning(const adsg_sim* h, int key);

@@ -92,6 +92,7 @@ def test_batch_equals_single_problem_path_bitwise(precision):
         g = ag.Simulation(2, [p, p], [ne, ne], precision=precision)
         g.prepare_matrices()
         g.set_tuning(ag.TUNE_SWEEP_VARIANT, 1)  # one lane per line, as the batch
+        g.set_tuning(ag.TUNE_RHS_TILING, 0)  # colour launches, as the batch's element kernels
         g.set(ag.U, _u0(i))
         g.step(6, _forms()[i])
         assert np.array_equal(g.get(ag.U), b.get(i)), i
@@ -140,6 +141,7 @@ def test_device_setup_on_another_interval_equals_single_problem_path_bitwise():
             g = ag.Simulation(2, [p, p], [ne, ne], precision=precision, a=[-1.0, -1.0], b=[2.0, 2.0])
             g.prepare_matrices()
             g.set_tuning(ag.TUNE_SWEEP_VARIANT, 1)
+            g.set_tuning(ag.TUNE_RHS_TILING, 0)
             g.set(ag.U, u0[i])
             g.step(4, forms[i])
             assert np.array_equal(g.get(ag.U), b.get(i)), (precision, i)

@@ -226,7 +226,8 @@ def test_ensemble256_fp32_vs_fp64_200_steps_one_graph_and_batch_equals_single():
     worst = max(po.rel_l2(a, c) for a, c in zip(out["fp32"], out["fp64"]))
     _log(f"  FP32 worst rel L2 vs FP64 over 256 problems {worst:.3e}")
     assert worst <= 1e-6
-    # one lane per line on both sides: the batched problem equals the problem run alone
+    # one lane per line and colour launches on both sides: the batched problem equals the
+    # problem run alone
     b = _batch(shape, forms, u0, ag.FP64, sweep=1)
     b.step(steps)
     picks = sorted({max(range(len(probs)), key=lambda j: (probs[j]["p"] == p, probs[j]["ne"]))
@@ -236,6 +237,7 @@ def test_ensemble256_fp32_vs_fp64_200_steps_one_graph_and_batch_equals_single():
         g = ag.Simulation(2, [q["p"]] * 2, [q["ne"]] * 2)
         g.prepare_matrices()
         g.set_tuning(ag.TUNE_SWEEP_VARIANT, 1)
+        g.set_tuning(ag.TUNE_RHS_TILING, 0)
         g.set(ag.U, u0[i])
         g.step(steps, forms[i])
         assert np.array_equal(g.get(ag.U), b.get(i)), i
