@@ -80,6 +80,7 @@ TUNE_RHS_KRONECKER = 3
 TUNE_BATCH_TILE = 2
 TUNE_RHS_COLUMNS = 4
 TUNE_RHS_TILING = 5
+TUNE_CACHE = 6
 
 
 def lib():

@@ -1636,6 +1636,7 @@ int adsg_set_tuning(adsg_sim* h, int key, int value) {
                 : key == ADSG_TUNE_RHS_KRONECKER ? &h->rhs_kron
                 : key == ADSG_TUNE_RHS_COLUMNS   ? &h->rhs_col_shape
                 : key == ADSG_TUNE_RHS_TILING    ? &h->rhs_tiling
+                : key == ADSG_TUNE_CACHE         ? &h->tune_cache
                                                  : nullptr;
     const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 5 : key == ADSG_TUNE_RHS_COLUMNS ? 3 : 1;
     if (!slot || value < 0 || value > max_value) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
@@ -1643,7 +1644,7 @@ int adsg_set_tuning(adsg_sim* h, int key, int value) {
         const int rc = ensure_spill(h);
         if (rc) return rc;
     }
-    if (*slot != value) drop_graphs(h);
+    if (*slot != value && key != ADSG_TUNE_CACHE) drop_graphs(h);
     *slot = value;
     return ADSG_OK;
 }
@@ -1655,12 +1656,60 @@ int adsg_get_tuning(const adsg_sim* h, int key) {
     if (key == ADSG_TUNE_RHS_KRONECKER) return h->rhs_kron;
     if (key == ADSG_TUNE_RHS_COLUMNS) return h->rhs_col_shape;
     if (key == ADSG_TUNE_RHS_TILING) return h->rhs_tiling;
+    if (key == ADSG_TUNE_CACHE) return h->tune_cache;
     return ADSG_EINVAL;
 }
+
+}  // extern "C"
+
+namespace {
+
+// The autotuner's results per configuration, shared by the handles of the process: the
+// device, dimension, precision, degree, element count and Gauss points of every direction,
+// the kind of form (mass term, k field, source, reaction), delta or explicit steps and the
+// step flags. The element and sweep variants' relative speed depends on these only.
+std::mutex g_tune_mutex;
+std::map<std::string, std::array<int, 4>> g_tune_cache;  // {variant, column shape, tiling, sweep}
+
+std::string tune_key(const adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags) {
+    std::string k = std::to_string(h->device) + "/" + std::to_string(h->dim) + "/" + std::to_string(h->precision);
+    for (int d = 0; d < h->dim; ++d)
+        k += "/" + std::to_string(h->dirs[d].tab.p) + "," + std::to_string(h->dirs[d].tab.elements) + "," +
+             std::to_string(h->dirs[d].tab.nq) + "," + std::to_string(h->dirs[d].tab.repeated_nodes);
+    k += "/" + std::to_string(form->mass == 0.0) + std::to_string(form->use_kfield != 0) +
+         std::to_string(form->use_source != 0) + std::to_string(form->reaction != 0.0);
+    k += "/" + std::to_string(slots != nullptr) + "/" + std::to_string(flags);
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
 
 int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags) {
     if (!h || !form) return fail(h, ADSG_EINVAL, "autotune: bad argument");
     if (!h->prepared) return fail(h, ADSG_ESTATE, "autotune: prepare_matrices first");
+    const std::string key = tune_key(h, form, slots, flags);
+    if (h->tune_cache) {
+        std::array<int, 4> c{};
+        bool hit = false;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) {
+                c = it->second;
+                hit = true;
+            }
+        }
+        if (hit && (c[2] == 0 || ensure_spill(h) == ADSG_OK)) {
+            h->rhs_variant = c[0];
+            h->rhs_col_shape = c[1];
+            h->rhs_tiling = c[2];
+            h->sweep_variant = c[3];
+            drop_graphs(h);
+            return ADSG_OK;
+        }
+    }
     // keep the caller's state: copies of u, u_prev (and their FP32 shadows)
     std::vector<void*> saved;
     std::vector<std::pair<void*, size_t>> what = {{h->vec[ADSG_U], h->N * sizeof(double)},
@@ -1729,6 +1778,10 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
     h->rhs_tiling = best_tiling;
     h->sweep_variant = best_sweep;
     drop_graphs(h);
+    if (rc == ADSG_OK && h->tune_cache) {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune_cache[key] = {best, best_shape, best_tiling, best_sweep};
+    }
     return rc;
 }
 

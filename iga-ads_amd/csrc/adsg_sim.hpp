@@ -89,6 +89,7 @@ struct adsg_sim {
     // spilled to `spill` (rhs_spill_slots values, FP64-sized) and added after
     int rhs_tiling = 0;
     void* spill = nullptr;
+    int tune_cache = 1;  // 1: adsg_autotune reuses / records the process's results per configuration
     std::string err;
     hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
     double* norm_scratch = nullptr;  // adsg_norm: block partial sums and the total

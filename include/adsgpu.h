@@ -338,6 +338,12 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
  * batches, whose problems therefore equal the same problems run alone with 0). The
  * autotuner times both. */
 #define ADSG_TUNE_RHS_TILING 5
+/* ADSG_TUNE_CACHE (default 1): adsg_autotune records its choices per configuration (device,
+ * dimension, precision, degree / elements / Gauss points / repeated nodes per direction, the
+ * kind of form, delta or explicit steps, the step flags) for the process, and a handle of a
+ * configuration already tuned takes them without timing; 0 on a handle times afresh (and
+ * records nothing). */
+#define ADSG_TUNE_CACHE 6
 int adsg_set_tuning(adsg_sim* h, int key, int value);
 int adsg_get_tuning(const adsg_sim* h, int key);
 int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int flags);

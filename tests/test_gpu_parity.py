@@ -454,6 +454,40 @@ def test_segmented_sweep_on_implicit_matrices():
         assert po.rel_l2(g.get(ag.U), o.solve(b.copy(), slots=slots)) < 1e-13
 
 
+def test_autotune_cache_reuses_the_choices_of_a_configuration():
+    """adsg_autotune records its choices per configuration (ADSG_TUNE_CACHE, default 1): a
+    second handle of the same configuration and form takes them without timing any step;
+    with the cache off a handle times afresh."""
+    import time
+
+    _log("case: autotune cache 3D p=2 22x18x16")
+    a, o = _pair(3, 2, (22, 18, 16))
+    b, _ = _pair(3, 2, (22, 18, 16))
+    c, _ = _pair(3, 2, (22, 18, 16))
+    u = o.init_gaussians(seed=4, count=2, sigma=(0.05, 0.15), centre=(0.2, 0.8), noise=1e-3)
+    form = ag.rhs_form(1e-4, reaction=0.5)  # a form no other test of this module tunes
+    keys = (ag.TUNE_RHS_VARIANT, ag.TUNE_RHS_COLUMNS, ag.TUNE_RHS_TILING, ag.TUNE_SWEEP_VARIANT)
+    for h in (a, b, c):
+        h.set(ag.U, u)
+    t0 = time.perf_counter()
+    a.autotune(form)
+    t1 = time.perf_counter()
+    b.autotune(form)
+    t2 = time.perf_counter()
+    assert [a.get_tuning(k) for k in keys] == [b.get_tuning(k) for k in keys]
+    assert t2 - t1 < 0.5 * (t1 - t0), (t1 - t0, t2 - t1)
+    assert c.get_tuning(ag.TUNE_CACHE) == 1
+    c.set_tuning(ag.TUNE_CACHE, 0)
+    c.autotune(form)
+    assert c.get_tuning(ag.TUNE_RHS_VARIANT) in range(6)
+    # the tuned handles still step correctly
+    b.step(1, form)
+    ref = o.solve(o.compute_rhs(u, po.heat_form(1e-4, 3, reaction=0.5), threads=8))
+    assert po.rel_l2(b.get(ag.U), ref) <= FP64_STEP_TOL
+    for h in (a, b, c):
+        h.close()
+
+
 def test_autotune_keeps_state_and_picks_a_variant():
     _log("case: autotune 3D p=2 24^3")
     g, o = _pair(3, 2, (24, 24, 24))
