@@ -609,6 +609,7 @@ int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
         prm.gx = form->grad[0];
         prm.gy = form->grad[1];
         prm.gz = form->grad[2];
+        prm.spill = h->rhs_tiling ? static_cast<double*>(h->spill) : nullptr;  // 2D: the element tiling's
         e = adsg::launch_kgrad<double>(h->dim, h->dirs[0].tab.p, un, w,
                                        static_cast<double*>(h->vec[ADSG_GRAD_K]), t, prm, h->stream);
         if (e != hipSuccess) return hip_fail(h, e, "k-field gradient");

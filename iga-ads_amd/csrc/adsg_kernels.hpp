@@ -186,6 +186,9 @@ hipError_t launch_sweep_seg_batch(const seg_item<T>* items, int nitems, int bloc
 
 // d/dk of w^T rhs(u; k) for the gradient term of prm, added into out:
 //   out_b += -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d (d_d u)(d_d w)
+// on the tiles of the element loop (FP64; dofs = elements + p), in one launch and the fix-up
+// of its spill arrays when prm.spill is set (rhs_spill_slots(dim, p, ...) values), else one
+// launch per parity colour
 template <typename T>
 hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,
                         const rhs_params<T>& prm, hipStream_t s);

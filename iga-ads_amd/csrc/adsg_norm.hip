@@ -1,12 +1,5 @@
-// HIP kernels of the adjoint step and of the norm functionals (gfx950). The adjoint needs the
-// derivative of w^T rhs(u; k) with respect to the B-spline coefficients of the diffusion
-// field k. The reference has no adjoint code;
-// the form follows its variable-coefficient integrand (flow.hpp:73-100, validation.hpp:
-// 110-115):  d/dk_b = -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d (d_d u)(d_d w).
-//
-// One lane per element. Elements whose indices agree modulo p+1 in every direction share
-// no DOF, so the (p+1)^dim colour launches add their local blocks into the output with
-// plain read-modify-writes, deterministically.
+// HIP kernels of the norm and error functionals (gfx950). (The adjoint's k-field gradient
+// runs on the element tiles of adsg_rhs.hip: kgrad2_tile, kgrad3_element.)
 #include <hip/hip_runtime.h>
 
 #include "adsg_kernels.hpp"
@@ -14,111 +7,6 @@
 namespace adsg {
 
 namespace {
-
-template <typename T, int DIM, int P>
-__global__ __launch_bounds__(256) void k_kgrad(const T* __restrict__ u, const T* __restrict__ w,
-                                               T* __restrict__ out, dev_tables<T> tx,
-                                               dev_tables<T> ty, dev_tables<T> tz, T scale, T gx,
-                                               T gy, T gz, int cx, int cy, int cz, int mx, int my,
-                                               int mz) {
-    constexpr int n = P + 1;
-    const long lane = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (lane >= static_cast<long>(mx) * my * mz) return;
-    const int ix = static_cast<int>(lane % mx);
-    const long r = lane / mx;
-    const int iy = static_cast<int>(r % my), iz = static_cast<int>(r / my);
-    const int ex = cx + n * ix, ey = cy + n * iy, ez = DIM == 3 ? cz + n * iz : 0;
-    const long nx = tx.dofs, ny = ty.dofs;
-    constexpr int NZ = DIM == 3 ? n : 1;
-    T U[NZ][n][n], W[NZ][n][n], G[NZ][n][n];
-    for (int c = 0; c < NZ; ++c)
-        for (int b = 0; b < n; ++b)
-            for (int a = 0; a < n; ++a) {
-                const long gi = (ex + a) + nx * ((ey + b) + ny * static_cast<long>(ez + c));
-                U[c][b][a] = u[gi];
-                W[c][b][a] = w[gi];
-                G[c][b][a] = T(0);
-            }
-    const T* vx = tx.val + static_cast<long>(ex) * n * n;
-    const T* dx = tx.der + static_cast<long>(ex) * n * n;
-    const T* vy = ty.val + static_cast<long>(ey) * n * n;
-    const T* dy = ty.der + static_cast<long>(ey) * n * n;
-    const T* vz = DIM == 3 ? tz.val + static_cast<long>(ez) * n * n : nullptr;
-    const T* dz = DIM == 3 ? tz.der + static_cast<long>(ez) * n * n : nullptr;
-    for (int qz = 0; qz < NZ; ++qz)
-        for (int qy = 0; qy < n; ++qy)
-            for (int qx = 0; qx < n; ++qx) {
-                T ux = 0, uy = 0, uz = 0, wx = 0, wy = 0, wz = 0;
-                for (int c = 0; c < NZ; ++c) {
-                    const T bz = DIM == 3 ? vz[qz * n + c] : T(1);
-                    const T bdz = DIM == 3 ? dz[qz * n + c] : T(0);
-                    for (int b = 0; b < n; ++b) {
-                        const T by = vy[qy * n + b], bdy = dy[qy * n + b];
-                        for (int a = 0; a < n; ++a) {
-                            const T bx = vx[qx * n + a], bdx = dx[qx * n + a];
-                            const T sx = bdx * by * bz, sy = bx * bdy * bz, sz = bx * by * bdz;
-                            ux += U[c][b][a] * sx;
-                            uy += U[c][b][a] * sy;
-                            wx += W[c][b][a] * sx;
-                            wy += W[c][b][a] * sy;
-                            if (DIM == 3) {
-                                uz += U[c][b][a] * sz;
-                                wz += W[c][b][a] * sz;
-                            }
-                        }
-                    }
-                }
-                T wq = tx.wj[ex * n + qx] * ty.wj[ey * n + qy];
-                if (DIM == 3) wq *= tz.wj[ez * n + qz];
-                T g = gx * ux * wx + gy * uy * wy;
-                if (DIM == 3) g += gz * uz * wz;
-                g *= scale * wq;
-                for (int c = 0; c < NZ; ++c) {
-                    const T bz = DIM == 3 ? vz[qz * n + c] : T(1);
-                    for (int b = 0; b < n; ++b)
-                        for (int a = 0; a < n; ++a) G[c][b][a] += g * (vx[qx * n + a] * vy[qy * n + b] * bz);
-                }
-            }
-    for (int c = 0; c < NZ; ++c)
-        for (int b = 0; b < n; ++b)
-            for (int a = 0; a < n; ++a) {
-                const long gi = (ex + a) + nx * ((ey + b) + ny * static_cast<long>(ez + c));
-                out[gi] += G[c][b][a];
-            }
-}
-
-template <typename T, int DIM, int P>
-hipError_t kgrad_colours(const T* u, const T* w, T* out, const dev_tables<T>* d,
-                         const rhs_params<T>& prm, hipStream_t s) {
-    constexpr int n = P + 1;
-    const dev_tables<T> none{};
-    const int ez_n = DIM == 3 ? n : 1;
-    for (int cz = 0; cz < ez_n; ++cz)
-        for (int cy = 0; cy < n; ++cy)
-            for (int cx = 0; cx < n; ++cx) {
-                // elements e = c + n*i < elements
-                const int mx = (d[0].elements - cx + n - 1) / n;
-                const int my = (d[1].elements - cy + n - 1) / n;
-                const int mz = DIM == 3 ? (d[2].elements - cz + n - 1) / n : 1;
-                const long lanes = static_cast<long>(mx) * my * mz;
-                if (mx <= 0 || my <= 0 || mz <= 0) continue;
-                const unsigned blocks = static_cast<unsigned>((lanes + 255) / 256);
-                hipLaunchKernelGGL((k_kgrad<T, DIM, P>), dim3(blocks), dim3(256), 0, s, u, w, out,
-                                   d[0], d[1], DIM == 3 ? d[2] : none, -prm.dt, prm.gx, prm.gy,
-                                   prm.gz, cx, cy, cz, mx, my, mz);
-                hipError_t e = hipGetLastError();
-                if (e != hipSuccess) return e;
-            }
-    return hipSuccess;
-}
-
-template <typename T, int P>
-hipError_t kgrad_dim(int dim, const T* u, const T* w, T* out, const dev_tables<T>* d,
-                     const rhs_params<T>& prm, hipStream_t s) {
-    if (dim == 2) return kgrad_colours<T, 2, P>(u, w, out, d, prm, s);
-    if (dim == 3) return kgrad_colours<T, 3, P>(u, w, out, d, prm, s);
-    return hipErrorInvalidValue;
-}
 
 // Squared norm and error functionals (basic_simulation_2d.hpp:42-52, :266-362): per element
 // and Gauss point w J phi(d), d = u_h - s, with u_h the spline of the coefficients u (or 0
@@ -286,24 +174,5 @@ template hipError_t launch_norm<double>(int, int, const double*, const double*, 
                                         int, double*, int, double*, hipStream_t);
 template hipError_t launch_norm<float>(int, int, const double*, const double*, const dev_tables<float>*,
                                        int, double*, int, double*, hipStream_t);
-
-template <typename T>
-hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,
-                        const rhs_params<T>& prm, hipStream_t s) {
-    switch (p) {
-        case 1:
-            return kgrad_dim<T, 1>(dim, u, w, out, dirs, prm, s);
-        case 2:
-            return kgrad_dim<T, 2>(dim, u, w, out, dirs, prm, s);
-        case 3:
-            return kgrad_dim<T, 3>(dim, u, w, out, dirs, prm, s);
-        default:
-            return hipErrorInvalidValue;
-    }
-}
-
-template hipError_t launch_kgrad<double>(int, int, const double*, const double*, double*,
-                                         const dev_tables<double>*, const rhs_params<double>&,
-                                         hipStream_t);
 
 }  // namespace adsg

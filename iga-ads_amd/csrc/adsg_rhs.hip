@@ -545,6 +545,190 @@ __global__ __launch_bounds__(TX* TY* TZ, (rhs3_waves<T, P, VAR>())) void k_rhs3(
     rhs3_add_block<ONE, T, P, TX, TY, TZ>(acc, o, sink, prm, X.dofs, Y.dofs, Z.dofs, tid);
 }
 
+// d/dk_b of w^T rhs(u; k) for one element of a 3D tile (lane lx, ly of wave lz; u in L.blk,
+// w in L.kb), added to the wave's DOF planes as rhs3_element adds its contribution. The
+// diffusion term of the element loop with a coefficient field (validation.hpp:110-115) is
+// linear in k:  d/dk_b = -dt sum_e sum_q w_q J_e B_b(x_q) sum_d grad_d (d_d u)(d_d w).
+// Sum-factorised: u, w and their x derivatives contracted along x for one x Gauss point at a
+// time, then along y and z; the weighted product tested against the value tables, z, y, x.
+// At p = 3 the element's block would not fit the registers: its part for each x point is
+// added to the planes at once.
+template <typename T, int P, int TX, int TY, int TZ>
+__device__ __forceinline__ void kgrad3_element(const rhs3_buf<T, P, TX, TY, TZ, true, false>& L, double* acc,
+                                               const dev_tables<T>& X, const dev_tables<T>& Y,
+                                               const dev_tables<T>& Z, const rhs_params<T>& prm, int lx, int ly,
+                                               int lz, int ex0, int ey0, int ez0) {
+    using LDS = rhs3_buf<T, P, TX, TY, TZ, true, false>;
+    constexpr int M = P + 1, BX = LDS::BX, BY = LDS::BY, PL = LDS::PL;
+    constexpr bool FLUSH = P == 3;
+    constexpr int RM = FLUSH ? 1 : M;
+    const int ez = ez0 + lz;
+    double* mine = acc + lz * M * PL;
+    T R[RM][RM][RM];
+#pragma unroll
+    for (int a = 0; a < RM; ++a)
+#pragma unroll
+        for (int b = 0; b < RM; ++b)
+#pragma unroll
+            for (int c = 0; c < RM; ++c) R[a][b][c] = T(0);
+    if (ez < Z.elements) {  // wave-uniform
+        T Bz[M][M], Dz[M][M], wz[M];
+        const T* zv = Z.val + static_cast<long>(ez) * M * M;
+        const T* zd = Z.der + static_cast<long>(ez) * M * M;
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            wz[q] = uniform_load(Z.wj + static_cast<long>(ez) * M + q);
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+                Bz[q][c] = uniform_load(zv + q * M + c);
+                Dz[q][c] = uniform_load(zd + q * M + c);
+            }
+        }
+        const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy, cz = -prm.dt * prm.gz;
+        if (ex0 + lx < X.elements && ey0 + ly < Y.elements) {
+#pragma unroll 1
+            for (int qx = 0; qx < M; ++qx) {
+                T bx[M], dx[M];
+#pragma unroll
+                for (int a = 0; a < M; ++a) {
+                    bx[a] = L.xt[FV][(lx * M + qx) * M + a];
+                    dx[a] = L.xt[FD][(lx * M + qx) * M + a];
+                }
+                T U0[M][M], U1[M][M], W0[M][M], W1[M][M];  // along x: value, x derivative
+#pragma unroll
+                for (int b = 0; b < M; ++b)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        T u0 = 0, u1 = 0, w0 = 0, w1 = 0;
+#pragma unroll
+                        for (int a = 0; a < M; ++a) {
+                            const int li = (lx + a) + BX * ((ly + b) + BY * (lz + c));
+                            const T cu = L.blk[li], cw = L.kb[li];
+                            u0 += bx[a] * cu;
+                            u1 += dx[a] * cu;
+                            w0 += bx[a] * cw;
+                            w1 += dx[a] * cw;
+                        }
+                        U0[b][c] = u0;
+                        U1[b][c] = u1;
+                        W0[b][c] = w0;
+                        W1[b][c] = w1;
+                    }
+                T K[M][M];
+#pragma unroll
+                for (int b = 0; b < M; ++b)
+#pragma unroll
+                    for (int c = 0; c < M; ++c) K[b][c] = T(0);
+#pragma unroll 1
+                for (int qy = 0; qy < M; ++qy) {
+                    T by[M], dy[M];
+#pragma unroll
+                    for (int b = 0; b < M; ++b) {
+                        by[b] = L.yt[FV][(ly * M + qy) * M + b];
+                        dy[b] = L.yt[FD][(ly * M + qy) * M + b];
+                    }
+                    // along y: (value, value), (value, y derivative), (x derivative, value)
+                    T u00[M], u01[M], u10[M], w00[M], w01[M], w10[M];
+#pragma unroll
+                    for (int c = 0; c < M; ++c) {
+                        T s0 = 0, s1 = 0, s2 = 0, t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+                        for (int b = 0; b < M; ++b) {
+                            s0 += by[b] * U0[b][c];
+                            s1 += dy[b] * U0[b][c];
+                            s2 += by[b] * U1[b][c];
+                            t0 += by[b] * W0[b][c];
+                            t1 += dy[b] * W0[b][c];
+                            t2 += by[b] * W1[b][c];
+                        }
+                        u00[c] = s0;
+                        u01[c] = s1;
+                        u10[c] = s2;
+                        w00[c] = t0;
+                        w01[c] = t1;
+                        w10[c] = t2;
+                    }
+                    T H[M];
+#pragma unroll
+                    for (int c = 0; c < M; ++c) H[c] = T(0);
+#pragma unroll
+                    for (int qz = 0; qz < M; ++qz) {
+                        T ux = 0, uy = 0, uz = 0, wx = 0, wy = 0, wv = 0;
+#pragma unroll
+                        for (int c = 0; c < M; ++c) {
+                            ux += Bz[qz][c] * u10[c];
+                            uy += Bz[qz][c] * u01[c];
+                            uz += Dz[qz][c] * u00[c];
+                            wx += Bz[qz][c] * w10[c];
+                            wy += Bz[qz][c] * w01[c];
+                            wv += Dz[qz][c] * w00[c];
+                        }
+                        const T g = wz[qz] * (cx * (ux * wx) + cy * (uy * wy) + cz * (uz * wv));
+#pragma unroll
+                        for (int c = 0; c < M; ++c) H[c] += Bz[qz][c] * g;
+                    }
+#pragma unroll
+                    for (int b = 0; b < M; ++b) {
+                        const T wb = L.yt[BV][(ly * M + qy) * M + b];
+#pragma unroll
+                        for (int c = 0; c < M; ++c) K[b][c] += wb * H[c];
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < M; ++a) {
+                    const T wa = L.xt[BV][(lx * M + qx) * M + a];
+#pragma unroll
+                    for (int b = 0; b < M; ++b)
+#pragma unroll
+                        for (int c = 0; c < M; ++c) {
+                            if constexpr (FLUSH)
+                                __hip_atomic_fetch_add(mine + c * PL + (lx + a) + BX * (ly + b),
+                                                       static_cast<double>(wa * K[b][c]), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+                            else
+                                R[FLUSH ? 0 : a][FLUSH ? 0 : b][FLUSH ? 0 : c] += wa * K[b][c];
+                        }
+                }
+            }
+        }
+    }
+    if constexpr (!FLUSH) {
+#pragma unroll
+        for (int c = 0; c < M; ++c)
+#pragma unroll
+            for (int b = 0; b < M; ++b)
+#pragma unroll
+                for (int a = 0; a < M; ++a)
+                    __hip_atomic_fetch_add(mine + c * PL + (lx + a) + BX * (ly + b),
+                                           static_cast<double>(R[FLUSH ? 0 : a][FLUSH ? 0 : b][FLUSH ? 0 : c]),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+}
+
+// The 3D k-field gradient over the tiles of the 3D element loop (k_rhs3's tile, staging and
+// block write): u is staged as the coefficient block, w (prm.kfield) as the k block.
+template <typename T, int P, int TX, int TY, int TZ, bool ONE>
+__global__ __launch_bounds__(TX* TY* TZ) void k_kgrad3(const T* __restrict__ u, T* __restrict__ out,
+                                                     dev_tables<T> X, dev_tables<T> Y, dev_tables<T> Z,
+                                                     rhs_params<T> prm, int colour, int ntx2, int nty2) {
+    constexpr int NT = TX * TY * TZ, NA = rhs3_acc_size<P, TX, TY, TZ>();
+    static_assert(TX * TY == 64, "a wave must cover exactly one z-layer of the tile");
+    __shared__ rhs3_lds<T, P, TX, TY, TZ, true, false, 1> S;
+    auto& L = S.b[0];
+    double* acc = S.acc[0];
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = (tid / TX) % TY;
+    const int lz = __builtin_amdgcn_readfirstlane(tid / (TX * TY));
+    const tile3 o = rhs3_tile_of<ONE, TX, TY, TZ>(xcd_contiguous(blockIdx.x, gridDim.x), colour, ntx2, nty2);
+    rhs3_stage(L, o, u, X, Y, Z.dofs, prm, tid);
+    for (int i = tid; i < NA; i += NT) acc[i] = 0.0;
+    __syncthreads();
+    kgrad3_element<T, P, TX, TY, TZ>(L, acc, X, Y, Z, prm, lx, ly, lz, o.ex0, o.ey0, o.ez0);
+    __syncthreads();
+    const block_sink<T> sink = rhs3_sink<ONE, TX, TY, TZ>(out, o, X.elements, Y.elements, Z.elements);
+    rhs3_add_block<ONE, T, P, TX, TY, TZ>(acc, o, sink, prm, X.dofs, Y.dofs, Z.dofs, tid);
+}
+
 // Variant 3: two tiles per workgroup (tiles 2w, 2w+1 of the colour for workgroup w after
 // XCD-contiguous renumbering). The second tile's data is copied into its own LDS buffer by
 // global-to-LDS loads issued before the first tile's integration, so that its HBM/L2
@@ -699,6 +883,59 @@ __global__ __launch_bounds__(256) void k_rhs3_spill(T* __restrict__ rhs, rhs_par
 // ------------------------------------------------------------------------------------
 // element-loop RHS, 2D
 // ------------------------------------------------------------------------------------
+// The end of a 2D tile (TX x TY elements, tile (tx, ty)): the element blocks R of its lanes
+// (lane = element (tid % TX, tid / TX)) are added per wave into the wave's DOF rows of acc
+// (zeroed by the caller before its first barrier; wave w holds element rows w*WY.. and DOF
+// rows w*WY .. w*WY+WR-1), and after one barrier the tile's DOF block is added into rhs, or
+// (ONE) its owned entries go to rhs and the others to the spill arrays of prm.
+template <typename T, int P, int TX, int TY, bool ONE>
+__device__ __forceinline__ void rhs2_flush(const T (&R)[P + 1][P + 1], double* __restrict__ acc, T* __restrict__ rhs,
+                                           const dev_tables<T>& X, const dev_tables<T>& Y, const rhs_params<T>& prm,
+                                           int tx, int ty) {
+    constexpr int M = P + 1, NT = TX * TY;
+    constexpr int BX = TX + P, BY = TY + P, NB = BX * BY;
+    constexpr int WY = 64 / TX, NW = NT / 64, WR = WY + P;
+    const int tid = threadIdx.x;
+    const int lx = tid % TX, ly = tid / TX;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int wy = ly - wave * WY;  // element row inside the wave
+    const int ex0 = tx * TX, ey0 = ty * TY;
+    const int nx = X.dofs, ny = Y.dofs;
+    double* mine = acc + wave * WR * BX;
+#pragma unroll
+    for (int b = 0; b < M; ++b)
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+            __hip_atomic_fetch_add(mine + (wy + b) * BX + lx + a, static_cast<double>(R[a][b]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __syncthreads();
+
+    const bool lastx = ex0 + TX >= X.elements, lasty = ey0 + TY >= Y.elements;
+    for (int i = tid; i < NB; i += NT) {
+        const int bx = i % BX, by = i / BX;
+        const int gx = ex0 + bx, gy = ey0 + by;
+        if (gx >= nx || gy >= ny) continue;
+        double sum = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int r = by - w * WY;
+            if (r >= 0 && r < WR) sum += acc[(w * WR + r) * BX + bx];
+        }
+        const long g = gx + static_cast<long>(nx) * gy;
+        if constexpr (ONE) {
+            const int dx = bx >= TX && !lastx, dy = by >= TY && !lasty;
+            if (!(dx | dy)) {
+                rhs[g] = prm.store ? static_cast<T>(sum) : rhs[g] + static_cast<T>(sum);
+            } else {
+                const int cx = dx ? (tx + 1) * P + bx - TX : gx;
+                const int cy = dy ? (ty + 1) * P + by - TY : gy;
+                prm.spill[spill_index(prm, dx, dy, 0, cx, cy, 0)] = static_cast<T>(sum);
+            }
+        } else {
+            rhs[g] += static_cast<T>(sum);
+        }
+    }
+}
+
 // One tile of TX x TY elements, tile t of the given parity colour (ntx2 tiles of that colour
 // per tile row), added into rhs; or (ONE) tile t of all tiles (ntx2 of them per tile row),
 // whose owned DOF-block entries go to the rhs and the others to the spill arrays of prm
@@ -764,8 +1001,6 @@ __device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restric
     __syncthreads();
 
     const int lx = tid % TX, ly = tid / TX;
-    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
-    const int wy = ly - wave * WY;  // element row inside the wave
     T R[M][M];
 #pragma unroll
     for (int a = 0; a < M; ++a)
@@ -849,39 +1084,7 @@ __device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restric
             }
         }
     }
-    double* mine = acc + wave * WR * BX;
-#pragma unroll
-    for (int b = 0; b < M; ++b)
-#pragma unroll
-        for (int a = 0; a < M; ++a)
-            __hip_atomic_fetch_add(mine + (wy + b) * BX + lx + a, static_cast<double>(R[a][b]),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    __syncthreads();
-
-    const bool lastx = ex0 + TX >= X.elements, lasty = ey0 + TY >= Y.elements;
-    for (int i = tid; i < NB; i += NT) {
-        const int bx = i % BX, by = i / BX;
-        const int gx = ex0 + bx, gy = ey0 + by;
-        if (gx >= nx || gy >= ny) continue;
-        double sum = 0;
-        for (int w = 0; w < NW; ++w) {
-            const int r = by - w * WY;
-            if (r >= 0 && r < WR) sum += acc[(w * WR + r) * BX + bx];
-        }
-        const long g = gx + static_cast<long>(nx) * gy;
-        if constexpr (ONE) {
-            const int dx = bx >= TX && !lastx, dy = by >= TY && !lasty;
-            if (!(dx | dy)) {
-                rhs[g] = prm.store ? static_cast<T>(sum) : rhs[g] + static_cast<T>(sum);
-            } else {
-                const int cx = dx ? (tx + 1) * P + bx - TX : gx;
-                const int cy = dy ? (ty + 1) * P + by - TY : gy;
-                prm.spill[spill_index(prm, dx, dy, 0, cx, cy, 0)] = static_cast<T>(sum);
-            }
-        } else {
-            rhs[g] += static_cast<T>(sum);
-        }
-    }
+    rhs2_flush<T, P, TX, TY, ONE>(R, acc, rhs, X, Y, prm, tx, ty);
 }
 
 // The 2D one-launch tiling's fix-up (rhs, DOF counts nx x ny, ntx x nty tiles of TX x TY
@@ -920,6 +1123,134 @@ __global__ __launch_bounds__(TX* TY) void k_rhs2(const T* __restrict__ up, T* __
                                                  dev_tables<T> X, dev_tables<T> Y,
                                                  rhs_params<T> prm, int colour, int ntx2) {
     rhs2_tile<T, P, TX, TY, KF, SRC, ONE>(up, rhs, X, Y, prm, colour, xcd_contiguous(blockIdx.x, gridDim.x), ntx2);
+}
+
+// d/dk_b of w^T rhs(u; k) over one 2D tile (the tile of rhs2_tile for colour, t, ntx2),
+// added into out like rhs2_tile adds its blocks. The diffusion term of the element loop
+// (validation.hpp:110-115 with a coefficient field) is linear in k, so
+//   d/dk_b = -dt sum_e sum_q w_q J_e B_b(x_q) (gx u_x w_x + gy u_y w_y)(x_q).
+// Per element: u and w and their x derivatives contracted along x for one x Gauss point at a
+// time, then along y for each y point; the weighted product g tested against the value
+// tables, y then x (the weights folded into the backward tables).
+template <typename T, int P, int TX, int TY, bool ONE>
+__device__ __forceinline__ void kgrad2_tile(const T* __restrict__ u, const T* __restrict__ w, T* __restrict__ out,
+                                            const dev_tables<T>& X, const dev_tables<T>& Y, const rhs_params<T>& prm,
+                                            int colour, int t, int ntx2) {
+    constexpr int M = P + 1, NT = TX * TY;
+    constexpr int BX = TX + P, BY = TY + P, NB = BX * BY;
+    constexpr int WY = 64 / TX, NW = NT / 64, WR = WY + P;
+    static_assert(TX <= 64 && 64 % TX == 0 && TY % WY == 0, "waves must cover whole element rows");
+    __shared__ T ub[NB], wb[NB];
+    __shared__ T fvx[TX * M * M], fdx[TX * M * M], bvx[TX * M * M];
+    __shared__ T fvy[TY * M * M], fdy[TY * M * M], bvy[TY * M * M];
+    __shared__ double acc[NW * WR * BX];
+
+    const int tid = threadIdx.x;
+    const int tx = ONE ? t % ntx2 : 2 * (t % ntx2) + (colour & 1);
+    const int ty = ONE ? t / ntx2 : 2 * (t / ntx2) + ((colour >> 1) & 1);
+    const int ex0 = tx * TX, ey0 = ty * TY;
+    const int nx = X.dofs, ny = Y.dofs;
+    for (int i = tid; i < NB; i += NT) {
+        const int gx = ex0 + i % BX, gy = ey0 + i / BX;
+        const bool in = gx < nx && gy < ny;
+        const long g = gx + static_cast<long>(nx) * gy;
+        ub[i] = in ? u[g] : T(0);
+        wb[i] = in ? w[g] : T(0);
+    }
+    for (int i = tid; i < TX * M * M; i += NT) {
+        const int e = ex0 + i / (M * M), q = (i / M) % M;
+        const bool in = e < X.elements;
+        const T v = in ? X.val[static_cast<long>(ex0) * M * M + i] : T(0);
+        fvx[i] = v;
+        fdx[i] = in ? X.der[static_cast<long>(ex0) * M * M + i] : T(0);
+        bvx[i] = v * (in ? X.wj[static_cast<long>(e) * M + q] : T(0));
+    }
+    for (int i = tid; i < TY * M * M; i += NT) {
+        const int e = ey0 + i / (M * M), q = (i / M) % M;
+        const bool in = e < Y.elements;
+        const T v = in ? Y.val[static_cast<long>(ey0) * M * M + i] : T(0);
+        fvy[i] = v;
+        fdy[i] = in ? Y.der[static_cast<long>(ey0) * M * M + i] : T(0);
+        bvy[i] = v * (in ? Y.wj[static_cast<long>(e) * M + q] : T(0));
+    }
+    for (int i = tid; i < NW * WR * BX; i += NT) acc[i] = 0.0;
+    __syncthreads();
+
+    const int lx = tid % TX, ly = tid / TX;
+    const T cx = -prm.dt * prm.gx, cy = -prm.dt * prm.gy;
+    T R[M][M];
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = 0; b < M; ++b) R[a][b] = T(0);
+    if (ex0 + lx < X.elements && ey0 + ly < Y.elements) {
+        T By[M][M], Dy[M][M];
+#pragma unroll
+        for (int q = 0; q < M; ++q)
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                By[q][b] = fvy[(ly * M + q) * M + b];
+                Dy[q][b] = fdy[(ly * M + q) * M + b];
+            }
+#pragma unroll
+        for (int qx = 0; qx < M; ++qx) {
+            T bx[M], dx[M];
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                bx[a] = fvx[(lx * M + qx) * M + a];
+                dx[a] = fdx[(lx * M + qx) * M + a];
+            }
+            T U0[M], U1[M], W0[M], W1[M];  // along x: value and x derivative, per DOF row b
+#pragma unroll
+            for (int b = 0; b < M; ++b) {
+                T u0 = 0, u1 = 0, w0 = 0, w1 = 0;
+#pragma unroll
+                for (int a = 0; a < M; ++a) {
+                    const int li = (lx + a) + BX * (ly + b);
+                    const T cu = ub[li], cw = wb[li];
+                    u0 += bx[a] * cu;
+                    u1 += dx[a] * cu;
+                    w0 += bx[a] * cw;
+                    w1 += dx[a] * cw;
+                }
+                U0[b] = u0;
+                U1[b] = u1;
+                W0[b] = w0;
+                W1[b] = w1;
+            }
+            T K[M];
+#pragma unroll
+            for (int b = 0; b < M; ++b) K[b] = T(0);
+#pragma unroll
+            for (int qy = 0; qy < M; ++qy) {
+                T ux = 0, uy = 0, wx = 0, wy = 0;
+#pragma unroll
+                for (int b = 0; b < M; ++b) {
+                    ux += By[qy][b] * U1[b];
+                    uy += Dy[qy][b] * U0[b];
+                    wx += By[qy][b] * W1[b];
+                    wy += Dy[qy][b] * W0[b];
+                }
+                const T g = cx * (ux * wx) + cy * (uy * wy);
+#pragma unroll
+                for (int b = 0; b < M; ++b) K[b] += bvy[(ly * M + qy) * M + b] * g;
+            }
+#pragma unroll
+            for (int a = 0; a < M; ++a) {
+                const T wa = bvx[(lx * M + qx) * M + a];
+#pragma unroll
+                for (int b = 0; b < M; ++b) R[a][b] += wa * K[b];
+            }
+        }
+    }
+    rhs2_flush<T, P, TX, TY, ONE>(R, acc, out, X, Y, prm, tx, ty);
+}
+
+template <typename T, int P, int TX, int TY, bool ONE>
+__global__ __launch_bounds__(TX* TY) void k_kgrad2(const T* __restrict__ u, const T* __restrict__ w,
+                                                   T* __restrict__ out, dev_tables<T> X, dev_tables<T> Y,
+                                                   rhs_params<T> prm, int colour, int ntx2) {
+    kgrad2_tile<T, P, TX, TY, ONE>(u, w, out, X, Y, prm, colour, xcd_contiguous(blockIdx.x, gridDim.x), ntx2);
 }
 
 // Many 2D problems of degree P in one launch per parity colour: block b belongs to the last
@@ -1126,6 +1457,81 @@ hipError_t rhs2_one(const T* up, T* rhs, const dev_tables<T>* d, const rhs_param
     return hipGetLastError();
 }
 
+// the 3D k-field gradient on the tiles of tile<3, P>: one launch over all tiles and the
+// fix-up when prm has a spill buffer (owned entries added), else one launch per colour
+template <typename T, int P>
+hipError_t kgrad3(const T* u, const T* w, T* out, const dev_tables<T>* d, const rhs_params<T>& prm, hipStream_t s) {
+    using TL = tile<3, P>;
+    constexpr int NT = TL::X * TL::Y * TL::Z;
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X;
+    const int nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    const int ntz = (d[2].elements + TL::Z - 1) / TL::Z;
+    if (ntx <= 0 || nty <= 0 || ntz <= 0) return hipSuccess;
+    const int n[3] = {d[0].dofs, d[1].dofs, d[2].dofs}, nt[3] = {ntx, nty, ntz};
+    for (int i = 0; i < 3; ++i)
+        if (n[i] != d[i].elements + P) return hipErrorInvalidValue;
+    rhs_params<T> q = prm;
+    q.kfield = w;  // staged as the k block
+    q.source = nullptr;
+    q.store = false;
+    if (prm.spill) {
+        spill_layout<P>(n, nt, q.spill_off, q.spill_ext);
+        const long tiles = static_cast<long>(ntx) * nty * ntz;
+        hipLaunchKernelGGL((k_kgrad3<T, P, TL::X, TL::Y, TL::Z, true>), dim3(tiles), dim3(NT), 0, s, u, out, d[0],
+                           d[1], d[2], q, 0, ntx, nty);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const long N = static_cast<long>(n[0]) * n[1] * n[2];
+        hipLaunchKernelGGL((k_rhs3_spill<T, P, TL::X, TL::Y, TL::Z>), dim3(static_cast<unsigned>((N + 255) / 256)),
+                           dim3(256), 0, s, out, q, n[0], n[1], n[2], ntx, nty, ntz);
+        return hipGetLastError();
+    }
+    for (int c = 0; c < 8; ++c) {
+        const int nx2 = (ntx - (c & 1) + 1) / 2, ny2 = (nty - ((c >> 1) & 1) + 1) / 2;
+        const int nz2 = (ntz - ((c >> 2) & 1) + 1) / 2;
+        const long tiles = static_cast<long>(nx2) * ny2 * nz2;
+        if (tiles == 0) continue;
+        hipLaunchKernelGGL((k_kgrad3<T, P, TL::X, TL::Y, TL::Z, false>), dim3(tiles), dim3(NT), 0, s, u, out, d[0],
+                           d[1], d[2], q, c, nx2, ny2);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// the 2D k-field gradient: one launch over all tiles and the fix-up when prm has a spill
+// buffer (prm.store false: the owned entries are added), else one launch per parity colour
+template <typename T, int P>
+hipError_t kgrad2(const T* u, const T* w, T* out, const dev_tables<T>* d, const rhs_params<T>& prm, hipStream_t s) {
+    using TL = tile<2, P>;
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X, nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    if (ntx <= 0 || nty <= 0) return hipSuccess;
+    if (d[0].dofs != d[0].elements + P || d[1].dofs != d[1].elements + P) return hipErrorInvalidValue;
+    if (prm.spill) {
+        rhs_params<T> q = prm;
+        q.store = false;
+        const int n[2] = {d[0].dofs, d[1].dofs}, nt[2] = {ntx, nty};
+        spill_layout2(P, n, nt, q.spill_off, q.spill_ext);
+        hipLaunchKernelGGL((k_kgrad2<T, P, TL::X, TL::Y, true>), dim3(ntx * nty), dim3(TL::X * TL::Y), 0, s, u, w, out,
+                           d[0], d[1], q, 0, ntx);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const long N = static_cast<long>(n[0]) * n[1];
+        hipLaunchKernelGGL((k_rhs2_spill<T, P, TL::X, TL::Y>), dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256),
+                           0, s, out, q, n[0], n[1], ntx, nty);
+        return hipGetLastError();
+    }
+    for (int c = 0; c < 4; ++c) {
+        const int nx2 = (ntx - (c & 1) + 1) / 2, ny2 = (nty - ((c >> 1) & 1) + 1) / 2;
+        if (nx2 * ny2 == 0) continue;
+        hipLaunchKernelGGL((k_kgrad2<T, P, TL::X, TL::Y, false>), dim3(nx2 * ny2), dim3(TL::X * TL::Y), 0, s, u, w,
+                           out, d[0], d[1], prm, c, nx2);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 template <typename T, int P, int FM, int VAR>
 hipError_t rhs3_forms(const T* up, T* rhs, const dev_tables<T>* d, const rhs_params<T>& prm,
                       hipStream_t s) {
@@ -1209,6 +1615,22 @@ hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<
     }
 }
 
+template <typename T>
+hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,
+                        const rhs_params<T>& prm, hipStream_t s) {
+    if (dim != 2 && dim != 3) return hipErrorInvalidValue;
+    switch (p) {
+        case 1:
+            return dim == 2 ? kgrad2<T, 1>(u, w, out, dirs, prm, s) : kgrad3<T, 1>(u, w, out, dirs, prm, s);
+        case 2:
+            return dim == 2 ? kgrad2<T, 2>(u, w, out, dirs, prm, s) : kgrad3<T, 2>(u, w, out, dirs, prm, s);
+        case 3:
+            return dim == 2 ? kgrad2<T, 3>(u, w, out, dirs, prm, s) : kgrad3<T, 3>(u, w, out, dirs, prm, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
 long rhs2_spill_layout(int p, int tx, int ty, const int elements[2], long off[8], int ext[3][2]) {
     const int n[2] = {elements[0] + p, elements[1] + p};
     const int nt[2] = {(elements[0] + tx - 1) / tx, (elements[1] + ty - 1) / ty};
@@ -1263,6 +1685,8 @@ hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int ni
     return hipErrorInvalidValue;
 }
 
+template hipError_t launch_kgrad<double>(int, int, const double*, const double*, double*, const dev_tables<double>*,
+                                         const rhs_params<double>&, hipStream_t);
 template hipError_t launch_rhs2_batch<double>(int, int, const rhs2_item<double>*, int, int, int, hipStream_t);
 template hipError_t launch_rhs2_batch<float>(int, int, const rhs2_item<float>*, int, int, int, hipStream_t);
 template hipError_t launch_rhs<double>(int, int, const double*, double*, const dev_tables<double>*,
