@@ -514,17 +514,17 @@ def main():
     ms_step = 1e3 * t64 / args.steps
 
     # dominant kernel: the element kernel (k_rhs3; k_rhs3_pipe, k_rhs3_zpair for variants 3, 4),
-    # launched once over all tiles followed by the spill fix-up k_rhs3_spill (tiling 1), or once
-    # per parity colour (8 launches per step, each over E/8 elements); HIP events on the
-    # handle's stream bracket the launches of the phase
+    # launched once over all tiles (tiling 1; its spill fix-up k_rhs_fixup runs at the start
+    # of the sweep phase), or once per parity colour (8 launches per step, each over E/8
+    # elements); HIP events on the handle's stream bracket the launches of the phase
     rhs_kernel = {3: "k_rhs3_pipe", 4: "k_rhs3_zpair"}.get(var64, "k_rhs3")
     one = til64 == 1 and var64 != 3
     launches = 1 if one else 8
     launch_flops = f_elem(3, 2) * E / launches
     launch_ms = ph64["rhs"] / launches
-    traffic_key = rhs_kernel + ("+k_rhs3_spill" if one else "")
-    launch_desc = ("one launch over all tiles + the k_rhs3_spill fix-up; achieved over both "
-                   "kernels' time" if one else "one launch per parity colour")
+    traffic_key = rhs_kernel + (" (one launch)" if one else "")
+    launch_desc = ("one launch over all tiles; its spill fix-up k_rhs_fixup (memory-bound, "
+                   "timed in the x-sweep phase) not included" if one else "one launch per parity colour")
     rhs_tflops = launch_flops / (launch_ms * 1e-3) / 1e12
     sweep_bytes = 2 * N * 8  # one directional sweep: read and write every coefficient once
     sweep_ms = [ph64["sweep_x"], ph64["sweep_y"], ph64["sweep_z"]]
@@ -561,9 +561,9 @@ def main():
             "note": ("FP64 vector-ALU bound (MI355X FP64 vector peak = FP64 matrix peak = "
                      "78.6 TFLOP/s); achieved = 3051 flop/element x the elements of one launch "
                      "(E, or E/8 per colour launch) / average launch time (HIP events around the "
-                     "phase's launches on the handle's stream, the fix-up kernel included); "
-                     "traffic = HBM bytes per launch (both kernels) from rocprofv3 FETCH_SIZE "
-                     "and WRITE_SIZE scaled by a copy-kernel calibration"),
+                     "phase's launches on the handle's stream); traffic = HBM bytes per launch "
+                     "from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a copy-kernel "
+                     "calibration"),
         },
         "phases_ms": ph64,
         "rhs_variant": var64,

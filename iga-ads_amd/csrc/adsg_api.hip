@@ -160,10 +160,12 @@ void dir_tables(const adsg_sim* h, adsg::dev_tables<TT>* t) {
 
 // the element loop of the handle: the general kernel for configurations outside the
 // sum-factorised kernels', else the tuned variant
-// (overwrite: out := the element sums, its start value unused; else out += them)
+// (overwrite: out := the element sums, its start value unused; else out += them; defer: a
+// one-launch tiling describes its fix-up there instead of running it, see sweep_all)
 template <typename T>
 hipError_t launch_elements(const adsg_sim* h, const T* up, T* out, const adsg::dev_tables<T>* t,
-                           const adsg::rhs_params<T>& prm, bool overwrite = false) {
+                           const adsg::rhs_params<T>& prm, bool overwrite = false,
+                           adsg::rhs_fixup<T>* defer = nullptr) {
     if (h->general) {
         if (overwrite) {
             const hipError_t e = adsg::launch_fill<T>(out, h->N, T(0), h->stream);
@@ -173,6 +175,7 @@ hipError_t launch_elements(const adsg_sim* h, const T* up, T* out, const adsg::d
     }
     adsg::rhs_params<T> q = prm;
     q.spill = h->rhs_tiling ? static_cast<T*>(h->spill) : nullptr;
+    q.defer = defer;  // the one-launch tiling leaves its fix-up to the caller
     return adsg::launch_rhs<T>(h->dim, h->dirs[0].tab.p, up, out, t, q,
                                rhs_code(h) | (overwrite ? adsg::kRhsOverwrite : 0), h->stream);
 }
@@ -227,7 +230,7 @@ adsg::line_geometry geometry(const adsg_sim* h, int dir) {
 
 // mode: 0 = A^-1, 1 = A^-T, 2 = A^-1 with one FP64-residual refinement (FP32 data)
 template <typename T>
-int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0) {
+int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fixup<T>* fix = nullptr) {
     if (sl < 0 || sl >= static_cast<int>(h->dirs[d].slots.size()) || !h->dirs[d].slots[sl].factored)
         return fail(h, ADSG_ESTATE, "ads_solve: matrix slot not set or not factored");
     const dev_slot& s = h->dirs[d].slots[sl];
@@ -271,9 +274,14 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0) {
         sf.mseg = s.seg.mseg;
         sf.transposed = mode == 1;
         sf.refine = mode == 2;
-        e = adsg::launch_line_sweep_seg<T, T>(u, geometry(h, d), sf, h->stream);
+        e = adsg::launch_line_sweep_seg<T, T>(u, geometry(h, d), sf, h->stream, fix);
         if (e == hipErrorInvalidValue && mode != 0)
             return fail(h, ADSG_EINVAL, "ads_solve: line staging does not fit the LDS for this solve");
+    }
+    if (e == hipErrorInvalidValue && mode == 0 && fix && fix->pending()) {
+        e = adsg::launch_rhs_fixup<T>(u, *fix, h->stream);  // the sweep below cannot take it
+        if (e != hipSuccess) return hip_fail(h, e, "rhs fix-up");
+        e = hipErrorInvalidValue;
     }
     if (e == hipErrorInvalidValue && mode == 0) e = adsg::launch_line_sweep<T, T>(u, geometry(h, d), f, h->stream);
     if (e == hipErrorInvalidValue) return fail(h, ADSG_EINVAL, "ads_solve: unsupported band widths (kl, ku)");
@@ -284,8 +292,11 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0) {
 // ads_solve: one sweep per direction in the order x, y, z (solver.hpp:35-41)
 // The generalized ADS (solver.hpp:43-195) solves the direction with per-line matrices first
 // and the others after it in cyclic order; the standard directions commute.
+// fix: the element loop's deferred fix-up and boundary zeroing of u, in one kernel before
+// the sweeps; a zeroing alone is done by the x sweep as it loads its lines when that sweep
+// runs first as the segmented plain solve
 template <typename T>
-int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0) {
+int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0, const adsg::rhs_fixup<T>* fix = nullptr) {
     int special = -1;
     for (int d = 0; d < h->dim; ++d) {
         const int sl = slots ? slots[d] : 0;
@@ -295,10 +306,22 @@ int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0) {
         }
     }
     const int start = special < 0 ? 0 : special;
+    bool fused = false;
+    if (fix && fix->pending()) {
+        const int sl = slots ? slots[0] : 0;
+        const bool seg_x = sl >= 0 && sl < static_cast<int>(h->dirs[0].slots.size()) &&
+                           h->dirs[0].slots[sl].factored && h->dirs[0].slots[sl].lines == 0 &&
+                           h->dirs[0].slots[sl].seg.nseg > 0 && h->sweep_variant == 0;
+        fused = !fix->spill && start == 0 && mode == 0 && seg_x;  // zeroing only: in the x sweep
+        if (!fused) {
+            const hipError_t e = adsg::launch_rhs_fixup<T>(u, *fix, h->stream);
+            if (e != hipSuccess) return hip_fail(h, e, "rhs fix-up");
+        }
+    }
     for (int i = 0; i < 3; ++i) {
         if (i < h->dim) {
             const int d = (start + i) % h->dim;
-            int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode);
+            int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode, i == 0 && fused ? fix : nullptr);
             if (rc) return rc;
         }
         mark(h, 4 + i);
@@ -338,7 +361,7 @@ int ensure_source_load(adsg_sim* h, const adsg_rhs_form* form) {
 }
 
 template <typename T>
-int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
+int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false, adsg::rhs_fixup<T>* defer = nullptr) {
     adsg::dev_tables<T> t[3];
     dir_tables(h, t);
     adsg::rhs_params<T> prm;
@@ -389,7 +412,7 @@ int rhs_impl(adsg_sim* h, const adsg_rhs_form* form, bool delta = false) {
         e = adsg::launch_rhs_kron<T>(h->dim, h->dirs[0].tab.p, up, rhs, kd, mass, form->dt, form->grad,
                                      form->use_source, h->stream);
     } else {
-        e = launch_elements<T>(h, up, rhs, t, prm, !form->use_source);
+        e = launch_elements<T>(h, up, rhs, t, prm, !form->use_source, defer);
     }
     if (e != hipSuccess) return hip_fail(h, e, "rhs assembly");
     mark(h, 2);
@@ -431,8 +454,10 @@ int zero_boundary_impl(adsg_sim* h) {
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "zero boundary");
 }
 
-int solve_impl(adsg_sim* h, const int* slots) {
-    if (h->precision == ADSG_FP64) return sweep_all<double>(h, static_cast<double*>(h->vec[ADSG_U]), slots);
+int solve_impl(adsg_sim* h, const int* slots, const adsg::rhs_fixup<double>* fix64 = nullptr,
+               const adsg::rhs_fixup<float>* fix32 = nullptr) {
+    if (h->precision == ADSG_FP64)
+        return sweep_all<double>(h, static_cast<double*>(h->vec[ADSG_U]), slots, 0, fix64);
     // FP32: u = u_prev + A^-1 (rhs - M u_prev); valid when A is the Gram matrix of every
     // direction (slot 0), which is what makes buffer = rhs - A u_prev
     if (!h->pending) return fail(h, ADSG_ESTATE, "ads_solve (FP32): call compute_rhs first");
@@ -442,7 +467,7 @@ int solve_impl(adsg_sim* h, const int* slots) {
                 return fail(h, ADSG_EINVAL,
                             "ads_solve (FP32): increments are defined against the Gram matrices (slot 0); "
                             "use adsg_step_delta for other matrices");
-    int rc = sweep_all<float>(h, static_cast<float*>(h->vec[ADSG_BUFFER]), nullptr);
+    int rc = sweep_all<float>(h, static_cast<float*>(h->vec[ADSG_BUFFER]), nullptr, 0, fix32);
     if (rc) return rc;
     int nx, ny, nz;
     extents(h, nx, ny, nz);
@@ -466,16 +491,32 @@ int rhs_dispatch(adsg_sim* h, const adsg_rhs_form* form) {
 }
 
 // one explicit step as validation.hpp:73-81 (before_step swap, compute_rhs, solve)
+// (the element loop's one-launch fix-up and, in FP64, the boundary zeroing are left to
+// sweep_all: rhs_fixup)
+template <typename T>
+void fixup_extents(const adsg_sim* h, adsg::rhs_fixup<T>& fx) {
+    extents(h, fx.n[0], fx.n[1], fx.n[2]);
+}
+
 int one_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
     swap_state(h);
-    int rc = rhs_dispatch(h, form);
-    if (rc) return rc;
-    if (flags & ADSG_STEP_DIRICHLET) {
-        rc = zero_boundary_impl(h);
-        if (rc) return rc;
+    const bool dir = flags & ADSG_STEP_DIRICHLET;
+    int rc;
+    if (h->precision == ADSG_FP64) {
+        adsg::rhs_fixup<double> fx;
+        if ((rc = rhs_impl<double>(h, form, false, &fx))) return rc;
+        fixup_extents(h, fx);
+        fx.zero_boundary = dir;
+        mark(h, 3);
+        return solve_impl(h, nullptr, &fx, nullptr);
     }
+    // FP32: the boundary residual reads the complete increment, so with Dirichlet rows the
+    // fix-up runs in the element phase
+    adsg::rhs_fixup<float> fx;
+    if ((rc = rhs_impl<float>(h, form, false, dir ? nullptr : &fx))) return rc;
+    if (dir && (rc = zero_boundary_impl(h))) return rc;
     mark(h, 3);
-    return solve_impl(h, nullptr);
+    return solve_impl(h, nullptr, nullptr, &fx);
 }
 
 // one delta step: u = u_prev + A^-1 r(u_prev)
@@ -487,22 +528,24 @@ int one_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int
     int rc;
     hipError_t e;
     if (h->precision == ADSG_FP32) {
-        if ((rc = rhs_impl<float>(h, form, true))) return rc;
+        adsg::rhs_fixup<float> fx;
+        if ((rc = rhs_impl<float>(h, form, true, &fx))) return rc;
         float* r = static_cast<float*>(h->vec[ADSG_BUFFER]);
-        if (dir && (e = adsg::launch_zero_boundary<float>(h->dim, r, nx, ny, nz, h->stream)) != hipSuccess)
-            return hip_fail(h, e, "zero boundary");
+        fixup_extents(h, fx);
+        fx.zero_boundary = dir;
         mark(h, 3);
-        if ((rc = sweep_all<float>(h, r, slots, (flags & ADSG_STEP_REFINE) ? 2 : 0))) return rc;
+        if ((rc = sweep_all<float>(h, r, slots, (flags & ADSG_STEP_REFINE) ? 2 : 0, &fx))) return rc;
         e = adsg::launch_apply_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
                                          static_cast<double*>(h->vec[ADSG_U]),
                                          static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz, dir, h->stream);
     } else {
-        if ((rc = rhs_impl<double>(h, form, true))) return rc;
+        adsg::rhs_fixup<double> fx;
+        if ((rc = rhs_impl<double>(h, form, true, &fx))) return rc;
         double* r = static_cast<double*>(h->vec[ADSG_BUFFER]);
-        if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, r, nx, ny, nz, h->stream)) != hipSuccess)
-            return hip_fail(h, e, "zero boundary");
+        fixup_extents(h, fx);
+        fx.zero_boundary = dir;
         mark(h, 3);
-        if ((rc = sweep_all<double>(h, r, slots))) return rc;
+        if ((rc = sweep_all<double>(h, r, slots, 0, &fx))) return rc;
         e = adsg::launch_add_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
                                        static_cast<double*>(h->vec[ADSG_U]), h->dim, nx, ny, nz, dir,
                                        h->stream);

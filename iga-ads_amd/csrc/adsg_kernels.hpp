@@ -45,9 +45,31 @@ struct seg_factors {
     bool transposed = false, refine = false;
 };
 
+// The one-launch element loop's fix-up (adding the spill entries of the earlier tiles that
+// share a DOF) and the Dirichlet zeroing of every boundary DOF, left to the caller of the
+// element loop: both in one pass over the DOFs (k_rhs_fixup: fix-up sum added, then the
+// boundary set to 0, the order of the separate passes, so the same bits), or, without spill
+// entries, the zeroing alone in the x-line sweep as it loads its lines. (The spill sums
+// inside the sweep measured slower: each lane's gathers for its ~15 rows run one row after
+// another.)
+template <typename T>
+struct rhs_fixup {
+    const T* spill = nullptr;  // null: no spill entries to add
+    long off[8] = {};          // the spill layout of rhs_params (spill_off, spill_ext)
+    int ext[3][2] = {};
+    int p = 0;
+    int tile[3] = {1, 1, 1}, nt[3] = {1, 1, 1};  // tile size (elements) and tile count per direction
+    int n[3] = {1, 1, 1};                        // DOFs per direction (1 beyond the dimension)
+    bool zero_boundary = false;
+    bool pending() const { return spill != nullptr || zero_boundary; }
+};
+
+// fix: a deferred boundary zeroing (rhs_fixup without spill entries) to apply while the x
+// lines (geometry with S == 1, the plain solve) are loaded, or null; the caller applies
+// other fix-ups first (launch_rhs_fixup)
 template <typename T, typename F>
 hipError_t launch_line_sweep_seg(T* u, const line_geometry& g, const seg_factors<F>& f,
-                                 hipStream_t s);
+                                 hipStream_t s, const rhs_fixup<T>* fix = nullptr);
 
 // Per-line factors (one band matrix per line, lines of geometry g in its order): entry k
 // of line l at k * n_lines + l, widths kl and ku (ku = kl + ku of the matrix when pivoted).
@@ -99,6 +121,9 @@ struct rhs_params {
     T* spill = nullptr;
     long spill_off[8] = {};
     int spill_ext[3][2] = {};
+    // host side only: when set, a one-launch element loop leaves its fix-up to the caller and
+    // describes it here (defer->spill null when there is none to do)
+    rhs_fixup<T>* defer = nullptr;
     // set by launch_rhs for the one-launch tiling when the rhs start value is not used: the
     // owners store their entries instead of adding them
     bool store = false;
@@ -119,6 +144,10 @@ constexpr int kRhsOverwrite = 1 << 9;
 // values the spill array of the one-launch tiling needs for degree p and these element
 // counts (the largest over the variants that use it)
 long rhs_spill_slots(int dim, int p, const int elements[3]);
+// a deferred fix-up (rhs_params::defer, launch_line_sweep_seg) as a kernel of its own over
+// the DOFs of rhs
+template <typename T>
+hipError_t launch_rhs_fixup(T* rhs, const rhs_fixup<T>& fx, hipStream_t s);
 
 // The element loop for any quadrature order, repeated interior knots and a degree per
 // direction (1..3), one lane per element (adsg_rhs_gen.hip); dirs[d].first, nq and p set.

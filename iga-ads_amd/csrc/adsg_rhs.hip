@@ -87,7 +87,7 @@ __device__ __forceinline__ tile3 rhs3_tile_of(int t, int colour, int ntx, int nt
 // direction when t T <= g < (t + 1) T, or t T <= g in the direction's last tile, so every
 // DOF has one owner; the owner adds its entry to the rhs (a plain read-modify-write: no
 // other workgroup writes it; a store when the rhs start value is unused) and the other tiles
-// store theirs in the spill arrays, which k_rhs3_spill adds to the rhs afterwards in a fixed
+// store theirs in the spill arrays, which k_rhs_fixup adds to the rhs afterwards in a fixed
 // order. An entry whose owner lies at offset delta from the tile (delta_d = 1 where the
 // entry is in the tile's top P layers of direction d and the tile is not the last there)
 // goes to array delta at the DOF's own index in the directions with delta_d = 0 and at
@@ -849,37 +849,6 @@ __global__ __launch_bounds__(TX* TY*(TZ / 2), 2) void k_rhs3_zpair(
     }
 }
 
-// The one-launch tiling's second kernel (tiles of TX x TY x TZ elements, ntx x nty x ntz of
-// them): DOF g lies in tile t_d = min(g_d / T_d, nt_d - 1) at local index l_d = g_d - T_d t_d
-// of each direction; the tile t_d - 1 holds an entry for it (in its top P layers) when
-// l_d < P and t_d > 0. Those entries of the (up to 7) earlier tiles are added in a fixed
-// order (delta = 1, ..., 7: the x neighbour first).
-template <typename T, int P, int TX, int TY, int TZ>
-__global__ __launch_bounds__(256) void k_rhs3_spill(T* __restrict__ rhs, rhs_params<T> prm, int nx, int ny,
-                                                    int nz, int ntx, int nty, int ntz) {
-    const long g = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
-    if (g >= static_cast<long>(nx) * ny * nz) return;
-    const int gx = static_cast<int>(g % nx);
-    const long r = g / nx;
-    const int gy = static_cast<int>(r % ny), gz = static_cast<int>(r / ny);
-    const int tx = min(gx / TX, ntx - 1), ty = min(gy / TY, nty - 1), tz = min(gz / TZ, ntz - 1);
-    const int lx = gx - TX * tx, ly = gy - TY * ty, lz = gz - TZ * tz;
-    const bool sx = lx < P && tx > 0, sy = ly < P && ty > 0, sz = lz < P && tz > 0;
-    if (!(sx || sy || sz)) return;
-    const T* __restrict__ spill = prm.spill;
-    double sum = 0;
-#pragma unroll
-    for (int d = 1; d < 8; ++d) {
-        const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
-        if ((dx && !sx) || (dy && !sy) || (dz && !sz)) continue;
-        const long i = prm.spill_off[d] + (dx ? tx * P + lx : gx) +
-                       static_cast<long>(prm.spill_ext[0][dx]) *
-                           ((dy ? ty * P + ly : gy) + static_cast<long>(prm.spill_ext[1][dy]) * (dz ? tz * P + lz : gz));
-        sum += static_cast<double>(spill[i]);
-    }
-    rhs[g] += static_cast<T>(sum);
-}
-
 // ------------------------------------------------------------------------------------
 // element-loop RHS, 2D
 // ------------------------------------------------------------------------------------
@@ -1085,37 +1054,6 @@ __device__ __forceinline__ void rhs2_tile(const T* __restrict__ up, T* __restric
         }
     }
     rhs2_flush<T, P, TX, TY, ONE>(R, acc, rhs, X, Y, prm, tx, ty);
-}
-
-// The 2D one-launch tiling's fix-up (rhs, DOF counts nx x ny, ntx x nty tiles of TX x TY
-// elements): the entries of the (up to 3) earlier tiles that share a DOF, added in the order
-// delta = 1, 2, 3 (k_rhs3_spill in two directions).
-template <typename T>
-__device__ __forceinline__ void rhs2_fixup(T* __restrict__ rhs, const rhs_params<T>& prm, int p, int TX, int TY,
-                                           int nx, int ny, int ntx, int nty, long g) {
-    const int gx = static_cast<int>(g % nx), gy = static_cast<int>(g / nx);
-    const int tx = min(gx / TX, ntx - 1), ty = min(gy / TY, nty - 1);
-    const int lx = gx - TX * tx, ly = gy - TY * ty;
-    const bool sx = lx < p && tx > 0, sy = ly < p && ty > 0;
-    if (!(sx || sy)) return;
-    double sum = 0;
-#pragma unroll
-    for (int d = 1; d < 4; ++d) {
-        const int dx = d & 1, dy = d >> 1;
-        if ((dx && !sx) || (dy && !sy)) continue;
-        const long i = prm.spill_off[d] + (dx ? tx * p + lx : gx) +
-                       static_cast<long>(prm.spill_ext[0][dx]) * (dy ? ty * p + ly : gy);
-        sum += static_cast<double>(prm.spill[i]);
-    }
-    rhs[g] += static_cast<T>(sum);
-}
-
-template <typename T, int P, int TX, int TY>
-__global__ __launch_bounds__(256) void k_rhs2_spill(T* __restrict__ rhs, rhs_params<T> prm, int nx, int ny, int ntx,
-                                                    int nty) {
-    const long g = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
-    if (g >= static_cast<long>(nx) * ny) return;
-    rhs2_fixup(rhs, prm, P, TX, TY, nx, ny, ntx, nty, g);
 }
 
 template <typename T, int P, int TX, int TY, bool KF, bool SRC, bool ONE>
@@ -1337,6 +1275,85 @@ hipError_t rhs3_colours(const T* up, T* rhs, const dev_tables<T>* d, const rhs_p
     return hipSuccess;
 }
 
+// describes the fix-up of a one-launch element loop (layout of q) for a later kernel
+template <typename T>
+void defer_fixup(rhs_fixup<T>& fx, const rhs_params<T>& q, int p, const int tile[3], const int nt[3],
+                 const int n[3]) {
+    fx.spill = q.spill;
+    for (int k = 0; k < 8; ++k) fx.off[k] = q.spill_off[k];
+    for (int d = 0; d < 3; ++d) {
+        fx.ext[d][0] = q.spill_ext[d][0];
+        fx.ext[d][1] = q.spill_ext[d][1];
+        fx.tile[d] = tile[d];
+        fx.nt[d] = nt[d];
+        fx.n[d] = n[d];
+    }
+    fx.p = p;
+}
+
+// The fix-up sum of DOF (gx, gy, gz), given its tile t_d = min(g_d / tile_d, nt_d - 1) and
+// local index l_d = g_d - tile_d t_d per direction: the spill entries of the earlier tiles
+// that hold it, added in the order delta = 1..7 as double; `any` tells whether there were some.
+template <typename T>
+__device__ __forceinline__ double rhs_fixup_sum(const rhs_fixup<T>& fx, int gx, int tx, int lx, int gy, int ty,
+                                                int ly, int gz, int tz, int lz, bool& any) {
+    const int P = fx.p;
+    const bool sx = lx < P && tx > 0, sy = ly < P && ty > 0, sz = lz < P && tz > 0;
+    any = sx || sy || sz;
+    double sum = 0;
+    if (!any) return sum;
+#pragma unroll
+    for (int d = 1; d < 8; ++d) {
+        const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+        if ((dx && !sx) || (dy && !sy) || (dz && !sz)) continue;
+        const long i = fx.off[d] + (dx ? tx * P + lx : gx) +
+                       static_cast<long>(fx.ext[0][dx]) *
+                           ((dy ? ty * P + ly : gy) + static_cast<long>(fx.ext[1][dy]) * (dz ? tz * P + lz : gz));
+        sum += static_cast<double>(fx.spill[i]);
+    }
+    return sum;
+}
+
+// A deferred fix-up over the DOFs (nx x ny x nz; nz = 1 in 2D): the spill sum of every DOF
+// that has one (the entries of the up to 7 earlier tiles that hold it: the tile t_d - 1 of a
+// direction holds an entry for DOF g in its top P layers when l_d < P and t_d > 0), then the
+// boundary zeroing (k_zero_boundary's, when asked for), in one pass.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_fixup(T* __restrict__ rhs, rhs_fixup<T> fx) {
+    const int nx = fx.n[0], ny = fx.n[1], nz = fx.n[2];
+    const long g = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (g >= static_cast<long>(nx) * ny * nz) return;
+    const int gx = static_cast<int>(g % nx);
+    const long r = g / nx;
+    const int gy = static_cast<int>(r % ny), gz = static_cast<int>(r / ny);
+    if (fx.zero_boundary &&
+        (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1 || (nz > 1 && (gz == 0 || gz == nz - 1)))) {
+        rhs[g] = T(0);
+        return;
+    }
+    if (!fx.spill) return;
+    const int tx = min(gx / fx.tile[0], fx.nt[0] - 1), ty = min(gy / fx.tile[1], fx.nt[1] - 1);
+    const int tz = min(gz / fx.tile[2], fx.nt[2] - 1);
+    bool any;
+    const double sum = rhs_fixup_sum(fx, gx, tx, gx - fx.tile[0] * tx, gy, ty, gy - fx.tile[1] * ty, gz, tz,
+                                     gz - fx.tile[2] * tz, any);
+    if (any) rhs[g] += static_cast<T>(sum);
+}
+
+// the end of a one-launch element loop (spill layout in q): its fix-up described for the
+// caller (prm.defer), or launched now
+template <typename T>
+hipError_t finish_one(const rhs_params<T>& prm, const rhs_params<T>& q, T* rhs, int p, const int tile[3],
+                      const int nt[3], const int n[3], hipStream_t s) {
+    rhs_fixup<T> fx;
+    defer_fixup(fx, q, p, tile, nt, n);
+    if (prm.defer) {
+        *prm.defer = fx;
+        return hipSuccess;
+    }
+    return launch_rhs_fixup<T>(rhs, fx, s);
+}
+
 // The spill arrays of n[d] DOFs in nt[d] tiles per direction: array delta (1..7) has
 // extent n[d] in the directions with delta_d = 0 and nt[d] P in the others, at off[delta];
 // returns the total.
@@ -1382,10 +1399,8 @@ hipError_t rhs3_one(const T* up, T* rhs, const dev_tables<T>* d, const rhs_param
                            dim3(TL::X * TL::Y * TZ), 0, s, up, rhs, d[0], d[1], d[2], q, 0, ntx, nty);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const long N = static_cast<long>(d[0].dofs) * d[1].dofs * d[2].dofs;
-    hipLaunchKernelGGL((k_rhs3_spill<T, P, TL::X, TL::Y, TZ>), dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256),
-                       0, s, rhs, q, d[0].dofs, d[1].dofs, d[2].dofs, ntx, nty, ntz);
-    return hipGetLastError();
+    const int tl[3] = {TL::X, TL::Y, TZ};
+    return finish_one(prm, q, rhs, P, tl, nt, n, s);
 }
 
 // spill values of one launch of a variant (dofs = elements + P)
@@ -1451,10 +1466,8 @@ hipError_t rhs2_one(const T* up, T* rhs, const dev_tables<T>* d, const rhs_param
                        d[0], d[1], q, 0, ntx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const long N = static_cast<long>(n[0]) * n[1];
-    hipLaunchKernelGGL((k_rhs2_spill<T, P, TL::X, TL::Y>), dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0,
-                       s, rhs, q, n[0], n[1], ntx, nty);
-    return hipGetLastError();
+    const int tl[3] = {TL::X, TL::Y, 1}, nt3[3] = {ntx, nty, 1}, n3[3] = {n[0], n[1], 1};
+    return finish_one(prm, q, rhs, P, tl, nt3, n3, s);
 }
 
 // the 3D k-field gradient on the tiles of tile<3, P>: one launch over all tiles and the
@@ -1474,6 +1487,7 @@ hipError_t kgrad3(const T* u, const T* w, T* out, const dev_tables<T>* d, const 
     q.kfield = w;  // staged as the k block
     q.source = nullptr;
     q.store = false;
+    q.defer = nullptr;
     if (prm.spill) {
         spill_layout<P>(n, nt, q.spill_off, q.spill_ext);
         const long tiles = static_cast<long>(ntx) * nty * ntz;
@@ -1481,10 +1495,8 @@ hipError_t kgrad3(const T* u, const T* w, T* out, const dev_tables<T>* d, const 
                            d[1], d[2], q, 0, ntx, nty);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        const long N = static_cast<long>(n[0]) * n[1] * n[2];
-        hipLaunchKernelGGL((k_rhs3_spill<T, P, TL::X, TL::Y, TL::Z>), dim3(static_cast<unsigned>((N + 255) / 256)),
-                           dim3(256), 0, s, out, q, n[0], n[1], n[2], ntx, nty, ntz);
-        return hipGetLastError();
+        const int tl[3] = {TL::X, TL::Y, TL::Z};
+        return finish_one(q, q, out, P, tl, nt, n, s);
     }
     for (int c = 0; c < 8; ++c) {
         const int nx2 = (ntx - (c & 1) + 1) / 2, ny2 = (nty - ((c >> 1) & 1) + 1) / 2;
@@ -1510,16 +1522,15 @@ hipError_t kgrad2(const T* u, const T* w, T* out, const dev_tables<T>* d, const 
     if (prm.spill) {
         rhs_params<T> q = prm;
         q.store = false;
+        q.defer = nullptr;
         const int n[2] = {d[0].dofs, d[1].dofs}, nt[2] = {ntx, nty};
         spill_layout2(P, n, nt, q.spill_off, q.spill_ext);
         hipLaunchKernelGGL((k_kgrad2<T, P, TL::X, TL::Y, true>), dim3(ntx * nty), dim3(TL::X * TL::Y), 0, s, u, w, out,
                            d[0], d[1], q, 0, ntx);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        const long N = static_cast<long>(n[0]) * n[1];
-        hipLaunchKernelGGL((k_rhs2_spill<T, P, TL::X, TL::Y>), dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256),
-                           0, s, out, q, n[0], n[1], ntx, nty);
-        return hipGetLastError();
+        const int tl[3] = {TL::X, TL::Y, 1}, nt3[3] = {ntx, nty, 1}, n3[3] = {n[0], n[1], 1};
+        return finish_one(q, q, out, P, tl, nt3, n3, s);
     }
     for (int c = 0; c < 4; ++c) {
         const int nx2 = (ntx - (c & 1) + 1) / 2, ny2 = (nty - ((c >> 1) & 1) + 1) / 2;
@@ -1637,6 +1648,15 @@ long rhs2_spill_layout(int p, int tx, int ty, const int elements[2], long off[8]
     return spill_layout2(p, n, nt, off, ext);
 }
 
+template <typename T>
+hipError_t launch_rhs_fixup(T* rhs, const rhs_fixup<T>& fx, hipStream_t s) {
+    if (!fx.pending()) return hipSuccess;
+    const long N = static_cast<long>(fx.n[0]) * fx.n[1] * fx.n[2];
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_rhs_fixup<T>), dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0, s, rhs, fx);
+    return hipGetLastError();
+}
+
 long rhs_spill_slots(int dim, int p, const int elements[3]) {
     if (dim == 2) {
         long off[8];
@@ -1687,6 +1707,8 @@ hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int ni
 
 template hipError_t launch_kgrad<double>(int, int, const double*, const double*, double*, const dev_tables<double>*,
                                          const rhs_params<double>&, hipStream_t);
+template hipError_t launch_rhs_fixup<double>(double*, const rhs_fixup<double>&, hipStream_t);
+template hipError_t launch_rhs_fixup<float>(float*, const rhs_fixup<float>&, hipStream_t);
 template hipError_t launch_rhs2_batch<double>(int, int, const rhs2_item<double>*, int, int, int, hipStream_t);
 template hipError_t launch_rhs2_batch<float>(int, int, const rhs2_item<float>*, int, int, int, hipStream_t);
 template hipError_t launch_rhs<double>(int, int, const double*, double*, const dev_tables<double>*,

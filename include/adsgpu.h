@@ -293,9 +293,12 @@ int adsg_check_finite(adsg_sim* h, int which);
  * slots is NULL, adsg_step_delta's with these slots otherwise -- measured with HIP events on
  * the handle's stream: the counterpart of the reference's StatTimer around compute_rhs
  * (examples/scalability/test3d.hpp:67-101). Averages per step in milliseconds: ms[0] the
- * element-loop kernels (2^dim colour launches), ms[1] boundary handling, ms[2], ms[3],
- * ms[4] line sweeps in x, y, z, ms[5] the whole step, ms[6] the start of the rhs (a
- * source's load vector scaled in; the zero fill, where one is needed, is part of ms[0]). */
+ * element-loop kernels (the 2^dim colour launches, or the one launch over all tiles),
+ * ms[1] boundary handling, ms[2], ms[3], ms[4] line sweeps in x, y, z, ms[5] the whole step,
+ * ms[6] the start of the rhs (a source's load vector scaled in; the zero fill, where one is
+ * needed, is part of ms[0]). With the one-launch tiling the spill fix-up, and the FP64
+ * boundary zeroing with it, run at the start of the x sweep and count in ms[2]; without
+ * the fix-up the FP64 zeroing is done by the x sweep as it loads its lines. */
 int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const int* slots, int flags,
                      double* ms);
 
