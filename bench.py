@@ -450,6 +450,8 @@ def main():
     ap.add_argument("--no-kron", action="store_true",
                     help="skip the separable-integrand (1D-matrix) rhs block")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D k-field adjoint block")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the 2D k-field adjoint block (for profiling) and print its JSON")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -482,6 +484,10 @@ def main():
                       centre=(0.2, 0.8), noise=1e-3)
     N = u0.size
     E = ne ** 3
+
+    if args.adjoint_only:
+        print(json.dumps({"adjoint_k_2d": adjoint_k_block(ag, np, 128, 200, 20, 1e-6)}), flush=True)
+        return
 
     if args.eager_only:
         prec = ag.FP64 if args.precision == "fp64" else ag.FP32

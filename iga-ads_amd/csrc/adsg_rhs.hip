@@ -1318,22 +1318,25 @@ __device__ __forceinline__ double rhs_fixup_sum(const rhs_fixup<T>& fx, int gx, 
 // that has one (the entries of the up to 7 earlier tiles that hold it: the tile t_d - 1 of a
 // direction holds an entry for DOF g in its top P layers when l_d < P and t_d > 0), then the
 // boundary zeroing (k_zero_boundary's, when asked for), in one pass.
-template <typename T>
+// (I: the index type, 32-bit unsigned below 2^32 DOFs: its divisions are the cheap ones)
+template <typename T, typename I>
 __global__ __launch_bounds__(256) void k_rhs_fixup(T* __restrict__ rhs, rhs_fixup<T> fx) {
-    const int nx = fx.n[0], ny = fx.n[1], nz = fx.n[2];
-    const long g = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
-    if (g >= static_cast<long>(nx) * ny * nz) return;
-    const int gx = static_cast<int>(g % nx);
-    const long r = g / nx;
-    const int gy = static_cast<int>(r % ny), gz = static_cast<int>(r / ny);
-    if (fx.zero_boundary &&
-        (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1 || (nz > 1 && (gz == 0 || gz == nz - 1)))) {
+    const I nx = fx.n[0], ny = fx.n[1], nz = fx.n[2];
+    const I g = static_cast<I>(blockIdx.x) * 256 + threadIdx.x;
+    if (g >= nx * ny * nz) return;
+    const I r = g / nx;
+    const int gx = static_cast<int>(g - r * nx);
+    const I rz = r / ny;
+    const int gy = static_cast<int>(r - rz * ny), gz = static_cast<int>(rz);
+    if (fx.zero_boundary && (gx == 0 || gx == fx.n[0] - 1 || gy == 0 || gy == fx.n[1] - 1 ||
+                             (fx.n[2] > 1 && (gz == 0 || gz == fx.n[2] - 1)))) {
         rhs[g] = T(0);
         return;
     }
     if (!fx.spill) return;
-    const int tx = min(gx / fx.tile[0], fx.nt[0] - 1), ty = min(gy / fx.tile[1], fx.nt[1] - 1);
-    const int tz = min(gz / fx.tile[2], fx.nt[2] - 1);
+    const int tx = min(static_cast<int>(static_cast<unsigned>(gx) / static_cast<unsigned>(fx.tile[0])), fx.nt[0] - 1);
+    const int ty = min(static_cast<int>(static_cast<unsigned>(gy) / static_cast<unsigned>(fx.tile[1])), fx.nt[1] - 1);
+    const int tz = min(static_cast<int>(static_cast<unsigned>(gz) / static_cast<unsigned>(fx.tile[2])), fx.nt[2] - 1);
     bool any;
     const double sum = rhs_fixup_sum(fx, gx, tx, gx - fx.tile[0] * tx, gy, ty, gy - fx.tile[1] * ty, gz, tz,
                                      gz - fx.tile[2] * tz, any);
@@ -1653,7 +1656,11 @@ hipError_t launch_rhs_fixup(T* rhs, const rhs_fixup<T>& fx, hipStream_t s) {
     if (!fx.pending()) return hipSuccess;
     const long N = static_cast<long>(fx.n[0]) * fx.n[1] * fx.n[2];
     if (N <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_rhs_fixup<T>), dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0, s, rhs, fx);
+    const dim3 grid(static_cast<unsigned>((N + 255) / 256));
+    if (N + 256 < (1L << 32))
+        hipLaunchKernelGGL((k_rhs_fixup<T, unsigned>), grid, dim3(256), 0, s, rhs, fx);
+    else
+        hipLaunchKernelGGL((k_rhs_fixup<T, long>), grid, dim3(256), 0, s, rhs, fx);
     return hipGetLastError();
 }
 
