@@ -4,7 +4,8 @@ kernel adds in a fixed order. Checked against the oracle for every variant that 
 form (0, 1, 2, 4), every form without a k field or source in the kernel, in both
 precisions, on meshes whose element counts are not multiples of the tile (partial last
 tiles) and on meshes smaller than one tile in some direction; against the colour launches
-of the same variant; and for determinism (repeated runs bitwise equal)."""
+of the same variant; and for determinism (repeated runs bitwise equal). Steps, which leave
+the fix-up to their sweep phase, against the hook sequence that runs it at once."""
 import numpy as np
 import pytest
 
@@ -103,6 +104,46 @@ def test_one_launch_graph_steps_equal_eager_and_autotune_keeps_tiling_valid():
     a.autotune(form)
     assert a.get_tuning(ag.TUNE_RHS_TILING) in (0, 1)
     assert a.get_tuning(ag.TUNE_RHS_VARIANT) in range(6)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("dim,prec,dirichlet,tiling,sweep", [
+    (3, ag.FP64, True, 1, 0), (3, ag.FP64, True, 0, 0), (3, ag.FP64, False, 1, 0), (3, ag.FP64, True, 1, 1),
+    (2, ag.FP64, True, 1, 0), (2, ag.FP64, True, 0, 0), (2, ag.FP64, True, 1, 1),
+    (3, ag.FP32, False, 1, 0), (3, ag.FP32, True, 1, 0), (2, ag.FP32, False, 1, 0),
+])
+def test_step_with_deferred_fixup_equals_the_hook_sequence(dim, prec, dirichlet, tiling, sweep):
+    """adsg_step leaves the one-launch fix-up, and in FP64 the boundary zeroing, to the sweep
+    phase (one pass before the x sweep; the zeroing alone inside the x sweep's line load with
+    the colour launches; a separate pass before the one-lane-per-line sweep). The steps equal
+    the reference's hook sequence (before_step swap, compute_rhs, boundary zeroing, ads_solve;
+    validation.hpp:73-81) through the C ABI, where each part runs on its own, bit for bit."""
+    ne = (21, 18, 19) if dim == 3 else (37, 40)
+    _log(f"case: deferred fix-up {dim}D prec={prec} dirichlet={dirichlet} tiling={tiling} sweep={sweep}")
+    sims = []
+    for _ in range(2):
+        h = ag.Simulation(dim, [2] * dim, list(ne), precision=prec)
+        if dirichlet:
+            h.fix_all_boundaries()
+        assert h.prepare_matrices() == 0
+        h.set_tuning(ag.TUNE_RHS_TILING, tiling)
+        h.set_tuning(ag.TUNE_SWEEP_VARIANT, sweep)
+        sims.append(h)
+    a, b = sims
+    u = np.random.default_rng(7 + dim).uniform(0, 1, a.size)
+    a.set(ag.U, u)
+    b.set(ag.U, u)
+    form = ag.rhs_form(2e-4, grad=(1.0, 0.8, 1.2))
+    flags = ag.STEP_DIRICHLET if dirichlet else 0
+    a.step(3, form, flags)
+    for _ in range(3):
+        b.swap()
+        b.compute_rhs(form)
+        if dirichlet:
+            b.zero_boundary()
+        b.ads_solve()
+    assert np.array_equal(a.get(ag.U), b.get(ag.U))
     a.close()
     b.close()
 
