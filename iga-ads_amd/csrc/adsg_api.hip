@@ -230,7 +230,9 @@ adsg::line_geometry geometry(const adsg_sim* h, int dir) {
 
 // mode: 0 = A^-1, 1 = A^-T, 2 = A^-1 with one FP64-residual refinement (FP32 data)
 template <typename T>
-int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fixup<T>* fix = nullptr) {
+int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fixup<T>* fix = nullptr,
+              const adsg::seg_apply* apply = nullptr, bool* applied = nullptr) {
+    if (applied) *applied = false;
     if (sl < 0 || sl >= static_cast<int>(h->dirs[d].slots.size()) || !h->dirs[d].slots[sl].factored)
         return fail(h, ADSG_ESTATE, "ads_solve: matrix slot not set or not factored");
     const dev_slot& s = h->dirs[d].slots[sl];
@@ -274,7 +276,7 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fi
         sf.mseg = s.seg.mseg;
         sf.transposed = mode == 1;
         sf.refine = mode == 2;
-        e = adsg::launch_line_sweep_seg<T, T>(u, geometry(h, d), sf, h->stream, fix);
+        e = adsg::launch_line_sweep_seg<T, T>(u, geometry(h, d), sf, h->stream, fix, apply, applied);
         if (e == hipErrorInvalidValue && mode != 0)
             return fail(h, ADSG_EINVAL, "ads_solve: line staging does not fit the LDS for this solve");
     }
@@ -294,9 +296,13 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fi
 // and the others after it in cyclic order; the standard directions commute.
 // fix: the element loop's deferred fix-up and boundary zeroing of u, in one kernel before
 // the sweeps; a zeroing alone is done by the x sweep as it loads its lines when that sweep
-// runs first as the segmented plain solve
+// runs first as the segmented plain solve. apply: the step's increment update, done by the
+// last sweep as it stores its lines where that is a segmented sweep of strided lines
+// (*applied tells; the caller runs the update otherwise).
 template <typename T>
-int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0, const adsg::rhs_fixup<T>* fix = nullptr) {
+int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0, const adsg::rhs_fixup<T>* fix = nullptr,
+              adsg::seg_apply* apply = nullptr, bool* applied = nullptr) {
+    if (applied) *applied = false;
     int special = -1;
     for (int d = 0; d < h->dim; ++d) {
         const int sl = slots ? slots[d] : 0;
@@ -321,7 +327,10 @@ int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0, const adsg::rhs
     for (int i = 0; i < 3; ++i) {
         if (i < h->dim) {
             const int d = (start + i) % h->dim;
-            int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode, i == 0 && fused ? fix : nullptr);
+            const bool last = i == h->dim - 1 && apply && d != 0;  // d = 0: x lines, stored through LDS
+            if (last) apply->dir = d;
+            int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode, i == 0 && fused ? fix : nullptr,
+                                  last ? apply : nullptr, last ? applied : nullptr);
             if (rc) return rc;
         }
         mark(h, 4 + i);
@@ -429,6 +438,17 @@ void extents(const adsg_sim* h, int& nx, int& ny, int& nz) {
     nz = h->dim == 3 ? h->dirs[2].tab.dofs : 1;
 }
 
+// the increment update u = u_prev + d (boundary 0 when dirichlet; FP32 shadow of u if given)
+adsg::seg_apply increment_update(const adsg_sim* h, float* shadow, bool dirichlet) {
+    adsg::seg_apply a;
+    a.x_prev = static_cast<const double*>(h->vec[ADSG_U_PREV]);
+    a.x_new = static_cast<double*>(h->vec[ADSG_U]);
+    a.shadow = shadow;
+    extents(h, a.n[0], a.n[1], a.n[2]);
+    a.dirichlet = dirichlet;
+    return a;
+}
+
 int zero_boundary_impl(adsg_sim* h) {
     int nx, ny, nz;
     extents(h, nx, ny, nz);
@@ -467,14 +487,17 @@ int solve_impl(adsg_sim* h, const int* slots, const adsg::rhs_fixup<double>* fix
                 return fail(h, ADSG_EINVAL,
                             "ads_solve (FP32): increments are defined against the Gram matrices (slot 0); "
                             "use adsg_step_delta for other matrices");
-    int rc = sweep_all<float>(h, static_cast<float*>(h->vec[ADSG_BUFFER]), nullptr, 0, fix32);
+    adsg::seg_apply ap = increment_update(h, static_cast<float*>(h->shadow[0]), h->pending_dirichlet);
+    bool applied = false;
+    int rc = sweep_all<float>(h, static_cast<float*>(h->vec[ADSG_BUFFER]), nullptr, 0, fix32, &ap, &applied);
     if (rc) return rc;
     int nx, ny, nz;
     extents(h, nx, ny, nz);
-    hipError_t e = adsg::launch_apply_increment(
-        static_cast<const double*>(h->vec[ADSG_U_PREV]), static_cast<const float*>(h->vec[ADSG_BUFFER]),
-        static_cast<double*>(h->vec[ADSG_U]), static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz,
-        h->pending_dirichlet, h->stream);
+    hipError_t e = applied ? hipSuccess
+                           : adsg::launch_apply_increment(
+                                 static_cast<const double*>(h->vec[ADSG_U_PREV]),
+                                 static_cast<const float*>(h->vec[ADSG_BUFFER]), static_cast<double*>(h->vec[ADSG_U]),
+                                 static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz, h->pending_dirichlet, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "apply increment");
     h->pending = false;
     h->pending_dirichlet = false;
@@ -497,6 +520,7 @@ template <typename T>
 void fixup_extents(const adsg_sim* h, adsg::rhs_fixup<T>& fx) {
     extents(h, fx.n[0], fx.n[1], fx.n[2]);
 }
+
 
 int one_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
     swap_state(h);
@@ -534,10 +558,14 @@ int one_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int
         fixup_extents(h, fx);
         fx.zero_boundary = dir;
         mark(h, 3);
-        if ((rc = sweep_all<float>(h, r, slots, (flags & ADSG_STEP_REFINE) ? 2 : 0, &fx))) return rc;
-        e = adsg::launch_apply_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
-                                         static_cast<double*>(h->vec[ADSG_U]),
-                                         static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz, dir, h->stream);
+        adsg::seg_apply ap = increment_update(h, static_cast<float*>(h->shadow[0]), dir);
+        bool applied = false;
+        if ((rc = sweep_all<float>(h, r, slots, (flags & ADSG_STEP_REFINE) ? 2 : 0, &fx, &ap, &applied))) return rc;
+        e = applied ? hipSuccess
+                    : adsg::launch_apply_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
+                                                   static_cast<double*>(h->vec[ADSG_U]),
+                                                   static_cast<float*>(h->shadow[0]), h->dim, nx, ny, nz, dir,
+                                                   h->stream);
     } else {
         adsg::rhs_fixup<double> fx;
         if ((rc = rhs_impl<double>(h, form, true, &fx))) return rc;
@@ -545,10 +573,13 @@ int one_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int
         fixup_extents(h, fx);
         fx.zero_boundary = dir;
         mark(h, 3);
-        if ((rc = sweep_all<double>(h, r, slots, 0, &fx))) return rc;
-        e = adsg::launch_add_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
-                                       static_cast<double*>(h->vec[ADSG_U]), h->dim, nx, ny, nz, dir,
-                                       h->stream);
+        adsg::seg_apply ap = increment_update(h, nullptr, dir);
+        bool applied = false;
+        if ((rc = sweep_all<double>(h, r, slots, 0, &fx, &ap, &applied))) return rc;
+        e = applied ? hipSuccess
+                    : adsg::launch_add_increment(static_cast<const double*>(h->vec[ADSG_U_PREV]), r,
+                                                 static_cast<double*>(h->vec[ADSG_U]), h->dim, nx, ny, nz, dir,
+                                                 h->stream);
     }
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "apply increment");
 }

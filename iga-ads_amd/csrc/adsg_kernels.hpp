@@ -64,12 +64,27 @@ struct rhs_fixup {
     bool pending() const { return spill != nullptr || zero_boundary; }
 };
 
+// The increment update of the delta and FP32 steps, x_new = x_prev + d with the boundary
+// entries 0 when dirichlet (k_add_increment / k_apply_increment; shadow = FP32 copy of x_new
+// when set), done by the last sweep of the step as it stores its lines instead of storing d.
+// dir: the direction of that sweep; n: DOFs per direction (1 beyond the dimension).
+struct seg_apply {
+    const double* x_prev = nullptr;
+    double* x_new = nullptr;
+    float* shadow = nullptr;
+    int n[3] = {1, 1, 1};
+    int dir = 0;
+    bool dirichlet = false;
+};
+
 // fix: a deferred boundary zeroing (rhs_fixup without spill entries) to apply while the x
 // lines (geometry with S == 1, the plain solve) are loaded, or null; the caller applies
-// other fix-ups first (launch_rhs_fixup)
+// other fix-ups first (launch_rhs_fixup). apply: the increment update to do while lines with
+// S != 1 are stored (not the transposed solve), or null: *applied tells whether it was done.
 template <typename T, typename F>
 hipError_t launch_line_sweep_seg(T* u, const line_geometry& g, const seg_factors<F>& f,
-                                 hipStream_t s, const rhs_fixup<T>* fix = nullptr);
+                                 hipStream_t s, const rhs_fixup<T>* fix = nullptr,
+                                 const seg_apply* apply = nullptr, bool* applied = nullptr);
 
 // Per-line factors (one band matrix per line, lines of geometry g in its order): entry k
 // of line l at k * n_lines + l, widths kl and ku (ku = kl + ku of the matrix when pivoted).
