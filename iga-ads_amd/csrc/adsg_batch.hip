@@ -218,6 +218,11 @@ int build_groups(adsg_batch* b) {
                 a.first = g.gx_blocks;
                 g.gx_blocks += static_cast<int>((ny + 63) / 64);
                 c.g = yi.g;
+                if constexpr (sizeof(T) == sizeof(float)) {  // the y sweep does the FP32 step's update
+                    c.x_prev = b->u[ph] + b->off[i];
+                    c.x_new = b->u[1 - ph] + b->off[i];
+                    c.shadow = b->sh[1 - ph] + b->off[i];
+                }
                 c.pack = static_cast<const T*>(sgy.pack);
                 c.n = static_cast<int>(ny);
                 c.m = sgy.m;
@@ -268,7 +273,8 @@ int build(adsg_batch* b) {
 
 // one step of every problem, starting at arena parity b->phase: the zero fill, the element
 // kernels of the problems of each degree in one launch per colour, the x lines of every
-// problem (every degree) in one launch, the y lines in one, and (FP32) the update
+// problem (every degree) in one launch, the y lines in one, and (FP32) the update: done by
+// the segmented y sweeps as they store their lines, else a pass of its own
 template <typename T>
 int batch_step_impl(adsg_batch* b) {
     const int ph = b->phase;
@@ -303,9 +309,13 @@ int batch_step_impl(adsg_batch* b) {
     }
     if constexpr (sizeof(T) == sizeof(float)) {
         // u = u_prev + increment in FP64, with its FP32 copy (no boundary rows: natural BC)
-        e = adsg::launch_apply_increment(b->u[ph], b->buf, b->u[1 - ph], b->sh[1 - ph], 2,
-                                         static_cast<int>(b->total), 1, 1, false, b->stream);
-        if (e != hipSuccess) return bhip(b, e, "batch: update");
+        bool updated = true;  // by the segmented y sweeps (seg_item::x_new), in every group
+        for (const batch_group& g : b->groups[ph]) updated = updated && g.seg && b->sweep == 0;
+        if (!updated) {
+            e = adsg::launch_apply_increment(b->u[ph], b->buf, b->u[1 - ph], b->sh[1 - ph], 2,
+                                             static_cast<int>(b->total), 1, 1, false, b->stream);
+            if (e != hipSuccess) return bhip(b, e, "batch: update");
+        }
     }
     b->phase = 1 - ph;
     return ADSG_OK;

@@ -224,6 +224,11 @@ struct seg_item {
     const T* pack = nullptr;
     int n = 0, m = 0, nseg = 0, p = 0;
     int first = 0;
+    // when x_new is set (the last sweep of an FP32 step): the solved values are not stored;
+    // x_new = x_prev + value and shadow = its FP32 copy are (same offsets as u; no boundary rows)
+    const double* x_prev = nullptr;
+    double* x_new = nullptr;
+    float* shadow = nullptr;
 };
 template <typename T>
 hipError_t launch_sweep_seg_batch(const seg_item<T>* items, int nitems, int blocks, hipStream_t s);
