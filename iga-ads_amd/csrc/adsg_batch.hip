@@ -177,6 +177,7 @@ int build_groups(adsg_batch* b) {
             it.prm.gy = static_cast<T>(fm.grad[1]);
             it.prm.gz = static_cast<T>(fm.grad[2]);
             it.prm.reaction = static_cast<T>(fm.reaction);
+            it.prm.store = true;  // no zero fill: the first colour to reach a DOF stores
             int count[4];
             adsg::rhs2_batch_tiles(b->shape, it.X.elements, it.Y.elements, it.ntx2, count);
             for (int c = 0; c < 4; ++c) {
@@ -271,20 +272,15 @@ int build(adsg_batch* b) {
     return b->precision == ADSG_FP32 ? build_groups<float>(b) : build_groups<double>(b);
 }
 
-// one step of every problem, starting at arena parity b->phase: the zero fill, the element
-// kernels of the problems of each degree in one launch per colour, the x lines of every
-// problem (every degree) in one launch, the y lines in one, and (FP32) the update: done by
-// the segmented y sweeps as they store their lines, else a pass of its own
+// one step of every problem, starting at arena parity b->phase: the element kernels of the
+// problems of each degree in one launch per colour (the first colour to reach a DOF stores:
+// no zero fill), the x lines of every problem (every degree) in one launch, the y lines in
+// one, and (FP32) the update: done by the segmented y sweeps as they store their lines,
+// else a pass of its own
 template <typename T>
 int batch_step_impl(adsg_batch* b) {
     const int ph = b->phase;
-    T* zero = nullptr;
-    if constexpr (sizeof(T) == sizeof(float))
-        zero = b->buf;
-    else
-        zero = b->u[1 - ph];
-    hipError_t e = adsg::launch_fill<T>(zero, b->total, T(0), b->stream);
-    if (e != hipSuccess) return bhip(b, e, "batch: zero fill");
+    hipError_t e = hipSuccess;
     for (const batch_group& g : b->groups[ph]) {
         const auto* items = static_cast<const adsg::rhs2_item<T>*>(g.rhs_items);
         for (int p = 1; p <= 3 && e == hipSuccess; ++p)

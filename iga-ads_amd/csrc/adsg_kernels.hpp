@@ -139,8 +139,9 @@ struct rhs_params {
     // host side only: when set, a one-launch element loop leaves its fix-up to the caller and
     // describes it here (defer->spill null when there is none to do)
     rhs_fixup<T>* defer = nullptr;
-    // set by launch_rhs for the one-launch tiling when the rhs start value is not used: the
-    // owners store their entries instead of adding them
+    // set by launch_rhs (and for batches) when the rhs start value is not used: the owners of
+    // the one-launch tiling, and the first colour to reach a DOF in the 2D colour launches,
+    // store their entries instead of adding them
     bool store = false;
 };
 

@@ -899,6 +899,16 @@ __device__ __forceinline__ void rhs2_flush(const T (&R)[P + 1][P + 1], double* _
                 const int cy = dy ? (ty + 1) * P + by - TY : gy;
                 prm.spill[spill_index(prm, dx, dy, 0, cx, cy, 0)] = static_cast<T>(sum);
             }
+        } else if (prm.store) {
+            // colour launches on an rhs without a start value: the first colour to reach a DOF
+            // stores. Another tile holds the entry in x when bx < P (tile tx - 1) or bx >= TX
+            // (tile tx + 1), in y likewise; those tiles' colours flip the parity bits, and the
+            // colours run in the order 0, 1, 2, 3.
+            const int c = (tx & 1) | ((ty & 1) << 1);
+            const bool hx = (bx < P && tx > 0) || (bx >= TX && !lastx);
+            const bool hy = (by < P && ty > 0) || (by >= TY && !lasty);
+            const bool first = (!hx || !(c & 1)) && (!hy || !(c & 2));
+            rhs[g] = first ? static_cast<T>(sum) : rhs[g] + static_cast<T>(sum);
         } else {
             rhs[g] += static_cast<T>(sum);
         }
@@ -1586,8 +1596,10 @@ hipError_t rhs_fields(int dim, const T* up, T* rhs, const dev_tables<T>* d,
     // per-element variant
     const bool one = (code & kRhsOneLaunch) != 0 && prm.spill &&
                      (dim == 2 || (variant != 3 && variant != 5 && !kf && !src));
-    prm.store = one && overwrite;
-    if (overwrite && !one) {
+    // overwrite: the one launch's owners store, and so do the first colour to reach a DOF in
+    // the 2D colour launches; the 3D colour launches add into a zero fill
+    prm.store = overwrite && (one || dim == 2);
+    if (overwrite && !one && dim != 2) {
         const long n = static_cast<long>(d[0].dofs) * d[1].dofs * (dim == 3 ? d[2].dofs : 1);
         const hipError_t e = launch_fill<T>(rhs, n, T(0), s);
         if (e != hipSuccess) return e;
