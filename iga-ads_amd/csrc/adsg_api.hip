@@ -265,7 +265,7 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fi
                     "ads_solve: transposed and refined solves need unpivoted factors and lines of "
                     "2..16 segments");
     hipError_t e = hipErrorInvalidValue;
-    if (seg_ok && (mode != 0 || h->sweep_variant == 0)) {
+    if (seg_ok && (mode != 0 || h->sweep_variant != 1)) {
         adsg::seg_factors<T> sf;
         sf.f = f;
         sf.pack = static_cast<const T*>(s.pack);
@@ -276,6 +276,7 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fi
         sf.mseg = s.seg.mseg;
         sf.transposed = mode == 1;
         sf.refine = mode == 2;
+        sf.direct = h->sweep_variant == 2;  // x lines read lane by lane (small meshes)
         e = adsg::launch_line_sweep_seg<T, T>(u, geometry(h, d), sf, h->stream, fix, apply, applied);
         if (e == hipErrorInvalidValue && mode != 0)
             return fail(h, ADSG_EINVAL, "ads_solve: line staging does not fit the LDS for this solve");
@@ -317,7 +318,7 @@ int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0, const adsg::rhs
         const int sl = slots ? slots[0] : 0;
         const bool seg_x = sl >= 0 && sl < static_cast<int>(h->dirs[0].slots.size()) &&
                            h->dirs[0].slots[sl].factored && h->dirs[0].slots[sl].lines == 0 &&
-                           h->dirs[0].slots[sl].seg.nseg > 0 && h->sweep_variant == 0;
+                           h->dirs[0].slots[sl].seg.nseg > 0 && h->sweep_variant != 1;
         fused = !fix->spill && start == 0 && mode == 0 && seg_x;  // zeroing only: in the x sweep
         if (!fused) {
             const hipError_t e = adsg::launch_rhs_fixup<T>(u, *fix, h->stream);
@@ -1175,10 +1176,10 @@ int adsg_prepare_matrices(adsg_sim* h) {
                 if ((rc = upload<double>(h, &s.rdiag, s.lf.rdiag))) return rc;
             }
             s.seg = adsg::line_segments{};
-            // LDS per segment wave (x-line staging 64 x (rows+1), chain and refinement halos)
-            // within 150 KB: FP64 14 waves of 16-row or 8 of 32-row segments, FP32 16 or 13
-            const bool f64 = h->precision == ADSG_FP64;
-            if (adsg::segment_lines(s.lf, s.seg, f64 ? 14 : 16, f64 ? 8 : 13)) {
+            // up to 16 segment waves (1024 lanes) of 16 or 32 rows; x lines are staged through
+            // LDS where that fits (150 KB: FP64 14 waves of 16 rows or 8 of 32, FP32 16 or 13)
+            // and read lane by lane otherwise
+            if (adsg::segment_lines(s.lf, s.seg, 16, 16)) {
                 if (h->precision == ADSG_FP32) {
                     if ((rc = upload<float>(h, &s.pack, s.seg.pack))) return rc;
                     if ((rc = upload<float>(h, &s.packt, s.seg.packt))) return rc;
@@ -1713,7 +1714,10 @@ int adsg_set_tuning(adsg_sim* h, int key, int value) {
                 : key == ADSG_TUNE_RHS_TILING    ? &h->rhs_tiling
                 : key == ADSG_TUNE_CACHE         ? &h->tune_cache
                                                  : nullptr;
-    const int max_value = key == ADSG_TUNE_RHS_VARIANT ? 5 : key == ADSG_TUNE_RHS_COLUMNS ? 3 : 1;
+    const int max_value = key == ADSG_TUNE_RHS_VARIANT     ? 5
+                          : key == ADSG_TUNE_RHS_COLUMNS   ? 3
+                          : key == ADSG_TUNE_SWEEP_VARIANT ? 2
+                                                           : 1;
     if (!slot || value < 0 || value > max_value) return fail(h, ADSG_EINVAL, "set_tuning: unknown key or value");
     if (key == ADSG_TUNE_RHS_TILING && value == 1) {
         const int rc = ensure_spill(h);
@@ -1819,8 +1823,9 @@ int adsg_autotune(adsg_sim* h, const adsg_rhs_form* form, const int* slots, int 
         h->rhs_variant = v;
         h->rhs_col_shape = cand[k][1];
         h->rhs_tiling = cand[k][2];
-        // both sweeps timed with variants 0, 1 in colour launches
-        const bool sweep_run = v <= 1 && cand[k][2] == 0;
+        // the sweep variants 0, 1, 2 timed with the element variants of the same number in
+        // colour launches
+        const bool sweep_run = v <= 2 && cand[k][2] == 0;
         h->sweep_variant = sweep_run ? v : best_sweep;
         double ms[7];
         rc = adsg_time_phases(h, 1, form, slots, flags, ms);  // warm-up (code load, caches)

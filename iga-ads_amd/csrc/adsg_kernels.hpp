@@ -43,6 +43,7 @@ struct seg_factors {
     const double* arow = nullptr;  // [n*(kl+ku+1)]
     int m = 0, nseg = 0, mseg = 0;  // mseg: register capacity per lane (16 or 32)
     bool transposed = false, refine = false;
+    bool direct = false;  // lines with S == 1 read by each lane directly, not staged through LDS
 };
 
 // The one-launch element loop's fix-up (adding the spill entries of the earlier tiles that

@@ -314,7 +314,9 @@ int adsg_time_phases(adsg_sim* h, int nsteps, const adsg_rhs_form* form, const i
  * 1; FP32: 1).
  * ADSG_TUNE_SWEEP_VARIANT selects the line sweep (0: lines cut into segments solved by
  * separate waves and joined through FP64-computed influence coefficients, where the factors
- * are unpivoted and the lines are 2..16 segments long; 1: one lane per whole line).
+ * are unpivoted and the lines are 2..16 segments of at most 32 rows long (up to 512 DOFs),
+ * x lines staged through LDS where that fits; 1: one lane per whole line; 2: as 0 with the x
+ * lines read lane by lane, faster on some small meshes).
  * adsg_autotune times every variant on the handle's current state with the given form
  * (explicit steps, or delta steps with slots) and keeps the fastest of each; the state
  * vectors are restored afterwards. */
