@@ -1,9 +1,11 @@
 // HIP element-loop RHS of the MI355X ADS step (gfx950): compute_rhs (validation.hpp:95-119)
 // with one workgroup per tile of elements, one lane per element, sum factorisation through
 // the 1D tables, local contributions combined in LDS, and tiles of one parity colour
-// (2^d colours) adding their DOF blocks into the global rhs without atomics. In 3D the tiles
-// can also run in one launch: each tile adds the block entries it owns and spills the rest,
-// which a second kernel adds in a fixed order (deterministic either way).
+// (2^d colours) adding their DOF blocks into the global rhs without atomics. The tiles can
+// also run in one launch: each tile adds the block entries it owns and spills the rest,
+// which a second kernel (k_rhs_fixup; in a step, at the start of the sweep phase) adds in a
+// fixed order (deterministic either way). The adjoint's k-field gradient runs on the same
+// tiles.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
