@@ -661,14 +661,16 @@ int accumulate_source_gradient(adsg_sim* h, const adsg_rhs_form* form, const dou
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "source gradient");
 }
 
-// One step of the adjoint of one_step (FP64): see adsg_adjoint_step in adsgpu.h.
-int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
+// One step of the adjoint of one_step (FP64): see adsg_adjoint_step in adsgpu.h. u_prev:
+// the forward state of the step (default: ADSG_U_PREV); ADSG_U_PREV is overwritten either way
+// when the form has a source.
+int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags, const double* u_prev = nullptr) {
     int nx, ny, nz;
     extents(h, nx, ny, nz);
     const bool dir = flags & ADSG_STEP_DIRICHLET;
     double* lam = static_cast<double*>(h->vec[ADSG_U]);
     double* w = static_cast<double*>(h->vec[ADSG_BUFFER]);
-    const double* un = static_cast<const double*>(h->vec[ADSG_U_PREV]);
+    const double* un = u_prev ? u_prev : static_cast<const double*>(h->vec[ADSG_U_PREV]);
     HIP_TRY(h, hipMemcpyAsync(w, lam, sizeof(double) * h->N, hipMemcpyDeviceToDevice, h->stream));
     int rc = sweep_all<double>(h, w, nullptr, 1);
     if (rc) return rc;
@@ -684,7 +686,7 @@ int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags) {
         prm.gx = form->grad[0];
         prm.gy = form->grad[1];
         prm.gz = form->grad[2];
-        prm.spill = h->rhs_tiling ? static_cast<double*>(h->spill) : nullptr;  // 2D: the element tiling's
+        prm.spill = h->rhs_tiling ? static_cast<double*>(h->spill) : nullptr;  // one launch and its fix-up
         e = adsg::launch_kgrad<double>(h->dim, h->dirs[0].tab.p, un, w,
                                        static_cast<double*>(h->vec[ADSG_GRAD_K]), t, prm, h->stream);
         if (e != hipSuccess) return hip_fail(h, e, "k-field gradient");
@@ -1426,9 +1428,8 @@ int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags)
         }
         HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U], lam_save, bytes, hipMemcpyDeviceToDevice, h->stream));
         for (int s = stop - 1; s >= start; --s) {
-            HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U_PREV], h->ck_seg + static_cast<size_t>(s - start) * h->N,
-                                      bytes, hipMemcpyDeviceToDevice, h->stream));
-            int rc = one_adjoint_step(h, form, flags);
+            // the step's forward state read where the recompute left it
+            int rc = one_adjoint_step(h, form, flags, h->ck_seg + static_cast<size_t>(s - start) * h->N);
             if (rc) return rc;
         }
     }
