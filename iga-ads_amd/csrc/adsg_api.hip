@@ -661,6 +661,41 @@ int accumulate_source_gradient(adsg_sim* h, const adsg_rhs_form* form, const dou
     return e == hipSuccess ? ADSG_OK : hip_fail(h, e, "source gradient");
 }
 
+// 2D with a k field in the one-launch tiling: the element loop for R(k) w (into lam) and
+// the k-field gradient (into ADSG_GRAD_K) in one launch over all tiles and one fix-up launch
+// for both (launch_adjoint2); false when that path does not apply (nothing launched)
+bool adjoint_pair_2d(adsg_sim* h, const adsg_rhs_form* form, const double* w, const double* un, double* lam,
+                     int& rc) {
+    rc = ADSG_OK;
+    if (h->dim != 2 || !form->use_kfield || h->general || !h->rhs_tiling || !h->spill || !h->vec[ADSG_KFIELD])
+        return false;
+    if (!h->spill2) {
+        const int el[3] = {h->dirs[0].tab.elements, h->dirs[1].tab.elements, 1};
+        const size_t bytes = static_cast<size_t>(adsg::rhs_spill_slots(2, h->dirs[0].tab.p, el)) * sizeof(double);
+        if (hipMalloc(&h->spill2, bytes) != hipSuccess) {
+            h->spill2 = nullptr;
+            return false;  // the separate passes need no second array
+        }
+    }
+    adsg::dev_tables<double> t[3];
+    dir_tables(h, t);
+    adsg::rhs_params<double> pr, pk;
+    pr.mass = form->mass;
+    pr.dt = pk.dt = form->dt;
+    pr.gx = pk.gx = form->grad[0];
+    pr.gy = pk.gy = form->grad[1];
+    pr.gz = pk.gz = form->grad[2];
+    pr.reaction = 0.0;
+    pr.kfield = static_cast<const double*>(h->vec[ADSG_KFIELD]);
+    mark(h, 1);
+    const hipError_t e = adsg::launch_adjoint2<double>(
+        h->dirs[0].tab.p, w, un, lam, static_cast<double*>(h->vec[ADSG_GRAD_K]), t, pr, pk,
+        static_cast<double*>(h->spill), static_cast<double*>(h->spill2), h->stream);
+    mark(h, 2);
+    if (e != hipSuccess) rc = hip_fail(h, e, "adjoint element passes");
+    return true;
+}
+
 // One step of the adjoint of one_step (FP64): see adsg_adjoint_step in adsgpu.h. u_prev:
 // the forward state of the step (default: ADSG_U_PREV); ADSG_U_PREV is overwritten either way
 // when the form has a source.
@@ -678,7 +713,10 @@ int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags, const do
     if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, w, nx, ny, nz, h->stream)) != hipSuccess)
         return hip_fail(h, e, "zero boundary");
     mark(h, 3);
-    if (form->use_kfield) {
+    int rp = ADSG_OK;
+    const bool paired = adjoint_pair_2d(h, form, w, un, lam, rp);
+    if (rp) return rp;
+    if (form->use_kfield && !paired) {
         adsg::dev_tables<double> t[3];
         dir_tables(h, t);
         adsg::rhs_params<double> prm;
@@ -694,7 +732,7 @@ int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags, const do
     // the source enters as dt M f: dL/df += dt M w (u_prev, read above, is free now)
     int rs = accumulate_source_gradient(h, form, w, static_cast<double*>(h->vec[ADSG_U_PREV]));
     if (rs) return rs;
-    return apply_linear_form(h, form, w, lam);
+    return paired ? ADSG_OK : apply_linear_form(h, form, w, lam);
 }
 
 // One step of the adjoint of one_delta_step (FP64) for a linear form r(u) = R u + c:
@@ -1064,6 +1102,7 @@ void adsg_destroy(adsg_sim* h) {
     if (h->nonfinite) (void)hipFree(h->nonfinite);
     if (h->source_load) (void)hipFree(h->source_load);
     if (h->spill) (void)hipFree(h->spill);
+    if (h->spill2) (void)hipFree(h->spill2);
     if (h->ckpt) (void)hipFree(h->ckpt);
     if (h->ck_seg) (void)hipFree(h->ck_seg);
     if (h->stream && h->owns_stream) (void)hipStreamDestroy(h->stream);

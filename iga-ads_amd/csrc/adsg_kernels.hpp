@@ -243,6 +243,14 @@ hipError_t launch_sweep_seg_batch(const seg_item<T>* items, int nitems, int bloc
 template <typename T>
 hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,
                         const rhs_params<T>& prm, hipStream_t s);
+// The 2D explicit adjoint step's two element passes in one launch over all tiles and one
+// fix-up launch for both: out_r := the element loop of prm_r (k field set) for w, out_k +=
+// the k-field gradient (prm_k's dt and gradient weights) of u and w; spill_r, spill_k: two
+// spill arrays of rhs_spill_slots(2, p, ...) values (FP64)
+template <typename T>
+hipError_t launch_adjoint2(int p, const T* w, const T* u, T* out_r, T* out_k, const dev_tables<T>* dirs,
+                           const rhs_params<T>& prm_r, const rhs_params<T>& prm_k, T* spill_r, T* spill_k,
+                           hipStream_t s);
 
 // Squared norm / error functional: sum_e sum_q w J phi(d), d = u_h - s, u_h the spline of
 // the FP64 coefficients u (0 when u is null), s given at the Gauss points as planes

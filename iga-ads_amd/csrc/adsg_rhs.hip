@@ -1203,6 +1203,23 @@ __global__ __launch_bounds__(TX* TY) void k_kgrad2(const T* __restrict__ u, cons
     kgrad2_tile<T, P, TX, TY, ONE>(u, w, out, X, Y, prm, colour, xcd_contiguous(blockIdx.x, gridDim.x), ntx2);
 }
 
+// The 2D adjoint step's two element passes in one launch over all tiles (one-launch tiling):
+// blocks [0, tiles) the element loop of the form for w (rhs2_tile, into out_r, prm_r), blocks
+// [tiles, 2 tiles) the k-field gradient of u and w (kgrad2_tile, into out_k, prm_k), each
+// with its own spill arrays; the two halves share nothing, and twice the blocks fill more of
+// the device than either pass alone on small meshes.
+template <typename T, int P, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_adjoint2(const T* __restrict__ w, const T* __restrict__ u,
+                                                     T* __restrict__ out_r, T* __restrict__ out_k, dev_tables<T> X,
+                                                     dev_tables<T> Y, rhs_params<T> prm_r, rhs_params<T> prm_k,
+                                                     int ntx, int tiles) {
+    const int b = static_cast<int>(blockIdx.x);
+    if (b < tiles)
+        rhs2_tile<T, P, TX, TY, true, false, true>(w, out_r, X, Y, prm_r, 0, b, ntx);
+    else
+        kgrad2_tile<T, P, TX, TY, true>(u, w, out_k, X, Y, prm_k, 0, b - tiles, ntx);
+}
+
 // Many 2D problems of degree P in one launch per parity colour: block b belongs to the last
 // item whose first[colour] <= b (items in problem order), and is tile b - first[colour] of
 // it. (A one-launch form with spill arrays measured slower here: every colour launch already
@@ -1330,12 +1347,11 @@ __device__ __forceinline__ double rhs_fixup_sum(const rhs_fixup<T>& fx, int gx, 
 // that has one (the entries of the up to 7 earlier tiles that hold it: the tile t_d - 1 of a
 // direction holds an entry for DOF g in its top P layers when l_d < P and t_d > 0), then the
 // boundary zeroing (k_zero_boundary's, when asked for), in one pass.
-// (I: the index type, 32-bit unsigned below 2^32 DOFs: its divisions are the cheap ones)
+// One DOF g < N of a fix-up (I: the index type, 32-bit unsigned below 2^32 DOFs: its
+// divisions are the cheap ones).
 template <typename T, typename I>
-__global__ __launch_bounds__(256) void k_rhs_fixup(T* __restrict__ rhs, rhs_fixup<T> fx) {
-    const I nx = fx.n[0], ny = fx.n[1], nz = fx.n[2];
-    const I g = static_cast<I>(blockIdx.x) * 256 + threadIdx.x;
-    if (g >= nx * ny * nz) return;
+__device__ __forceinline__ void rhs_fixup_one(T* __restrict__ rhs, const rhs_fixup<T>& fx, I g) {
+    const I nx = fx.n[0], ny = fx.n[1];
     const I r = g / nx;
     const int gx = static_cast<int>(g - r * nx);
     const I rz = r / ny;
@@ -1353,6 +1369,28 @@ __global__ __launch_bounds__(256) void k_rhs_fixup(T* __restrict__ rhs, rhs_fixu
     const double sum = rhs_fixup_sum(fx, gx, tx, gx - fx.tile[0] * tx, gy, ty, gy - fx.tile[1] * ty, gz, tz,
                                      gz - fx.tile[2] * tz, any);
     if (any) rhs[g] += static_cast<T>(sum);
+}
+
+template <typename T, typename I>
+__global__ __launch_bounds__(256) void k_rhs_fixup(T* __restrict__ rhs, rhs_fixup<T> fx) {
+    const I g = static_cast<I>(blockIdx.x) * 256 + threadIdx.x;
+    if (g >= static_cast<I>(fx.n[0]) * fx.n[1] * fx.n[2]) return;
+    rhs_fixup_one<T, I>(rhs, fx, g);
+}
+
+// the fix-ups of two one-launch passes in one launch: blocks [0, half) the first, the rest
+// the second
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_fixup_pair(T* __restrict__ a, rhs_fixup<T> fa, T* __restrict__ b,
+                                                        rhs_fixup<T> fb, unsigned half) {
+    const bool first = blockIdx.x < half;
+    const unsigned g = (first ? blockIdx.x : blockIdx.x - half) * 256 + threadIdx.x;
+    const unsigned N = static_cast<unsigned>(fa.n[0]) * fa.n[1] * fa.n[2];
+    if (g >= N) return;
+    if (first)
+        rhs_fixup_one<T, unsigned>(a, fa, g);
+    else
+        rhs_fixup_one<T, unsigned>(b, fb, g);
 }
 
 // the end of a one-launch element loop (spill layout in q): its fix-up described for the
@@ -1643,6 +1681,58 @@ hipError_t launch_rhs(int dim, int p, const T* u_prev, T* rhs, const dev_tables<
     }
 }
 
+// the 2D adjoint pair (k_adjoint2): out_r := the element loop of prm_r for w (stored: the
+// start value of out_r is unused), out_k += the k-field gradient of u and w; spill_r and
+// spill_k: two spill arrays of rhs_spill_slots(2, p, ...) values
+template <typename T, int P>
+hipError_t adjoint2(const T* w, const T* u, T* out_r, T* out_k, const dev_tables<T>* d, const rhs_params<T>& prm_r,
+                    const rhs_params<T>& prm_k, T* spill_r, T* spill_k, hipStream_t s) {
+    using TL = tile<2, P>;
+    const int ntx = (d[0].elements + TL::X - 1) / TL::X, nty = (d[1].elements + TL::Y - 1) / TL::Y;
+    const int tiles = ntx * nty;
+    if (tiles <= 0) return hipSuccess;
+    if (d[0].dofs != d[0].elements + P || d[1].dofs != d[1].elements + P || !spill_r || !spill_k)
+        return hipErrorInvalidValue;
+    const int n[2] = {d[0].dofs, d[1].dofs}, nt[2] = {ntx, nty};
+    rhs_params<T> qr = prm_r, qk = prm_k;
+    qr.spill = spill_r;
+    qr.store = true;
+    qr.defer = nullptr;
+    qk.spill = spill_k;
+    qk.store = false;
+    qk.defer = nullptr;
+    spill_layout2(P, n, nt, qr.spill_off, qr.spill_ext);
+    spill_layout2(P, n, nt, qk.spill_off, qk.spill_ext);
+    hipLaunchKernelGGL((k_adjoint2<T, P, TL::X, TL::Y>), dim3(2 * tiles), dim3(TL::X * TL::Y), 0, s, w, u, out_r, out_k,
+                       d[0], d[1], qr, qk, ntx, tiles);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int tl[3] = {TL::X, TL::Y, 1}, nt3[3] = {ntx, nty, 1}, n3[3] = {n[0], n[1], 1};
+    rhs_fixup<T> fr, fk;
+    defer_fixup(fr, qr, P, tl, nt3, n3);
+    defer_fixup(fk, qk, P, tl, nt3, n3);
+    const long N = static_cast<long>(n[0]) * n[1];
+    const unsigned half = static_cast<unsigned>((N + 255) / 256);
+    hipLaunchKernelGGL((k_rhs_fixup_pair<T>), dim3(2 * half), dim3(256), 0, s, out_r, fr, out_k, fk, half);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_adjoint2(int p, const T* w, const T* u, T* out_r, T* out_k, const dev_tables<T>* dirs,
+                           const rhs_params<T>& prm_r, const rhs_params<T>& prm_k, T* spill_r, T* spill_k,
+                           hipStream_t s) {
+    switch (p) {
+        case 1:
+            return adjoint2<T, 1>(w, u, out_r, out_k, dirs, prm_r, prm_k, spill_r, spill_k, s);
+        case 2:
+            return adjoint2<T, 2>(w, u, out_r, out_k, dirs, prm_r, prm_k, spill_r, spill_k, s);
+        case 3:
+            return adjoint2<T, 3>(w, u, out_r, out_k, dirs, prm_r, prm_k, spill_r, spill_k, s);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
 template <typename T>
 hipError_t launch_kgrad(int dim, int p, const T* u, const T* w, T* out, const dev_tables<T>* dirs,
                         const rhs_params<T>& prm, hipStream_t s) {
@@ -1728,6 +1818,9 @@ hipError_t launch_rhs2_batch(int p, int shape, const rhs2_item<T>* items, int ni
 
 template hipError_t launch_kgrad<double>(int, int, const double*, const double*, double*, const dev_tables<double>*,
                                          const rhs_params<double>&, hipStream_t);
+template hipError_t launch_adjoint2<double>(int, const double*, const double*, double*, double*,
+                                            const dev_tables<double>*, const rhs_params<double>&,
+                                            const rhs_params<double>&, double*, double*, hipStream_t);
 template hipError_t launch_rhs_fixup<double>(double*, const rhs_fixup<double>&, hipStream_t);
 template hipError_t launch_rhs_fixup<float>(float*, const rhs_fixup<float>&, hipStream_t);
 template hipError_t launch_rhs2_batch<double>(int, int, const rhs2_item<double>*, int, int, int, hipStream_t);

@@ -89,6 +89,7 @@ struct adsg_sim {
     // spilled to `spill` (rhs_spill_slots values, FP64-sized) and added after
     int rhs_tiling = 0;
     void* spill = nullptr;
+    void* spill2 = nullptr;  // a second spill array of the same size (the 2D adjoint's paired passes)
     int tune_cache = 1;  // 1: adsg_autotune reuses / records the process's results per configuration
     std::string err;
     hipEvent_t* marks = nullptr;  // set only while adsg_time_phases records phase boundaries
