@@ -707,10 +707,15 @@ int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags, const do
     double* w = static_cast<double*>(h->vec[ADSG_BUFFER]);
     const double* un = u_prev ? u_prev : static_cast<const double*>(h->vec[ADSG_U_PREV]);
     HIP_TRY(h, hipMemcpyAsync(w, lam, sizeof(double) * h->N, hipMemcpyDeviceToDevice, h->stream));
-    int rc = sweep_all<double>(h, w, nullptr, 1);
+    // w = Z M^-T lam: the zeroing Z by the last transposed sweep as it stores its lines
+    adsg::seg_apply zs;
+    extents(h, zs.n[0], zs.n[1], zs.n[2]);
+    zs.dirichlet = dir;
+    bool zeroed = false;
+    int rc = sweep_all<double>(h, w, nullptr, 1, nullptr, dir ? &zs : nullptr, &zeroed);
     if (rc) return rc;
     hipError_t e;
-    if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, w, nx, ny, nz, h->stream)) != hipSuccess)
+    if (dir && !zeroed && (e = adsg::launch_zero_boundary<double>(h->dim, w, nx, ny, nz, h->stream)) != hipSuccess)
         return hip_fail(h, e, "zero boundary");
     mark(h, 3);
     int rp = ADSG_OK;
@@ -752,9 +757,13 @@ int one_adjoint_delta_step(adsg_sim* h, const adsg_rhs_form* form, const int* sl
         return hip_fail(h, e, "zero boundary");
     HIP_TRY(h, hipMemcpyAsync(v, lam, sizeof(double) * h->N, hipMemcpyDeviceToDevice, h->stream));
     mark(h, 3);
-    int rc = sweep_all<double>(h, v, slots, 1);
+    adsg::seg_apply zs;  // the zeroing of v by the last transposed sweep as it stores its lines
+    extents(h, zs.n[0], zs.n[1], zs.n[2]);
+    zs.dirichlet = dir;
+    bool zeroed = false;
+    int rc = sweep_all<double>(h, v, slots, 1, nullptr, dir ? &zs : nullptr, &zeroed);
     if (rc) return rc;
-    if (dir && (e = adsg::launch_zero_boundary<double>(h->dim, v, nx, ny, nz, h->stream)) != hipSuccess)
+    if (dir && !zeroed && (e = adsg::launch_zero_boundary<double>(h->dim, v, nx, ny, nz, h->stream)) != hipSuccess)
         return hip_fail(h, e, "zero boundary");
     // r = R u + dt M f: dL/df += dt M v
     if ((rc = accumulate_source_gradient(h, form, v, w))) return rc;

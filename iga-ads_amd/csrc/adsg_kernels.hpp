@@ -67,8 +67,10 @@ struct rhs_fixup {
 
 // The increment update of the delta and FP32 steps, x_new = x_prev + d with the boundary
 // entries 0 when dirichlet (k_add_increment / k_apply_increment; shadow = FP32 copy of x_new
-// when set), done by the last sweep of the step as it stores its lines instead of storing d.
-// dir: the direction of that sweep; n: DOFs per direction (1 beyond the dimension).
+// when set), done by the last sweep of the step as it stores its lines instead of storing d;
+// without x_prev, the last sweep (also of a transposed solve) stores d with its boundary
+// entries 0 when dirichlet (k_zero_boundary). dir: the direction of that sweep; n: DOFs per
+// direction (1 beyond the dimension).
 struct seg_apply {
     const double* x_prev = nullptr;
     double* x_new = nullptr;
