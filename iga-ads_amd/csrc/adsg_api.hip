@@ -231,13 +231,22 @@ adsg::line_geometry geometry(const adsg_sim* h, int dir) {
 // mode: 0 = A^-1, 1 = A^-T, 2 = A^-1 with one FP64-residual refinement (FP32 data)
 template <typename T>
 int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fixup<T>* fix = nullptr,
-              const adsg::seg_apply* apply = nullptr, bool* applied = nullptr) {
+              const adsg::seg_apply* apply = nullptr, bool* applied = nullptr, const T* src = nullptr) {
     if (applied) *applied = false;
+    // src: the sweep reads its lines there and writes them to u (the segmented sweep); the
+    // other sweeps work in place on a copy
+    auto in_place = [&]() -> hipError_t {
+        if (!src) return hipSuccess;
+        const hipError_t e = hipMemcpyAsync(u, src, sizeof(T) * h->N, hipMemcpyDeviceToDevice, h->stream);
+        src = nullptr;
+        return e;
+    };
     if (sl < 0 || sl >= static_cast<int>(h->dirs[d].slots.size()) || !h->dirs[d].slots[sl].factored)
         return fail(h, ADSG_ESTATE, "ads_solve: matrix slot not set or not factored");
     const dev_slot& s = h->dirs[d].slots[sl];
     if (s.lines > 0) {
         if (mode != 0) return fail(h, ADSG_EINVAL, "ads_solve: per-line matrices support the plain solve only");
+        HIP_TRY(h, in_place());
         adsg::own_factors<T> of;
         of.lower = static_cast<const T*>(s.lower);
         of.upper = static_cast<const T*>(s.upper);
@@ -265,6 +274,7 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fi
                     "ads_solve: transposed and refined solves need unpivoted factors and lines of "
                     "2..16 segments");
     hipError_t e = hipErrorInvalidValue;
+    if (!(seg_ok && (mode != 0 || h->sweep_variant != 1))) HIP_TRY(h, in_place());
     if (seg_ok && (mode != 0 || h->sweep_variant != 1)) {
         adsg::seg_factors<T> sf;
         sf.f = f;
@@ -277,10 +287,12 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fi
         sf.transposed = mode == 1;
         sf.refine = mode == 2;
         sf.direct = h->sweep_variant == 2;  // x lines read lane by lane (small meshes)
+        sf.src = src;
         e = adsg::launch_line_sweep_seg<T, T>(u, geometry(h, d), sf, h->stream, fix, apply, applied);
         if (e == hipErrorInvalidValue && mode != 0)
             return fail(h, ADSG_EINVAL, "ads_solve: line staging does not fit the LDS for this solve");
     }
+    if (e == hipErrorInvalidValue) HIP_TRY(h, in_place());  // the fallback below works in place
     if (e == hipErrorInvalidValue && mode == 0 && fix && fix->pending()) {
         e = adsg::launch_rhs_fixup<T>(u, *fix, h->stream);  // the sweep below cannot take it
         if (e != hipSuccess) return hip_fail(h, e, "rhs fix-up");
@@ -302,7 +314,7 @@ int sweep_dir(adsg_sim* h, T* u, int d, int sl, int mode = 0, const adsg::rhs_fi
 // (*applied tells; the caller runs the update otherwise).
 template <typename T>
 int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0, const adsg::rhs_fixup<T>* fix = nullptr,
-              adsg::seg_apply* apply = nullptr, bool* applied = nullptr) {
+              adsg::seg_apply* apply = nullptr, bool* applied = nullptr, const T* src = nullptr) {
     if (applied) *applied = false;
     int special = -1;
     for (int d = 0; d < h->dim; ++d) {
@@ -321,6 +333,10 @@ int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0, const adsg::rhs
                            h->dirs[0].slots[sl].seg.nseg > 0 && h->sweep_variant != 1;
         fused = !fix->spill && start == 0 && mode == 0 && seg_x;  // zeroing only: in the x sweep
         if (!fused) {
+            if (src) {  // the fix-up works on u: the sweeps then run in place
+                HIP_TRY(h, hipMemcpyAsync(u, src, sizeof(T) * h->N, hipMemcpyDeviceToDevice, h->stream));
+                src = nullptr;
+            }
             const hipError_t e = adsg::launch_rhs_fixup<T>(u, *fix, h->stream);
             if (e != hipSuccess) return hip_fail(h, e, "rhs fix-up");
         }
@@ -331,7 +347,7 @@ int sweep_all(adsg_sim* h, T* u, const int* slots, int mode = 0, const adsg::rhs
             const bool last = i == h->dim - 1 && apply && d != 0;  // d = 0: x lines, stored through LDS
             if (last) apply->dir = d;
             int rc = sweep_dir<T>(h, u, d, slots ? slots[d] : 0, mode, i == 0 && fused ? fix : nullptr,
-                                  last ? apply : nullptr, last ? applied : nullptr);
+                                  last ? apply : nullptr, last ? applied : nullptr, i == 0 ? src : nullptr);
             if (rc) return rc;
         }
         mark(h, 4 + i);
@@ -706,13 +722,13 @@ int one_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags, const do
     double* lam = static_cast<double*>(h->vec[ADSG_U]);
     double* w = static_cast<double*>(h->vec[ADSG_BUFFER]);
     const double* un = u_prev ? u_prev : static_cast<const double*>(h->vec[ADSG_U_PREV]);
-    HIP_TRY(h, hipMemcpyAsync(w, lam, sizeof(double) * h->N, hipMemcpyDeviceToDevice, h->stream));
-    // w = Z M^-T lam: the zeroing Z by the last transposed sweep as it stores its lines
+    // w = Z M^-T lam: the first transposed sweep reads lam and writes w, the zeroing Z by the
+    // last one as it stores its lines
     adsg::seg_apply zs;
     extents(h, zs.n[0], zs.n[1], zs.n[2]);
     zs.dirichlet = dir;
     bool zeroed = false;
-    int rc = sweep_all<double>(h, w, nullptr, 1, nullptr, dir ? &zs : nullptr, &zeroed);
+    int rc = sweep_all<double>(h, w, nullptr, 1, nullptr, dir ? &zs : nullptr, &zeroed, lam);
     if (rc) return rc;
     hipError_t e;
     if (dir && !zeroed && (e = adsg::launch_zero_boundary<double>(h->dim, w, nx, ny, nz, h->stream)) != hipSuccess)

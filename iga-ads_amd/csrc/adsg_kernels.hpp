@@ -44,6 +44,7 @@ struct seg_factors {
     int m = 0, nseg = 0, mseg = 0;  // mseg: register capacity per lane (16 or 32)
     bool transposed = false, refine = false;
     bool direct = false;  // lines with S == 1 read by each lane directly, not staged through LDS
+    const F* src = nullptr;  // when set, the lines are read from here and written to u (out of place)
 };
 
 // The one-launch element loop's fix-up (adding the spill entries of the earlier tiles that
