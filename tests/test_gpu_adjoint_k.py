@@ -63,11 +63,15 @@ def _oracle_adjoint(o, traj, lam, k, dt, dirichlet):
     return dk, lam
 
 
-@pytest.mark.parametrize("dirichlet", [True, False])
-def test_adjoint_step_matches_oracle(dirichlet):
+@pytest.mark.parametrize("dirichlet,sweep", [(True, 0), (False, 0), (True, 1), (True, 2), (False, 2)])
+def test_adjoint_step_matches_oracle(dirichlet, sweep):
+    """sweep: the line-sweep variant; the transposed sweeps are segmented in all of them (the
+    first reads lambda and writes w), with x lines staged through LDS (0, 1) or read lane by
+    lane (2)."""
     p, ne, dt = 2, (40, 36), 1e-5
-    _log(f"case: explicit adjoint step 2D p=2 {ne} dirichlet={dirichlet}")
+    _log(f"case: explicit adjoint step 2D p=2 {ne} dirichlet={dirichlet} sweep variant {sweep}")
     g, o = _pair(p, ne, dirichlet)
+    g.set_tuning(ag.TUNE_SWEEP_VARIANT, sweep)
     k = _kfield(p, ne, seed=31)
     rng = np.random.default_rng(5)
     un = rng.standard_normal(g.size)
