@@ -1455,6 +1455,34 @@ int adsg_forward_checkpointed(adsg_sim* h, int nsteps, int every, const adsg_rhs
     return ADSG_OK;
 }
 
+namespace {
+
+// One segment of the checkpointed adjoint after its checkpoint is restored into u: the
+// states u(start) .. u(stop-1) saved into ck_seg while the len - 1 explicit steps recompute
+// them (one_step itself when captured, the one-step graph otherwise), lambda restored from
+// ck_seg[every], then the len adjoint steps, each reading its forward state from ck_seg.
+int adjoint_segment(adsg_sim* h, const adsg_rhs_form* form, int flags, int len, bool captured) {
+    const size_t bytes = h->N * sizeof(double);
+    for (int s = 0; s < len; ++s) {
+        HIP_TRY(h, hipMemcpyAsync(h->ck_seg + static_cast<size_t>(s) * h->N, h->vec[ADSG_U], bytes,
+                                  hipMemcpyDeviceToDevice, h->stream));
+        if (s + 1 < len) {
+            const int rc = captured ? one_step(h, form, flags) : run_graphs(h, EXPLICIT_STEP, 1, form, flags, nullptr);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U], h->ck_seg + static_cast<size_t>(h->ck_every) * h->N, bytes,
+                              hipMemcpyDeviceToDevice, h->stream));
+    for (int s = len - 1; s >= 0; --s) {
+        // the step's forward state read where the recompute left it
+        const int rc = one_adjoint_step(h, form, flags, h->ck_seg + static_cast<size_t>(s) * h->N);
+        if (rc) return rc;
+    }
+    return ADSG_OK;
+}
+
+}  // namespace
+
 int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags) {
     if (!h || !form) return fail(h, ADSG_EINVAL, "adjoint_checkpointed: bad argument");
     if (!h->ck_steps || h->ck_form != form_key(form) || h->ck_flags != flags)
@@ -1467,6 +1495,7 @@ int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags)
     const int nchk = (nsteps + every - 1) / every;
     const size_t need = static_cast<size_t>(every + 1) * h->N;  // segment states + lambda
     if (h->ck_seg_cap < need) {
+        drop_graphs(h);  // the adjoint segment graphs hold the old buffer
         if (h->ck_seg) (void)hipFree(h->ck_seg);
         h->ck_seg = nullptr;
         h->ck_seg_cap = 0;
@@ -1476,26 +1505,53 @@ int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags)
     }
     const size_t bytes = h->N * sizeof(double);
     double* lam_save = h->ck_seg + static_cast<size_t>(every) * h->N;
+    int rc;
+    if ((rc = ensure_source_load(h, form))) return rc;  // not captured
     for (int c = nchk - 1; c >= 0; --c) {
-        const int start = c * every, stop = std::min(nsteps, start + every);
+        const int start = c * every, stop = std::min(nsteps, start + every), len = stop - start;
         HIP_TRY(h, hipMemcpyAsync(lam_save, h->vec[ADSG_U], bytes, hipMemcpyDeviceToDevice, h->stream));
         // recompute the segment's states u(start) .. u(stop-1) from its checkpoint
         HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U], h->ckpt + static_cast<size_t>(c) * h->N, bytes,
                                   hipMemcpyDeviceToDevice, h->stream));
-        for (int s = start; s < stop; ++s) {
-            HIP_TRY(h, hipMemcpyAsync(h->ck_seg + static_cast<size_t>(s - start) * h->N, h->vec[ADSG_U],
-                                      bytes, hipMemcpyDeviceToDevice, h->stream));
-            if (s + 1 < stop) {
-                int rc = run_graphs(h, EXPLICIT_STEP, 1, form, flags, nullptr);
-                if (rc) return rc;
+        if (c == nchk - 1) {
+            // the first segment eagerly (its adjoint steps allocate what they need on first use)
+            if ((rc = adjoint_segment(h, form, flags, len, false))) return rc;
+            continue;
+        }
+        // the others replay one graph of the segment (recompute, copies and adjoint steps on
+        // buffers that stay fixed), captured once per starting state; its recompute swaps u
+        // and u_prev once per step, as the eager steps do
+        const std::string k = "adjoint segment|" + form_key(form) + "|" + std::to_string(every);
+        const auto key = std::make_tuple(k, flags, h->vec[ADSG_U], len);
+        auto it = h->graphs.find(key);
+        hipGraphExec_t ex = nullptr;
+        if (it != h->graphs.end()) {
+            ex = it->second;
+        } else {
+            void* u0 = h->vec[ADSG_U];
+            void* p0 = h->vec[ADSG_U_PREV];
+            void* s0 = h->shadow[0];
+            void* s1 = h->shadow[1];
+            HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            rc = adjoint_segment(h, form, flags, len, true);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(h->stream, &g);
+            h->vec[ADSG_U] = u0;  // capture only recorded the swaps
+            h->vec[ADSG_U_PREV] = p0;
+            h->shadow[0] = s0;
+            h->shadow[1] = s1;
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
             }
+            if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
+            h->graphs[key] = ex;
         }
-        HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U], lam_save, bytes, hipMemcpyDeviceToDevice, h->stream));
-        for (int s = stop - 1; s >= start; --s) {
-            // the step's forward state read where the recompute left it
-            int rc = one_adjoint_step(h, form, flags, h->ck_seg + static_cast<size_t>(s - start) * h->N);
-            if (rc) return rc;
-        }
+        HIP_TRY(h, hipGraphLaunch(ex, h->stream));
+        if ((len - 1) % 2) swap_state(h);
     }
     return ADSG_OK;
 }
