@@ -255,8 +255,10 @@ int adsg_adjoint_step(adsg_sim* h, const adsg_rhs_form* form, int flags);
  * adsg_adjoint_step backwards over the same steps -- each segment of `every` steps is
  * recomputed from its checkpoint -- and returns u = dL/du(0), adding dL/dk into
  * ADSG_GRAD_K when the form uses the k field (and dL/df into ADSG_GRAD_F when the form uses
- * a source and that vector is set). u_prev and buffer are overwritten. No
- * counterpart in the reference (it keeps no checkpoints and has no adjoint). */
+ * a source and that vector is set). u_prev and buffer are overwritten. The segments after
+ * the first are replayed from HIP graphs the handle captures and keeps (freed with the
+ * handle's other step graphs). No counterpart in the reference (it keeps no checkpoints
+ * and has no adjoint). */
 int adsg_forward_checkpointed(adsg_sim* h, int nsteps, int every, const adsg_rhs_form* form, int flags);
 int adsg_adjoint_checkpointed(adsg_sim* h, const adsg_rhs_form* form, int flags);
 
