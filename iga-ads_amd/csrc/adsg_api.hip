@@ -1461,18 +1461,20 @@ namespace {
 // states u(start) .. u(stop-1) saved into ck_seg while the len - 1 explicit steps recompute
 // them (one_step itself when captured, the one-step graph otherwise), lambda restored from
 // ck_seg[every], then the len adjoint steps, each reading its forward state from ck_seg.
+// Its copies are kernels (out = 1 * in, exact): the runtime's copies, eager or as graph
+// nodes, measured from 2 to 4 µs each and made the whole adjoint vary from box to box.
 int adjoint_segment(adsg_sim* h, const adsg_rhs_form* form, int flags, int len, bool captured) {
-    const size_t bytes = h->N * sizeof(double);
+    auto copy = [&](double* dst, const void* src) {
+        return adsg::launch_scale<double>(dst, static_cast<const double*>(src), h->N, 1.0, h->stream);
+    };
     for (int s = 0; s < len; ++s) {
-        HIP_TRY(h, hipMemcpyAsync(h->ck_seg + static_cast<size_t>(s) * h->N, h->vec[ADSG_U], bytes,
-                                  hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(h, copy(h->ck_seg + static_cast<size_t>(s) * h->N, h->vec[ADSG_U]));
         if (s + 1 < len) {
             const int rc = captured ? one_step(h, form, flags) : run_graphs(h, EXPLICIT_STEP, 1, form, flags, nullptr);
             if (rc) return rc;
         }
     }
-    HIP_TRY(h, hipMemcpyAsync(h->vec[ADSG_U], h->ck_seg + static_cast<size_t>(h->ck_every) * h->N, bytes,
-                              hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(h, copy(static_cast<double*>(h->vec[ADSG_U]), h->ck_seg + static_cast<size_t>(h->ck_every) * h->N));
     for (int s = len - 1; s >= 0; --s) {
         // the step's forward state read where the recompute left it
         const int rc = one_adjoint_step(h, form, flags, h->ck_seg + static_cast<size_t>(s) * h->N);
