@@ -1462,7 +1462,7 @@ namespace {
 // them (one_step itself when captured, the one-step graph otherwise), lambda restored from
 // ck_seg[every], then the len adjoint steps, each reading its forward state from ck_seg.
 // Its copies are kernels (out = 1 * in, exact): the runtime's copies, eager or as graph
-// nodes, measured from 2 to 4 µs each and made the whole adjoint vary from box to box.
+// nodes, measured from 2 to 4 µs each and made the whole adjoint vary from machine to machine.
 int adjoint_segment(adsg_sim* h, const adsg_rhs_form* form, int flags, int len, bool captured) {
     auto copy = [&](double* dst, const void* src) {
         return adsg::launch_scale<double>(dst, static_cast<const double*>(src), h->N, 1.0, h->stream);
