@@ -78,4 +78,4 @@ def test_host_setup_matches_oracle_tables_and_matrices():
     ip = np.zeros(s.shape[0], dtype=np.int32)
     assert adsgpu.lib().adsg_band_factorize(s.shape[0], 2, 2, mine.ctypes.data_as(adsgpu._dp),
                                             ip.ctypes.data_as(adsgpu._ip)) == 0
-    assert np.all(ip == np.arange(1, s.shape[0] + 1))  # no interchanges: the 1D mass matrices are diagonally dominant enough
+    assert np.all(ip == np.arange(1, s.shape[0] + 1))  # no interchanges: the pivot column's largest entry is on the diagonal at every step
